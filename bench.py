@@ -1,0 +1,162 @@
+"""Benchmark: Q(f) evaluations per second, hard spheres, L=8, K=10, batch 4096, on one MI355X.
+
+A step is one pass of the hot path over the whole batch resident in HBM: Q = 0; Q += Q(f) for 4096
+distributions through the mixed-precision path (symmetrized FP32 contraction, FP64 accumulation
+and conservation projection).  Its error is measured against the plain FP64 GPU operator on the
+same batch (must be <= 1e-6 normwise relative per distribution), and the FP64 path is timed too.
+
+Prints ONE JSON line (see the "metric" key).  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "collision-factorization_amd"))
+
+METRIC = "Q(f) evals/sec at HS Lmax=8 batch 4096 vs FP64 GPU path @<=1e-6 rel err"
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
+
+
+def work_model(lab, ic, K):
+    """Algorithmic FLOP per evaluation of the angular-first structure: 2 (N_G K^2 + S K^3)."""
+    n_g = lab.shape[0]
+    S = len(set(zip(np.asarray(ic).astype(int).tolist(), np.asarray(lab)[:, 0].astype(int).tolist())))
+    return 2.0 * (n_g * K * K + S * K ** 3), n_g, S
+
+
+def cpu_baseline(lab, val, ic, R, nq, K, f, budget_s):
+    """The oracle (plain-C restatement of angular_first_collision_kernel_mex) on one core."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    n = 4
+    t0 = time.perf_counter()
+    oracle.angular_first_batch(np.zeros(n * nq * K), f[:n], lab, val, ic, R, nq, K, nthreads=1)
+    per = (time.perf_counter() - t0) / n
+    m = int(max(8, min(f.shape[0], budget_s / max(per, 1e-9))))
+    t0 = time.perf_counter()
+    oracle.angular_first_batch(np.zeros(m * nq * K), f[:m], lab, val, ic, R, nq, K, nthreads=1)
+    dt = time.perf_counter() - t0
+    return {"value": m / dt, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": "%d of the %d distributions, oracle angular-first (restated reference MEX), 1 thread, %.1f s"
+                      % (m, f.shape[0], dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cfk
+    from cfk.inputs import perturbed_maxwellians
+
+    L, K_max, gamma, B = 8, 9, 1.0, args.batch
+    K = K_max + 1
+    nq = (L + 1) ** 2
+    grid = (48,) * 7   # Gauss-Legendre 32 points per dimension, padded 1.5x, on every axis
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    lab, val, ic = cfk.real_gaunt(L)
+    ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
+    t1 = time.perf_counter()
+    R = cfk.build_rtensor(K_max, L, gamma, grid, geometry=(lab, val, ic))
+    t2 = time.perf_counter()
+    plan = cfk.CollisionPlan(ls, vs, ics, R, nq, K)
+    F, n_g, S = work_model(ls, ics, K)
+
+    f_host = perturbed_maxwellians(L, K, B, seed=0)
+    f = torch.from_numpy(f_host).to(dev)
+    # batch-fastest layout [N_Q*K][B] is what a spatial solver holds; convert once, outside timing
+    fT = f.reshape(B, K * nq).t().contiguous()
+    Q = torch.zeros(K * nq, B, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step(precision):
+        Q.zero_()
+        plan.execute(Q, fT, B, precision=precision, layout="batch_fastest", stream=stream)
+
+    # FP64 reference result on the same batch
+    step("fp64")
+    torch.cuda.synchronize()
+    Q64 = Q.clone()
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            step(precision)
+        torch.cuda.synchronize()
+        plan.read_timing()
+        plan.set_timing(True)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(steps):
+            step(precision)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        plan.set_timing(False)
+        kms, nl = plan.read_timing()
+        return (te - ts) / steps, kms / max(nl, 1) / 1e3
+
+    t64, k64 = timed("fp64", max(3, args.steps // 4), 1)
+    tsym, ksym = timed("fp64_sym", max(3, args.steps // 4), 1)
+    tmix, kmix = timed("mixed", args.steps, args.warmup)
+    Qm = Q.clone()
+    err = (Qm - Q64).abs().amax(dim=0) / Q64.abs().amax(dim=0)
+    max_rel = float(err.max())
+    inv_rows = [0, 1, 2, 3, nq]   # (q=0..3, k=0) and (q=0, k=1) in the [K*N_Q] row index k*N_Q + q
+    inv_abs = float(Qm[inv_rows].abs().max())
+
+    value = B / tmix
+    achieved = F * B / kmix
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmix * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "hard spheres (gamma=1), L=8, K=10 radial modes, batch 4096 perturbed Maxwellians, "
+                               "mixed-precision angular-first contraction", "L": L, "K": K, "batch": B,
+                   "gamma": gamma, "N_G": n_g, "slices": S, "sumfac_grid": list(grid)},
+        "max_rel_err_vs_fp64": max_rel,
+        "invariants_max_abs": inv_abs,
+        "fp64_evals_per_s": B / t64,
+        "fp64_sym_evals_per_s": B / tsym,
+        "speedup_vs_fp64": t64 / tmix,
+        "kernel_ms": {"mixed": kmix * 1e3, "fp64": k64 * 1e3, "fp64_sym": ksym * 1e3},
+        "flop_per_eval": F,
+        "setup_s": {"geometry": t1 - t0, "rtensor_sumfac_gpu": t2 - t1},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "fp64_path_frac": F * B / k64 / PEAK["fp64"]},
+        "cpu_baseline": None,
+    }
+    if max_rel > 1e-6:
+        out["error"] = "mixed path exceeds 1e-6 relative error"
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ls, vs, ics, R, nq, K, f_host, args.cpu_seconds)
+    plan.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

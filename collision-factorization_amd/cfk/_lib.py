@@ -1,0 +1,97 @@
+"""ctypes binding of libcfk.so (the HIP library built from ../csrc by ``make -C collision-factorization_amd``).
+
+There is no CPU fallback anywhere in this package: if the library is missing, or no HIP device is
+visible, the compute calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcfk.so")
+
+CFK_OK = 0
+CFK_EINVAL = -1
+CFK_EHIP = -2
+CFK_ENOMEM = -3
+CFK_ENODEV = -4
+CFK_EUNSUP = -5
+
+VARIANT = {"angular_first": 0, "radial_first": 1, "naive": 2}
+PRECISION = {"fp64": 0, "mixed": 1, "fp64_sym": 2}
+LAYOUT = {"natural": 0, "batch_fastest": 1}
+
+
+class CfkError(RuntimeError):
+    """A libcfk call returned a non-zero status; ``code`` is the CFK_E* value."""
+
+    def __init__(self, code, msg):
+        super().__init__("libcfk error %d: %s" % (code, msg))
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_vp = ctypes.c_void_p
+
+# (name, argtypes) of every exported entry point of include/cfk.h
+SIGNATURES = {
+    "cfk_last_error": ([], ctypes.c_char_p),
+    "cfk_version": ([], ctypes.c_int),
+    "cfk_device_count": ([], ctypes.c_int),
+    "cfk_angular_first_collision_kernel": ([_dp, _dp, _dp, _dp, _dp, _dp, _i64, _i64, _i32, _i32], ctypes.c_int),
+    "cfk_radial_first_collision_kernel": ([_dp, _dp, _dp, _dp, _dp, _dp, _i64, _i64, _i32, _i32], ctypes.c_int),
+    "cfk_naive_collision_kernel": ([_dp, _dp, _dp, _dp, _dp, _dp, _i64, _i64, _i32, _i32], ctypes.c_int),
+    "cfk_dense_tensor_kernel": ([_dp, _dp, _dp, _i64], ctypes.c_int),
+    "cfk_compute_rtensor_sumfac": ([_dp, _i32, _i32, _i32, ctypes.c_double] + [_dp, _dp, _i32] * 7 + [_dp] * 8
+                                   + [_i32, _i32], ctypes.c_int),
+    "cfk_real_gaunt": ([_i32, ctypes.POINTER(_i64), _dp, _dp, _dp], ctypes.c_int),
+    "cfk_gauss_legendre": ([_i32, ctypes.c_double, ctypes.c_double, _dp, _dp], ctypes.c_int),
+    "cfk_gauss_gen_laguerre": ([_i32, ctypes.c_double, _dp, _dp], ctypes.c_int),
+    "cfk_sumfac_grid_sizes": ([_i32, _i32, _i32, _i32, ctypes.POINTER(_i32)], ctypes.c_int),
+    "cfk_build_rtensor": ([_dp, _i32, _i32, ctypes.c_double, ctypes.POINTER(_i32), _dp, _dp, _dp, _i64],
+                          ctypes.c_int),
+    "cfk_plan_create": ([_dp, _dp, _dp, _i64, _dp, _i64, _i32, _i32, ctypes.POINTER(_vp)], ctypes.c_int),
+    "cfk_plan_destroy": ([_vp], ctypes.c_int),
+    "cfk_plan_execute": ([_vp, _i32, _i32, _i32, _vp, _vp, _i64, _vp], ctypes.c_int),
+    "cfk_plan_set_timing": ([_vp, _i32], ctypes.c_int),
+    "cfk_plan_read_timing": ([_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)], ctypes.c_int),
+    "cfk_dense_execute": ([_vp, _vp, _vp, _i64, _i64, _vp], ctypes.c_int),
+    "cfk_conservation_project": ([_vp, _i64, _i32, _i32, _i32, _vp], ctypes.c_int),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libcfk.so once; raise if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise CfkError(CFK_ENODEV, "libcfk.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != CFK_OK:
+        raise CfkError(rc, lib().cfk_last_error().decode(errors="replace"))
+    return rc
+
+
+def dptr(a):
+    """double* of a C-contiguous float64 numpy array."""
+    assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def f64(a, order="F"):
+    """Contiguous float64 copy/view of ``a`` flattened in MATLAB column-major order."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order=order))

@@ -1,0 +1,85 @@
+"""Device-resident collision plans (batched contraction on the GPU).
+
+A ``CollisionPlan`` uploads the Gaunt routing table and the radial tensor once; ``execute``
+accumulates ``Q += Q(f)`` for a batch of distributions already resident in device memory
+(any object exposing ``data_ptr()``, e.g. torch tensors on ``cuda:0``; the library only sees raw
+device pointers).
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import LAYOUT, PRECISION, VARIANT, CfkError, _vp, check, dptr, f64, lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(int(t.data_ptr()))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return ctypes.c_void_p(int(stream.cuda_stream))
+    return ctypes.c_void_p(int(stream))
+
+
+class CollisionPlan:
+    """Routing tables + R on the current HIP device (cf. cfk_plan_create in include/cfk.h)."""
+
+    def __init__(self, labels, vals, ic_map, R, N_Q, K):
+        labels = np.asarray(labels)
+        self.N_G = labels.shape[0]
+        self.N_Q = int(N_Q)
+        self.K = int(K)
+        Rf = f64(R)
+        self.N_L = Rf.size // (self.K ** 3)
+        h = _vp()
+        check(lib().cfk_plan_create(dptr(f64(labels)), dptr(f64(vals)), dptr(f64(ic_map)), self.N_G, dptr(Rf),
+                                    Rf.size, self.N_Q, self.K, ctypes.byref(h)))
+        self._h = h
+
+    def execute(self, Q, f, B=None, variant="angular_first", precision="fp64", layout="natural", stream=None):
+        """Q += Q(f) for B distributions (device buffers of float64)."""
+        if self._h is None:
+            raise CfkError(-1, "plan destroyed")
+        n = self.N_Q * self.K
+        if B is None:
+            B = Q.numel() // n
+        if Q.numel() < B * n or f.numel() < B * n:
+            raise ValueError("Q/f too small for B=%d distributions of %d coefficients" % (B, n))
+        check(lib().cfk_plan_execute(self._h, VARIANT[variant], PRECISION[precision], LAYOUT[layout], _ptr(Q),
+                                     _ptr(f), int(B), _stream_ptr(stream)))
+        return Q
+
+    def set_timing(self, enable=True):
+        """Bracket each execute's main kernel with HIP events (see cfk_plan_set_timing)."""
+        check(lib().cfk_plan_set_timing(self._h, 1 if enable else 0))
+
+    def read_timing(self):
+        """(total kernel ms, launches) since the last read; waits for the recorded events."""
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        check(lib().cfk_plan_read_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cfk_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dense_execute(Q, f, C, N, B=1, stream=None):
+    check(lib().cfk_dense_execute(_ptr(Q), _ptr(f), _ptr(C), int(N), int(B), _stream_ptr(stream)))
+    return Q
+
+
+def conservation_project(Q, B, N_Q, K, layout="natural", stream=None):
+    check(lib().cfk_conservation_project(_ptr(Q), int(B), int(N_Q), int(K), LAYOUT[layout], _stream_ptr(stream)))
+    return Q

@@ -1,0 +1,490 @@
+// C ABI of libcfk: argument validation, plan construction, MEX-equivalent one-shot entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <array>
+#include <map>
+#include <numeric>
+#include <unordered_map>
+
+#include "common.hpp"
+#include "prep.hpp"
+
+namespace cfk {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return set_error(CFK_ENODEV, "no usable HIP device (%s)", e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+    return CFK_OK;
+}
+
+int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, const double *fT, double *QT, int64_t B,
+                       int accumulate, hipStream_t st);
+int launch_transpose(const double *in, double *out, int64_t rows, int64_t cols, bool add, hipStream_t st);
+int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t B, double *part, int nsplit,
+                 hipStream_t st);
+int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStream_t st);
+int default_kc(int K);
+int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, const double *fT, double *QT, int64_t B, int accumulate,
+                      hipStream_t st);
+int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, const float *fT, double *QT, int64_t B, int accumulate,
+                      hipStream_t st);
+int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t cols, hipStream_t st);
+int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
+
+// 1-based integral index stored as double, checked against [1, hi].
+static inline bool idx_ok(double v, int64_t hi) { return v >= 1.0 && v <= (double)hi && v == std::floor(v); }
+
+// Symmetrized tables: keep entries with q2 <= q3 (g halved on the diagonal q2 == q3), merge each
+// kept block t=(l1,l2,l3) with its mirror t'=(l1,l3,l2) into R'_t[k1,k2,k3] = R_t[k1,k2,k3] +
+// R_t'[k1,k3,k2].  Valid only if every entry's mirror (q1,q3,q2) exists with the same value (the
+// real Gaunt tensor is symmetric in its last two indices); otherwise the symmetrized paths are
+// disabled for this plan and report why.
+static int build_sym_tables(const double *labels, const double *vals, const double *ic_map, int64_t N_G,
+                            const double *R, const std::vector<int32_t> &perm, cfk_plan_s *p) {
+    const int N_Q = p->N_Q, K = p->K, n_l = p->n_l;
+    const int64_t K3 = (int64_t)K * K * K;
+    p->sym_ok = false;
+    std::vector<std::array<int, 3>> trip(n_l, {-1, -1, -1});
+    std::unordered_map<int64_t, double> gmap;
+    gmap.reserve((size_t)N_G * 2);
+    auto key = [&](int a, int b, int c) { return ((int64_t)a * N_Q + b) * N_Q + c; };
+    for (int64_t z = 0; z < N_G; ++z) {
+        const int q1 = (int)labels[z] - 1, q2 = (int)labels[z + N_G] - 1, q3 = (int)labels[z + 2 * N_G] - 1;
+        const int t = (int)ic_map[z] - 1;
+        const std::array<int, 3> lt = {l_of_q0(q1), l_of_q0(q2), l_of_q0(q3)};
+        if (trip[t][0] < 0)
+            trip[t] = lt;
+        else if (trip[t] != lt) {
+            p->sym_why = "ic_map groups entries of different (l1,l2,l3)";
+            return CFK_OK;
+        }
+        gmap[key(q1, q2, q3)] = vals[z];
+    }
+    double gmax = 0.0;
+    for (int64_t z = 0; z < N_G; ++z) gmax = std::max(gmax, std::fabs(vals[z]));
+    for (int64_t z = 0; z < N_G; ++z) {
+        const int q1 = (int)labels[z] - 1, q2 = (int)labels[z + N_G] - 1, q3 = (int)labels[z + 2 * N_G] - 1;
+        auto it = gmap.find(key(q1, q3, q2));
+        if (it == gmap.end() || std::fabs(it->second - vals[z]) > 1e-12 * gmax) {
+            p->sym_why = "Gaunt COO is not symmetric in (q2,q3)";
+            return CFK_OK;
+        }
+    }
+    std::map<std::array<int, 3>, int> tof;
+    for (int t = 0; t < n_l; ++t)
+        if (trip[t][0] >= 0) tof[trip[t]] = t;
+    std::vector<double> Rs((size_t)n_l * K3, 0.0);
+    for (int t = 0; t < n_l; ++t) {
+        if (trip[t][0] < 0 || trip[t][1] > trip[t][2]) continue;
+        auto it = tof.find({trip[t][0], trip[t][2], trip[t][1]});
+        if (it == tof.end()) {
+            p->sym_why = "mirror l-triplet missing";
+            return CFK_OK;
+        }
+        const double *Rt = R + (int64_t)t * K3, *Rm = R + (int64_t)it->second * K3;
+        for (int k3 = 0; k3 < K; ++k3)
+            for (int k2 = 0; k2 < K; ++k2)
+                for (int k1 = 0; k1 < K; ++k1)
+                    Rs[(size_t)t * K3 + k1 + k2 * K + k3 * K * K] =
+                        Rt[k1 + k2 * K + k3 * K * K] + Rm[k1 + k3 * K + k2 * K * K];
+    }
+    std::vector<int2> zq;
+    std::vector<double> zg;
+    std::vector<Slice> tq;
+    for (int64_t i = 0; i < N_G; ++i) {
+        const int32_t z = perm[i];
+        const int q2 = (int)labels[z + N_G] - 1, q3 = (int)labels[z + 2 * N_G] - 1;
+        if (q2 > q3) continue;
+        const int t = (int)ic_map[z] - 1, q1 = (int)labels[z] - 1;
+        const int32_t j = (int32_t)zq.size();
+        zq.push_back(make_int2(q2, q3));
+        zg.push_back(q2 == q3 ? 0.5 * vals[z] : vals[z]);
+        if (tq.empty() || tq.back().t != t || tq.back().q1 != q1)
+            tq.push_back(Slice{t, j, j + 1, q1});
+        else
+            tq.back().z1 = j + 1;
+    }
+    std::vector<int32_t> row_ptr(N_Q + 1, 0), order(N_Q);
+    for (const Slice &s : tq) row_ptr[s.q1 + 1]++;
+    for (int r = 0; r < N_Q; ++r) row_ptr[r + 1] += row_ptr[r];
+    std::vector<Slice> sl(tq.size());
+    {
+        std::vector<int32_t> fill(row_ptr.begin(), row_ptr.end() - 1);
+        for (const Slice &s : tq) sl[fill[s.q1]++] = s;
+    }
+    std::vector<double> work(N_Q, 0.0);
+    for (const Slice &s : tq) work[s.q1] += (double)(s.z1 - s.z0) * K * K + (double)K3;
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    p->sym_N_G = (int64_t)zq.size();
+    p->sym_S = (int64_t)tq.size();
+    int rc;
+    if ((rc = p->sym_slices.upload(sl.data(), sl.size()))) return rc;
+    if ((rc = p->sym_row_ptr.upload(row_ptr.data(), row_ptr.size()))) return rc;
+    if ((rc = p->sym_row_order.upload(order.data(), order.size()))) return rc;
+    if ((rc = p->sym_zq.upload(zq.data(), zq.size()))) return rc;
+    if ((rc = p->sym_zg.upload(zg.data(), zg.size()))) return rc;
+    if ((rc = p->sym_R.upload(Rs.data(), Rs.size()))) return rc;
+    std::vector<float> g32(zg.begin(), zg.end()), r32(Rs.begin(), Rs.end());
+    if ((rc = p->sym_zg32.upload(g32.data(), g32.size()))) return rc;
+    if ((rc = p->sym_R32.upload(r32.data(), r32.size()))) return rc;
+    p->sym_ok = true;
+    return CFK_OK;
+}
+
+// Host-side validation of a COO + R (before any device work).
+int validate_coo(const double *labels, const double *vals, const double *ic_map, int64_t N_G, const double *R,
+                 int64_t R_numel, int32_t N_Q, int32_t K) {
+    if (N_Q < 1) return set_error(CFK_EINVAL, "N_Q must be >= 1 (got %d)", N_Q);
+    if (K < 1) return set_error(CFK_EINVAL, "K must be >= 1 (got %d)", K);
+    if (K > 16) return set_error(CFK_EUNSUP, "K=%d unsupported (1..16)", K);
+    if (N_G < 0) return set_error(CFK_EINVAL, "N_G must be >= 0");
+    if (N_G > 0 && (!labels || !vals || !ic_map)) return set_error(CFK_EINVAL, "null COO pointer");
+    const int64_t K3 = (int64_t)K * K * K;
+    if (R_numel < 0 || R_numel % K3 != 0)
+        return set_error(CFK_EINVAL, "R has %lld elements, not a multiple of K^3=%lld", (long long)R_numel, (long long)K3);
+    if (R_numel > 0 && !R) return set_error(CFK_EINVAL, "null R pointer");
+    const int64_t n_l = R_numel / K3;
+    if (N_G > INT32_MAX / 2) return set_error(CFK_EINVAL, "N_G too large");
+    for (int64_t z = 0; z < N_G; ++z) {
+        for (int c = 0; c < 3; ++c)
+            if (!idx_ok(labels[z + c * N_G], N_Q))
+                return set_error(CFK_EINVAL, "labels(%lld,%d)=%g outside 1..N_Q=%d", (long long)z + 1, c + 1,
+                                 labels[z + c * N_G], N_Q);
+        if (!idx_ok(ic_map[z], n_l))
+            return set_error(CFK_EINVAL, "ic_map(%lld)=%g outside 1..N_L=%lld", (long long)z + 1, ic_map[z],
+                             (long long)n_l);
+        if (!std::isfinite(vals[z])) return set_error(CFK_EINVAL, "vals(%lld) is not finite", (long long)z + 1);
+    }
+    return CFK_OK;
+}
+
+int build_plan(const double *labels, const double *vals, const double *ic_map, int64_t N_G, const double *R,
+               int64_t R_numel, int32_t N_Q, int32_t K, cfk_plan_s *p) {
+    int rc;
+    if ((rc = validate_coo(labels, vals, ic_map, N_G, R, R_numel, N_Q, K))) return rc;
+    const int64_t K3 = (int64_t)K * K * K;
+    const int64_t n_l = R_numel / K3;
+    p->N_Q = N_Q;
+    p->K = K;
+    p->n_l = (int)n_l;
+    p->N_G = N_G;
+    // Stable sort by (t, q1) as sortrows([ic_map, q1]) (benchmark_collision_contraction_scaling.m:82-85)
+    std::vector<int32_t> perm(N_G);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
+        if (ic_map[a] != ic_map[b]) return ic_map[a] < ic_map[b];
+        return labels[a] < labels[b];
+    });
+    std::vector<int2> zq(N_G);
+    std::vector<double> zg(N_G);
+    std::vector<Slice> tq;  // slices in (t,q1) order
+    for (int64_t i = 0; i < N_G; ++i) {
+        const int32_t z = perm[i];
+        zq[i] = make_int2((int)labels[z + N_G] - 1, (int)labels[z + 2 * N_G] - 1);
+        zg[i] = vals[z];
+        const int t = (int)ic_map[z] - 1, q1 = (int)labels[z] - 1;
+        if (tq.empty() || tq.back().t != t || tq.back().q1 != q1)
+            tq.push_back(Slice{t, (int32_t)i, (int32_t)i + 1, q1});
+        else
+            tq.back().z1 = (int32_t)i + 1;
+    }
+    p->S = (int64_t)tq.size();
+    // group slices by output row q1 (stable: t ascending inside a row)
+    p->h_row_ptr.assign(N_Q + 1, 0);
+    for (const Slice &s : tq) p->h_row_ptr[s.q1 + 1]++;
+    for (int r = 0; r < N_Q; ++r) p->h_row_ptr[r + 1] += p->h_row_ptr[r];
+    p->h_slices.resize(tq.size());
+    {
+        std::vector<int32_t> fill(p->h_row_ptr.begin(), p->h_row_ptr.end() - 1);
+        for (const Slice &s : tq) p->h_slices[fill[s.q1]++] = s;
+    }
+    // schedule heavy rows first
+    std::vector<double> work(N_Q, 0.0);
+    for (const Slice &s : tq) work[s.q1] += (double)(s.z1 - s.z0) * K * K + (double)K3;
+    p->h_row_order.resize(N_Q);
+    std::iota(p->h_row_order.begin(), p->h_row_order.end(), 0);
+    std::stable_sort(p->h_row_order.begin(), p->h_row_order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    if ((rc = p->R.upload(R, (size_t)R_numel))) return rc;
+    if ((rc = p->slices.upload(p->h_slices.data(), p->h_slices.size()))) return rc;
+    if ((rc = p->row_ptr.upload(p->h_row_ptr.data(), p->h_row_ptr.size()))) return rc;
+    if ((rc = p->row_order.upload(p->h_row_order.data(), p->h_row_order.size()))) return rc;
+    if ((rc = p->zq.upload(zq.data(), zq.size()))) return rc;
+    if ((rc = p->zg.upload(zg.data(), zg.size()))) return rc;
+    return build_sym_tables(labels, vals, ic_map, N_G, R, perm, p);
+}
+
+int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *Q, const double *f, int64_t B,
+                 hipStream_t st) {
+    if (B < 0) return set_error(CFK_EINVAL, "B must be >= 0");
+    if (B == 0) return CFK_OK;
+    if (!Q || !f) return set_error(CFK_EINVAL, "null Q or f");
+    if (layout != CFK_LAYOUT_NATURAL && layout != CFK_LAYOUT_BATCH_FASTEST)
+        return set_error(CFK_EINVAL, "unknown layout %d", layout);
+    if (variant < 0 || variant > 2) return set_error(CFK_EINVAL, "unknown variant %d", variant);
+    if (precision != CFK_PREC_FP64 && precision != CFK_PREC_MIXED && precision != CFK_PREC_FP64_SYM)
+        return set_error(CFK_EINVAL, "unknown precision %d", precision);
+    int dev = -1;
+    CFK_HIP(hipGetDevice(&dev));
+    if (dev != p->device) return set_error(CFK_EINVAL, "plan lives on device %d, current device is %d", p->device, dev);
+    const int64_t NK = (int64_t)p->N_Q * p->K;
+    const bool natural = layout == CFK_LAYOUT_NATURAL;
+    int rc;
+    if (precision != CFK_PREC_FP64) {
+        if (variant != CFK_VARIANT_ANGULAR_FIRST)
+            return set_error(CFK_EUNSUP, "symmetrized/mixed precision is implemented for the angular-first variant only");
+        if (!p->sym_ok) return set_error(CFK_EUNSUP, "symmetrized tables unavailable: %s", p->sym_why.c_str());
+    }
+    if (precision == CFK_PREC_MIXED) {
+        if ((rc = p->fT32.ensure((size_t)(NK * B)))) return rc;
+        if (natural) {
+            if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
+            if ((rc = launch_transpose_f32(f, p->fT32.p, B, NK, st))) return rc;
+            KernelTimer tm(p, st);
+            if ((rc = launch_af_sym_f32(p, true, default_kc(p->K), p->fT32.p, p->QT.p, B, 0, st))) return rc;
+            tm.stop();
+            return launch_transpose(p->QT.p, Q, NK, B, true, st);
+        }
+        if ((rc = launch_to_f32(f, p->fT32.p, NK * B, st))) return rc;
+        KernelTimer tm(p, st);
+        if ((rc = launch_af_sym_f32(p, true, default_kc(p->K), p->fT32.p, Q, B, 1, st))) return rc;
+        tm.stop();
+        return CFK_OK;
+    }
+    const bool sym = precision == CFK_PREC_FP64_SYM;
+    if (!natural) {
+        KernelTimer tm(p, st);
+        rc = sym ? launch_af_sym_f64(p, false, default_kc(p->K), f, Q, B, 1, st)
+                 : launch_sparse_fp64(p, variant, default_kc(p->K), f, Q, B, 1, st);
+        if (rc) return rc;
+        tm.stop();
+        return CFK_OK;
+    }
+    if ((rc = p->fT.ensure((size_t)(NK * B)))) return rc;
+    if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
+    if ((rc = launch_transpose(f, p->fT.p, B, NK, false, st))) return rc;
+    {
+        KernelTimer tm(p, st);
+        rc = sym ? launch_af_sym_f64(p, false, default_kc(p->K), p->fT.p, p->QT.p, B, 0, st)
+                 : launch_sparse_fp64(p, variant, default_kc(p->K), p->fT.p, p->QT.p, B, 0, st);
+        if (rc) return rc;
+        tm.stop();
+    }
+    return launch_transpose(p->QT.p, Q, NK, B, true, st);
+}
+
+// One-shot host path shared by the three sparse MEX equivalents.
+int oneshot_sparse(int variant, double *Q, const double *f, const double *labels, const double *vals,
+                   const double *ic_map, const double *R, int64_t N_G, int64_t R_numel, int32_t N_Q, int32_t K) {
+    int rc;
+    if (!Q || !f) return set_error(CFK_EINVAL, "null Q or f");
+    if ((rc = validate_coo(labels, vals, ic_map, N_G, R, R_numel, N_Q, K))) return rc;
+    if ((rc = check_device())) return rc;
+    cfk_plan_s plan;
+    CFK_HIP(hipGetDevice(&plan.device));
+    if ((rc = build_plan(labels, vals, ic_map, N_G, R, R_numel, N_Q, K, &plan))) return rc;
+    const size_t NK = (size_t)N_Q * K;
+    DevBuf<double> dQ, df;
+    if ((rc = dQ.upload(Q, NK))) return rc;
+    if ((rc = df.upload(f, NK))) return rc;
+    if ((rc = plan_execute(&plan, variant, CFK_PREC_FP64, CFK_LAYOUT_NATURAL, dQ.p, df.p, 1, nullptr))) return rc;
+    CFK_HIP(hipMemcpy(Q, dQ.p, NK * sizeof(double), hipMemcpyDeviceToHost));
+    return CFK_OK;
+}
+
+}  // namespace cfk
+
+using namespace cfk;
+
+extern "C" {
+
+const char *cfk_last_error(void) { return g_err.c_str(); }
+
+int cfk_version(void) { return 1; }
+
+int cfk_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int cfk_angular_first_collision_kernel(double *Q, const double *f, const double *labels, const double *vals,
+                                       const double *ic_map, const double *R, int64_t N_G, int64_t R_numel,
+                                       int32_t N_Q, int32_t K) {
+    return oneshot_sparse(CFK_VARIANT_ANGULAR_FIRST, Q, f, labels, vals, ic_map, R, N_G, R_numel, N_Q, K);
+}
+
+int cfk_radial_first_collision_kernel(double *Q, const double *f, const double *labels, const double *vals,
+                                      const double *ic_map, const double *R, int64_t N_G, int64_t R_numel,
+                                      int32_t N_Q, int32_t K) {
+    return oneshot_sparse(CFK_VARIANT_RADIAL_FIRST, Q, f, labels, vals, ic_map, R, N_G, R_numel, N_Q, K);
+}
+
+int cfk_naive_collision_kernel(double *Q, const double *f, const double *labels, const double *vals,
+                               const double *ic_map, const double *R, int64_t N_G, int64_t R_numel, int32_t N_Q,
+                               int32_t K) {
+    return oneshot_sparse(CFK_VARIANT_NAIVE, Q, f, labels, vals, ic_map, R, N_G, R_numel, N_Q, K);
+}
+
+int cfk_dense_execute(double *Q_dev, const double *f_dev, const double *C_dev, int64_t N, int64_t B, void *stream) {
+    int rc;
+    if ((rc = check_device())) return rc;
+    if (N < 1 || B < 0) return set_error(CFK_EINVAL, "N must be >= 1 and B >= 0");
+    if (B == 0) return CFK_OK;
+    if (!Q_dev || !f_dev || !C_dev) return set_error(CFK_EINVAL, "null pointer");
+    const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(N, 256));
+    DevBuf<double> part;
+    if ((rc = part.alloc((size_t)nsplit * 4 * N))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = launch_dense(Q_dev, f_dev, C_dev, N, B, part.p, nsplit, st))) return rc;
+    CFK_HIP(hipStreamSynchronize(st));  // part is freed on return
+    return CFK_OK;
+}
+
+int cfk_dense_tensor_kernel(double *Q, const double *f, const double *C, int64_t N) {
+    int rc;
+    if (!Q || !f || !C) return set_error(CFK_EINVAL, "null pointer");
+    if (N < 1) return set_error(CFK_EINVAL, "N must be >= 1");
+    if (N > 4096) return set_error(CFK_EINVAL, "N=%lld: dense tensor of N^3 doubles is too large", (long long)N);
+    if ((rc = check_device())) return rc;
+    DevBuf<double> dQ, df, dC;
+    if ((rc = dQ.upload(Q, (size_t)N))) return rc;
+    if ((rc = df.upload(f, (size_t)N))) return rc;
+    if ((rc = dC.upload(C, (size_t)N * N * N))) return rc;
+    if ((rc = cfk_dense_execute(dQ.p, df.p, dC.p, N, 1, nullptr))) return rc;
+    CFK_HIP(hipMemcpy(Q, dQ.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+    return CFK_OK;
+}
+
+int cfk_plan_create(const double *labels, const double *vals, const double *ic_map, int64_t N_G, const double *R,
+                    int64_t R_numel, int32_t N_Q, int32_t K, cfk_plan *out) {
+    int rc;
+    if (!out) return set_error(CFK_EINVAL, "null plan output");
+    *out = nullptr;
+    if ((rc = validate_coo(labels, vals, ic_map, N_G, R, R_numel, N_Q, K))) return rc;
+    if ((rc = check_device())) return rc;
+    cfk_plan_s *p = new cfk_plan_s();
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        delete p;
+        return set_error(CFK_EHIP, "hipGetDevice failed");
+    }
+    if ((rc = build_plan(labels, vals, ic_map, N_G, R, R_numel, N_Q, K, p))) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return CFK_OK;
+}
+
+int cfk_plan_destroy(cfk_plan plan) {
+    delete plan;
+    return CFK_OK;
+}
+
+int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t layout, double *Q_dev,
+                     const double *f_dev, int64_t B, void *stream) {
+    if (!plan) return set_error(CFK_EINVAL, "null plan");
+    return plan_execute(plan, variant, precision, layout, Q_dev, f_dev, B, (hipStream_t)stream);
+}
+
+int cfk_plan_set_timing(cfk_plan plan, int32_t enable) {
+    if (!plan) return set_error(CFK_EINVAL, "null plan");
+    plan->timing = enable != 0;
+    return CFK_OK;
+}
+
+int cfk_plan_read_timing(cfk_plan plan, double *total_ms, int64_t *launches) {
+    if (!plan || !total_ms || !launches) return set_error(CFK_EINVAL, "null argument");
+    double tot = 0.0;
+    for (auto &e : plan->events) {
+        CFK_HIP(hipEventSynchronize(e.second));
+        float ms = 0.0f;
+        CFK_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int64_t)plan->events.size();
+    for (auto &e : plan->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    plan->events.clear();
+    return CFK_OK;
+}
+
+int cfk_conservation_project(double *Q_dev, int64_t B, int32_t N_Q, int32_t K, int32_t layout, void *stream) {
+    int rc;
+    if ((rc = check_device())) return rc;
+    if (!Q_dev || N_Q < 1 || K < 1 || B < 0) return set_error(CFK_EINVAL, "bad arguments");
+    if (layout != CFK_LAYOUT_NATURAL && layout != CFK_LAYOUT_BATCH_FASTEST)
+        return set_error(CFK_EINVAL, "unknown layout %d", layout);
+    return launch_conservation(Q_dev, B, N_Q, K, layout, (hipStream_t)stream);
+}
+
+int cfk_real_gaunt(int32_t L, int64_t *N_G, double *labels, double *vals, double *ic_map) {
+    if (!N_G) return set_error(CFK_EINVAL, "null N_G");
+    if (L < 0 || L > 40) return set_error(CFK_EINVAL, "L=%d outside 0..40", L);
+    RealGaunt g;
+    try {
+        g = real_gaunt(L);
+    } catch (const std::exception &e) {
+        return set_error(CFK_EINVAL, "%s", e.what());
+    }
+    const int64_t n = (int64_t)g.val.size();
+    if (!labels || !vals || !ic_map) {
+        *N_G = n;
+        return CFK_OK;
+    }
+    if (*N_G < n) return set_error(CFK_EINVAL, "output capacity %lld < N_G=%lld", (long long)*N_G, (long long)n);
+    for (int64_t z = 0; z < n; ++z) {
+        labels[z] = g.q1[z];
+        labels[z + n] = g.q2[z];
+        labels[z + 2 * n] = g.q3[z];
+        vals[z] = g.val[z];
+        ic_map[z] = g.t[z];
+    }
+    *N_G = n;
+    return CFK_OK;
+}
+
+int cfk_gauss_legendre(int32_t n, double a, double b, double *x, double *w) {
+    if (n < 1 || !x || !w) return set_error(CFK_EINVAL, "bad arguments");
+    gauss_legendre(n, a, b, x, w);
+    return CFK_OK;
+}
+
+int cfk_gauss_gen_laguerre(int32_t n, double alpha, double *x, double *w) {
+    if (n < 1 || !x || !w || !(alpha > -1.0)) return set_error(CFK_EINVAL, "bad arguments");
+    gauss_gen_laguerre(n, alpha, x, w);
+    return CFK_OK;
+}
+
+int cfk_sumfac_grid_sizes(int32_t K_max, int32_t L, int32_t radial_pad, int32_t angular_pad, int32_t sizes[7]) {
+    if (K_max < 0 || L < 0 || radial_pad < 0 || angular_pad < 0 || !sizes) return set_error(CFK_EINVAL, "bad arguments");
+    int s[7];
+    sumfac_grid_sizes(K_max, L, radial_pad, angular_pad, s);
+    for (int i = 0; i < 7; ++i) sizes[i] = s[i];
+    return CFK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// Shared definitions for the libcfk sources: status handling, device buffers, plan layout.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cfk.h"
+
+namespace cfk {
+
+int set_error(int code, const char *fmt, ...);
+int check_device();
+
+#define CFK_HIP(call)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return ::cfk::set_error(CFK_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                    __FILE__, __LINE__);                                            \
+    } while (0)
+
+// Owning device buffer (freed with hipFree).
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(CFK_ENOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+        return CFK_OK;
+    }
+    int upload(const T *host, size_t count) {
+        int rc = alloc(count);
+        if (rc) return rc;
+        if (count) CFK_HIP(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+        return CFK_OK;
+    }
+    int ensure(size_t count) { return (count <= n && p) ? CFK_OK : alloc(count); }
+};
+
+// One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.
+struct Slice {
+    int32_t t, z0, z1, q1;
+};
+
+}  // namespace cfk
+
+// Plan: geometry + R resident on the device.
+struct cfk_plan_s {
+    int device = 0;
+    int N_Q = 0, K = 0, n_l = 0;
+    int64_t N_G = 0, S = 0;
+    // host copies (for re-planning and validation)
+    std::vector<cfk::Slice> h_slices;  // ordered by (q1, t)
+    std::vector<int32_t> h_row_ptr;    // [N_Q+1] into slices
+    std::vector<int32_t> h_row_order;  // rows by descending work
+    // device tables
+    cfk::DevBuf<double> R;          // [n_l][K^3], column-major inside each block (k1 fastest)
+    cfk::DevBuf<cfk::Slice> slices;
+    cfk::DevBuf<int32_t> row_ptr, row_order;
+    cfk::DevBuf<int2> zq;           // (q2, q3) 0-based, (t,q1)-sorted
+    cfk::DevBuf<double> zg;         // Gaunt values, (t,q1)-sorted
+    // Symmetrized tables (used by CFK_PREC_FP64_SYM and CFK_PREC_MIXED): only entries with
+    // q2 <= q3 are kept (g halved when q2 == q3) and each kept block t=(l1,l2,l3), l2 <= l3, carries
+    // R'_t[k1,k2,k3] = R_t[k1,k2,k3] + R_t'[k1,k3,k2] with t' = (l1,l3,l2).  Exact because the real
+    // Gaunt tensor is symmetric in (q2,q3) and only the (q2,k2)<->(q3,k3) symmetric part of the
+    // quadratic form contributes to Q.
+    bool sym_ok = false;
+    std::string sym_why;
+    int64_t sym_N_G = 0, sym_S = 0;
+    cfk::DevBuf<cfk::Slice> sym_slices;
+    cfk::DevBuf<int32_t> sym_row_ptr, sym_row_order;
+    cfk::DevBuf<int2> sym_zq;
+    cfk::DevBuf<double> sym_zg, sym_R;
+    cfk::DevBuf<float> sym_zg32, sym_R32;
+    // scratch for the natural layout (batch-fastest staging)
+    cfk::DevBuf<double> fT, QT;
+    cfk::DevBuf<float> fT32;
+    // kernel timing (cfk_plan_set_timing)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    ~cfk_plan_s() {
+        for (auto &e : events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+};
+
+namespace cfk {
+// Brackets one kernel launch with HIP events when the plan's timing is enabled.
+struct KernelTimer {
+    cfk_plan_s *p;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(cfk_plan_s *plan, hipStream_t stream) : p(plan), st(stream) {
+        if (p->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, st);
+    }
+    void stop() {
+        if (p->timing && a && b) {
+            (void)hipEventRecord(b, st);
+            p->events.emplace_back(a, b);
+            a = b = nullptr;
+        }
+    }
+    ~KernelTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+}  // namespace cfk

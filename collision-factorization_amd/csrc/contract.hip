@@ -1,0 +1,411 @@
+// FP64 contraction kernels for gfx950 (wave64).
+//
+// Work decomposition shared by the sparse variants: one wavefront owns 64 distributions of the
+// batch (one per lane) and one output row q1.  Every lane walks the SAME routing-table entries
+// (the (t,q1) slices of that row and their Gaunt nonzeros), so the table entries, Gaunt values and
+// the radial block R_t are wave-uniform and are fetched with scalar loads into SGPRs; each lane
+// only loads its own distribution's coefficients, which sit batch-fastest ([N_Q][K][B]) so a
+// wave's load of one (q,k) coefficient is one contiguous 512-byte row.  Each row's output
+// Q[q1,:,b] is owned by exactly one lane, so no atomics are needed and results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace cfk {
+
+constexpr int WAVE = 64;
+
+#define CFK_K_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+// Device pointers of one set of routing tables.
+template <typename T>
+struct SparseTables {
+    const Slice *slices;
+    const int32_t *row_ptr;
+    const int32_t *row_order;
+    const int2 *zq;
+    const T *zg;
+    const T *R;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Angular-first (angular_first_collision_kernel_mex.cpp:24-75).  Per slice (t,q1):
+//   Phi[k2,k3] = sum_z g_z f[q2_z,k2] f[q3_z,k3]   (VALU, per lane)
+//   Q[q1,k1]  += sum_{k2,k3} R_t[k1,k2,k3] Phi[k2,k3]   (R_t from SGPRs)
+// k3 is processed in chunks of KC columns so Phi fits in registers for large K.
+// T is the arithmetic type of Phi and of the per-slice R.Phi product (double for the FP64
+// operator, float for the mixed path); the per-row sum over slices is kept in double.
+// PROJ forces the five collision invariants of the contribution to exactly zero.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int K, int KC, bool PROJ>
+__global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
+                                                  const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
+                                                  const T *__restrict__ zg, const T *__restrict__ R, int64_t B,
+                                                  int accumulate) {
+    constexpr int K2 = K * K, K3 = K2 * K;
+    const int64_t b = (int64_t)blockIdx.x * WAVE + threadIdx.x;
+    const bool active = b < B;
+    const int64_t bb = active ? b : 0;
+    const int row = row_order[blockIdx.y];
+    const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
+    const int64_t rowstride = (int64_t)K * B;
+    double q[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) q[k] = 0.0;
+    for (int s = s0; s < s1; ++s) {
+        const Slice sl = slices[s];
+        const T *Rt = R + (int64_t)sl.t * K3;
+#pragma unroll
+        for (int c0 = 0; c0 < K; c0 += KC) {
+            T phi[K][KC];
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+#pragma unroll
+                for (int j = 0; j < KC; ++j) phi[i][j] = T(0);
+            for (int z = sl.z0; z < sl.z1; ++z) {
+                const int2 qq = zq[z];
+                const T g = zg[z];
+                const T *f2p = fT + (int64_t)qq.x * rowstride + bb;
+                const T *f3p = fT + (int64_t)qq.y * rowstride + bb;
+                T gf2[K];
+#pragma unroll
+                for (int k2 = 0; k2 < K; ++k2) gf2[k2] = g * f2p[(int64_t)k2 * B];
+#pragma unroll
+                for (int j = 0; j < KC; ++j) {
+                    if (c0 + j < K) {
+                        const T f3 = f3p[(int64_t)(c0 + j) * B];
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(gf2[k2], f3, phi[k2][j]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int k1 = 0; k1 < K; ++k1) {
+                T acc = T(0);
+#pragma unroll
+                for (int j = 0; j < KC; ++j) {
+                    if (c0 + j < K) {
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) acc = fma(Rt[k1 + k2 * K + (c0 + j) * K2], phi[k2][j], acc);
+                    }
+                }
+                q[k1] += (double)acc;
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int k1 = 0; k1 < K; ++k1) {
+            double v = q[k1];
+            if (PROJ && ((row < 4 && k1 == 0) || (row == 0 && k1 == 1))) v = 0.0;
+            double *o = QT + ((int64_t)row * K + k1) * B + b;
+            *o = accumulate ? *o + v : v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Naive (naive_collision_kernel_mex.cpp:23-46): K^3 per nonzero, loop order k2 -> k3 -> k1,
+// Q[q1,k1] += g * R[k1,k2,k3,t] * (f2 f3).
+// Radial-first (radial_first_collision_kernel_mex.cpp:41-72): Psi[k1] = sum R f2 f3 (k3 outer,
+// k2 inner), then Q[q1,k1] += g * Psi[k1].
+// Both use the same row ownership as the angular-first kernel.
+// ---------------------------------------------------------------------------------------------
+template <int K, bool RADIAL_FIRST>
+__global__ __launch_bounds__(WAVE) void k3_fp64_kernel(const double *__restrict__ fT, double *__restrict__ QT,
+                                                       const Slice *__restrict__ slices,
+                                                       const int32_t *__restrict__ row_ptr,
+                                                       const int32_t *__restrict__ row_order,
+                                                       const int2 *__restrict__ zq, const double *__restrict__ zg,
+                                                       const double *__restrict__ R, int64_t B, int accumulate) {
+    constexpr int K2 = K * K, K3 = K2 * K;
+    const int64_t b = (int64_t)blockIdx.x * WAVE + threadIdx.x;
+    const bool active = b < B;
+    const int64_t bb = active ? b : 0;
+    const int row = row_order[blockIdx.y];
+    const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
+    const int64_t rowstride = (int64_t)K * B;
+    double q[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) q[k] = 0.0;
+    for (int s = s0; s < s1; ++s) {
+        const Slice sl = slices[s];
+        const double *Rt = R + (int64_t)sl.t * K3;
+        for (int z = sl.z0; z < sl.z1; ++z) {
+            const int2 qq = zq[z];
+            const double g = zg[z];
+            double f2[K], f3[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                f2[k] = fT[(int64_t)qq.x * rowstride + (int64_t)k * B + bb];
+                f3[k] = fT[(int64_t)qq.y * rowstride + (int64_t)k * B + bb];
+            }
+            if (RADIAL_FIRST) {
+#pragma unroll
+                for (int k1 = 0; k1 < K; ++k1) {
+                    double psi = 0.0;
+#pragma unroll
+                    for (int k3 = 0; k3 < K; ++k3)
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) psi = fma(Rt[k1 + k2 * K + k3 * K2] * f2[k2], f3[k3], psi);
+                    q[k1] = fma(g, psi, q[k1]);
+                }
+            } else {
+#pragma unroll
+                for (int k2 = 0; k2 < K; ++k2)
+#pragma unroll
+                    for (int k3 = 0; k3 < K; ++k3) {
+                        const double f2f3 = f2[k2] * f3[k3];
+#pragma unroll
+                        for (int k1 = 0; k1 < K; ++k1) q[k1] = fma(g * Rt[k1 + k2 * K + k3 * K2], f2f3, q[k1]);
+                    }
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int k1 = 0; k1 < K; ++k1) {
+            double *o = QT + ((int64_t)row * K + k1) * B + b;
+            *o = accumulate ? *o + q[k1] : q[k1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Layout changes between [B][NK] (natural) and [NK][B] (batch fastest), 32x32 LDS tiles.
+// ---------------------------------------------------------------------------------------------
+template <typename TO, bool ADD>
+__global__ __launch_bounds__(256) void transpose_kernel(const double *__restrict__ in, TO *__restrict__ out,
+                                                         int64_t rows, int64_t cols) {
+    // in: [rows][cols]; out: [cols][rows] (out += in^T when ADD); converts to TO on the way
+    __shared__ double tile[32][33];
+    const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int64_t r = r0 + i, c = c0 + tx;
+        tile[i][tx] = (r < rows && c < cols) ? in[r * cols + c] : 0.0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int64_t c = c0 + i, r = r0 + tx;
+        if (r < rows && c < cols) {
+            TO *o = out + c * rows + r;
+            *o = ADD ? (TO)(*o + tile[tx][i]) : (TO)tile[tx][i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void to_f32_kernel(const double *__restrict__ in, float *__restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Dense (dense_tensor_kernel_mex.cpp:14-29): Q[i] += sum_{j,k} C[i + jN + kN^2] f[j] f[k].
+// C is streamed once for up to NB distributions; k is split over gridDim.y and the per-split
+// partial sums are reduced in split order (deterministic).
+// ---------------------------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(256) void dense_partial_kernel(const double *__restrict__ C, const double *__restrict__ f,
+                                                             double *__restrict__ part, int64_t N, int nb,
+                                                             int64_t k_per_split) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t k0 = (int64_t)blockIdx.y * k_per_split;
+    const int64_t k1 = min(N, k0 + k_per_split);
+    double acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = 0.0;
+    const int64_t ii = i < N ? i : 0;
+    for (int64_t k = k0; k < k1; ++k) {
+        double fk[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) fk[b] = (b < nb) ? f[(int64_t)b * N + k] : 0.0;
+        const double *Ck = C + k * N * N + ii;
+        for (int64_t j = 0; j < N; ++j) {
+            const double c = Ck[j * N];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) acc[b] = fma(c, ((b < nb) ? f[(int64_t)b * N + j] : 0.0) * fk[b], acc[b]);
+        }
+    }
+    if (i < N) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (b < nb) part[((int64_t)blockIdx.y * nb + b) * N + i] = acc[b];
+    }
+}
+
+__global__ __launch_bounds__(256) void dense_reduce_kernel(const double *__restrict__ part, double *__restrict__ Q,
+                                                            int64_t N, int nb, int nsplit) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= N * nb) return;
+    double s = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) s += part[(int64_t)sp * nb * N + idx];
+    Q[idx] += s;
+}
+
+// Zero the five collision invariants of each distribution.
+__global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int layout) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int slots_q[5] = {0, 1, 2, 3, 0};
+    const int slots_k[5] = {0, 0, 0, 0, 1};
+    for (int s = 0; s < 5; ++s) {
+        if (slots_q[s] >= N_Q || slots_k[s] >= K) continue;
+        const int64_t c = (int64_t)slots_k[s] * N_Q + slots_q[s];
+        if (layout == CFK_LAYOUT_NATURAL)
+            Q[b * (int64_t)N_Q * K + c] = 0.0;
+        else
+            Q[c * B + b] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+template <typename T, int K, int KC>
+void launch_af(const SparseTables<T> &tb, int N_Q, bool proj, const T *fT, double *QT, int64_t B, int acc,
+               hipStream_t st) {
+    dim3 grid((unsigned)((B + WAVE - 1) / WAVE), (unsigned)N_Q);
+    // double runs the plain operator (no projection), float the mixed path (always projected)
+    constexpr bool PROJ = std::is_same<T, float>::value;
+    (void)proj;
+    hipLaunchKernelGGL((af_kernel<T, K, KC, PROJ>), grid, dim3(WAVE), 0, st, fT, QT, tb.slices, tb.row_ptr,
+                       tb.row_order, tb.zq, tb.zg, tb.R, B, acc);
+}
+
+// Default k3 chunk: keep the Phi accumulators (K*KC values) within ~100 registers' worth.
+constexpr int kc_default(int K) { return (K * K <= 100) ? K : ((K * ((K + 1) / 2) <= 100) ? (K + 1) / 2 : (K + 3) / 4); }
+
+template <typename T, int K>
+void launch_af_k(const SparseTables<T> &tb, int N_Q, bool proj, int kc, const T *fT, double *QT, int64_t B, int acc,
+                 hipStream_t st) {
+    (void)kc;
+    launch_af<T, K, kc_default(K)>(tb, N_Q, proj, fT, QT, B, acc, st);
+}
+
+template <int K>
+void launch_k3(const cfk_plan_s *p, bool rf, const double *fT, double *QT, int64_t B, int acc, hipStream_t st) {
+    dim3 grid((unsigned)((B + WAVE - 1) / WAVE), (unsigned)p->N_Q);
+    if (rf)
+        hipLaunchKernelGGL((k3_fp64_kernel<K, true>), grid, dim3(WAVE), 0, st, fT, QT, p->slices.p, p->row_ptr.p,
+                           p->row_order.p, p->zq.p, p->zg.p, p->R.p, B, acc);
+    else
+        hipLaunchKernelGGL((k3_fp64_kernel<K, false>), grid, dim3(WAVE), 0, st, fT, QT, p->slices.p, p->row_ptr.p,
+                           p->row_order.p, p->zq.p, p->zg.p, p->R.p, B, acc);
+}
+
+}  // namespace
+
+int default_kc(int K) { return kc_default(K); }
+
+int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, const double *fT, double *QT, int64_t B,
+                       int accumulate, hipStream_t st) {
+    if (p->K < 1 || p->K > 16) return set_error(CFK_EUNSUP, "K=%d unsupported (1..16)", p->K);
+    if (B <= 0) return CFK_OK;
+    if ((B + WAVE - 1) / WAVE > 0x7fffffffLL) return set_error(CFK_EINVAL, "batch too large");
+    const SparseTables<double> tb{p->slices.p, p->row_ptr.p, p->row_order.p, p->zq.p, p->zg.p, p->R.p};
+    switch (p->K) {
+#define X(KK)                                                                                  \
+    case KK:                                                                                   \
+        if (variant == CFK_VARIANT_ANGULAR_FIRST)                                              \
+            launch_af_k<double, KK>(tb, p->N_Q, false, kc, fT, QT, B, accumulate, st);         \
+        else                                                                                   \
+            launch_k3<KK>(p, variant == CFK_VARIANT_RADIAL_FIRST, fT, QT, B, accumulate, st);  \
+        break;
+        CFK_K_CASES(X)
+#undef X
+    }
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+// Angular-first over the symmetrized tables (q2 <= q3 entries only, merged R blocks).
+template <typename T>
+int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, bool proj, int kc, const T *fT, double *QT,
+                  int64_t B, int accumulate, hipStream_t st) {
+    if (p->K < 1 || p->K > 16) return set_error(CFK_EUNSUP, "K=%d unsupported (1..16)", p->K);
+    if (B <= 0) return CFK_OK;
+    switch (p->K) {
+#define X(KK)                                                                   \
+    case KK:                                                                    \
+        launch_af_k<T, KK>(tb, p->N_Q, proj, kc, fT, QT, B, accumulate, st);    \
+        break;
+        CFK_K_CASES(X)
+#undef X
+    }
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, const double *fT, double *QT, int64_t B, int accumulate,
+                      hipStream_t st) {
+    const SparseTables<double> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg.p,
+                                  p->sym_R.p};
+    return launch_af_sym<double>(p, tb, proj, kc, fT, QT, B, accumulate, st);
+}
+
+int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, const float *fT, double *QT, int64_t B, int accumulate,
+                      hipStream_t st) {
+    const SparseTables<float> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg32.p,
+                                 p->sym_R32.p};
+    return launch_af_sym<float>(p, tb, proj, kc, fT, QT, B, accumulate, st);
+}
+
+int launch_transpose(const double *in, double *out, int64_t rows, int64_t cols, bool add, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return CFK_OK;
+    dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+    if ((rows + 31) / 32 > 65535) return set_error(CFK_EINVAL, "transpose: too many rows (%lld)", (long long)rows);
+    if (add)
+        hipLaunchKernelGGL((transpose_kernel<double, true>), grid, dim3(256), 0, st, in, out, rows, cols);
+    else
+        hipLaunchKernelGGL((transpose_kernel<double, false>), grid, dim3(256), 0, st, in, out, rows, cols);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t cols, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return CFK_OK;
+    dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+    if ((rows + 31) / 32 > 65535) return set_error(CFK_EINVAL, "transpose: too many rows (%lld)", (long long)rows);
+    hipLaunchKernelGGL((transpose_kernel<float, false>), grid, dim3(256), 0, st, in, out, rows, cols);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st) {
+    if (n <= 0) return CFK_OK;
+    hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t B, double *part, int nsplit,
+                 hipStream_t st) {
+    const int64_t kps = (N + nsplit - 1) / nsplit;
+    for (int64_t b0 = 0; b0 < B; b0 += 4) {
+        const int nb = (int)std::min<int64_t>(4, B - b0);
+        dim3 grid((unsigned)((N + 255) / 256), (unsigned)nsplit);
+        hipLaunchKernelGGL(dense_partial_kernel<4>, grid, dim3(256), 0, st, C, f + b0 * N, part, N, nb, kps);
+        CFK_HIP(hipGetLastError());
+        const int64_t tot = N * nb;
+        hipLaunchKernelGGL(dense_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+                           Q + b0 * N, N, nb, nsplit);
+        CFK_HIP(hipGetLastError());
+    }
+    return CFK_OK;
+}
+
+int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStream_t st) {
+    if (B <= 0) return CFK_OK;
+    hipLaunchKernelGGL(conservation_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, Q, B, N_Q, K,
+                       layout);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+}  // namespace cfk

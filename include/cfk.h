@@ -1,0 +1,158 @@
+/*
+ * cfk.h -- C ABI of the MI355X collision-factorization library (libcfk.so, gfx950).
+ *
+ * The library implements the Wigner-Eckart-factorized spectral Boltzmann collision hot path:
+ *   - the Gaunt-COO-routed quadratic contraction Q[q1,k1] += sum A f f, and
+ *   - the 5-D Duffy sum-factorized quadrature that produces the radial tensor R.
+ *
+ * Conventions (identical to the reference MEX interface):
+ *   - one distribution f / Q is an N_Q x K column-major array: f[q + k*N_Q]
+ *     (q = l^2+l+m, real spherical harmonics; k = Laguerre degree);
+ *   - R is K x K x K x N_L column-major: R[k1 + k2*K + k3*K^2 + t*K^3];
+ *   - Gaunt COO: labels is N_G x 3 column-major (q1, q2, q3), vals N_G, ic_map N_G (t),
+ *     all indices 1-based and stored as double, exactly as MATLAB hands them to a MEX;
+ *   - the contraction entry points ACCUMULATE into Q (Q += Q(f)), as the reference MEX do.
+ *
+ * Every function returns CFK_OK (0) or a negative status; cfk_last_error() returns the
+ * message of the last failure on the calling thread.  Nothing here ever falls back to a CPU
+ * computation of the contraction or the quadrature: without a usable HIP device the compute
+ * entry points fail with CFK_ENODEV.
+ *
+ * Batched device API: a distribution batch is B consecutive N_Q x K arrays ([B][K][N_Q],
+ * CFK_LAYOUT_NATURAL) or the batch-fastest array [N_Q][K][B] (CFK_LAYOUT_BATCH_FASTEST).
+ * Device pointers must come from hipMalloc (or torch) on the current device.
+ */
+#ifndef CFK_H
+#define CFK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CFK_OK 0
+#define CFK_EINVAL (-1)   /* bad argument: size, index out of range, null pointer, unsorted input */
+#define CFK_EHIP (-2)     /* HIP runtime error */
+#define CFK_ENOMEM (-3)   /* device or host allocation failed */
+#define CFK_ENODEV (-4)   /* no usable HIP device */
+#define CFK_EUNSUP (-5)   /* combination not supported (e.g. K too large for a variant) */
+
+/* contraction variants (the four reference MEX kernels) */
+#define CFK_VARIANT_ANGULAR_FIRST 0
+#define CFK_VARIANT_RADIAL_FIRST 1
+#define CFK_VARIANT_NAIVE 2
+
+/* precision of a plan execution */
+#define CFK_PREC_FP64 0   /* plain FP64 operator (parity with the reference at 1e-12) */
+#define CFK_PREC_MIXED 1     /* FP32 symmetrized contraction, FP64 accumulation + conservation projection */
+#define CFK_PREC_FP64_SYM 2  /* FP64 over the symmetrized tables (q2<=q3 half of the routing table) */
+
+/* batch layouts */
+#define CFK_LAYOUT_NATURAL 0        /* [B][K][N_Q]  (B MATLAB N_Q x K arrays back to back) */
+#define CFK_LAYOUT_BATCH_FASTEST 1  /* [N_Q][K][B]  (cells fastest, as a spatial solver stores them) */
+
+const char *cfk_last_error(void);
+int cfk_version(void);
+int cfk_device_count(void);
+
+/* ---------------------------------------------------------------------------------------
+ * MEX-equivalent one-shot entry points: host pointers, one distribution, MATLAB layout.
+ * A MEX gateway forwards its prhs[] here unchanged.
+ * R_numel is mxGetNumberOfElements(prhs[5]) and bounds ic_map (the reference does not check).
+ * ------------------------------------------------------------------------------------- */
+
+/* replaces angular_first_collision_kernel_mex (src/mex/angular_first_collision_kernel_mex.cpp:6-76) */
+int cfk_angular_first_collision_kernel(double *Q, const double *f, const double *labels, const double *vals,
+                                       const double *ic_map, const double *R, int64_t N_G, int64_t R_numel,
+                                       int32_t N_Q, int32_t K);
+/* replaces radial_first_collision_kernel_mex (src/mex/radial_first_collision_kernel_mex.cpp:5-77) */
+int cfk_radial_first_collision_kernel(double *Q, const double *f, const double *labels, const double *vals,
+                                      const double *ic_map, const double *R, int64_t N_G, int64_t R_numel,
+                                      int32_t N_Q, int32_t K);
+/* replaces naive_collision_kernel_mex (src/mex/naive_collision_kernel_mex.cpp:4-47) */
+int cfk_naive_collision_kernel(double *Q, const double *f, const double *labels, const double *vals,
+                               const double *ic_map, const double *R, int64_t N_G, int64_t R_numel,
+                               int32_t N_Q, int32_t K);
+/* replaces dense_tensor_kernel_mex (src/mex/dense_tensor_kernel_mex.cpp:4-31): Q[i] += C[i,j,k] f[j] f[k] */
+int cfk_dense_tensor_kernel(double *Q, const double *f, const double *C, int64_t N);
+
+/* replaces compute_rtensor_sumfac_mex (src/mex/compute_rtensor_sumfac_mex.cpp:80-398).
+ * The 26 MEX inputs in MEX order, each array followed by its element count where the MEX reads
+ * one with mxGetNumberOfElements.  R_out (N_K^3*N_L doubles, N_K = K_max+1) is OVERWRITTEN with
+ * the contribution of the x nodes [x_begin, x_end) (0, N_x for the full tensor), as the MEX
+ * returns a fresh array.  No conservation zeroing (the MATLAB host applies it afterwards). */
+int cfk_compute_rtensor_sumfac(double *R_out, int32_t K_max, int32_t N_L, int32_t N_Q, double alpha,
+                               const double *x_nodes, const double *W_x, int32_t N_x,
+                               const double *u1_nodes, const double *W_u1, int32_t N_u1,
+                               const double *t1_nodes, const double *W_t1, int32_t N_t1,
+                               const double *y2_nodes, const double *W_y2, int32_t N_y2,
+                               const double *t2_nodes, const double *W_t2, int32_t N_t2,
+                               const double *mu_chi, const double *W_chi, int32_t N_chi,
+                               const double *eps_vec, const double *W_eps, int32_t N_eps,
+                               const double *RadialNorm, const double *SH_Norm, const double *L_triplets,
+                               const double *qi_valid_mat, const double *P_loss_p1, const double *P_gain_p1,
+                               const double *P_loss_p2, const double *P_gain_p2, int32_t x_begin, int32_t x_end);
+
+/* ---------------------------------------------------------------------------------------
+ * Host-side preparation (the MATLAB host code of the reference, native here).
+ * ------------------------------------------------------------------------------------- */
+
+/* Real Gaunt COO for degree L (src/GeneralCollisionTensor.m:251-281 with src/SHL/gaunt_*.m and
+ * complex2real.m).  Two-call pattern: pass NULL arrays to get *N_G; then arrays of N_G*3, N_G, N_G
+ * doubles (1-based, MATLAB find order: sorted by (q3,q2,q1)). */
+int cfk_real_gaunt(int32_t L, int64_t *N_G, double *labels, double *vals, double *ic_map);
+
+/* Gauss-Legendre on [a,b] (src/Gauss.m:38-105) and generalized Gauss-Laguerre for x^alpha e^-x
+ * (src/Gauss.m:128-173); ascending nodes. */
+int cfk_gauss_legendre(int32_t n, double a, double b, double *x, double *w);
+int cfk_gauss_gen_laguerre(int32_t n, double alpha, double *x, double *w);
+
+/* Grid sizes (N_x, N_u1, N_t1, N_y2, N_t2, N_chi, N_eps) for given paddings
+ * (src/GeneralCollisionTensor.m:55-64). */
+int cfk_sumfac_grid_sizes(int32_t K_max, int32_t L, int32_t radial_pad, int32_t angular_pad, int32_t sizes[7]);
+
+/* Build R [N_K,N_K,N_K,N_L] on the GPU end to end: P tables (src/GeneralCollisionTensor.m:106-174,
+ * on the device and compacted to the valid q_i), the sumfac quadrature, and the conservation
+ * zeroing (src/GeneralCollisionTensor.m:183-191).  grid_sizes as from cfk_sumfac_grid_sizes (or
+ * chosen explicitly); labels/vals/ic_map as returned by cfk_real_gaunt. */
+int cfk_build_rtensor(double *R_out, int32_t K_max, int32_t L, double alpha, const int32_t grid_sizes[7],
+                      const double *labels, const double *vals, const double *ic_map, int64_t N_G);
+
+/* ---------------------------------------------------------------------------------------
+ * Device-resident plans: tables uploaded once, many batched executions.
+ * ------------------------------------------------------------------------------------- */
+typedef struct cfk_plan_s *cfk_plan;
+
+/* labels/vals/ic_map: host COO (any order; it is stably sorted by (t,q1) as
+ * benchmark_collision_contraction_scaling.m:82-85 does).  R: host R tensor (R_numel doubles). */
+int cfk_plan_create(const double *labels, const double *vals, const double *ic_map, int64_t N_G,
+                    const double *R, int64_t R_numel, int32_t N_Q, int32_t K, cfk_plan *out);
+int cfk_plan_destroy(cfk_plan plan);
+
+/* Q_dev += Q(f_dev) for B distributions on `stream` (hipStream_t, NULL = default).
+ * Asynchronous.  variant: CFK_VARIANT_*; precision: CFK_PREC_*; layout: CFK_LAYOUT_*. */
+int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t layout, double *Q_dev,
+                     const double *f_dev, int64_t B, void *stream);
+
+/* Kernel timing: when enabled, every execute brackets its main contraction kernel (not the
+ * layout/precision conversions around it) with a pair of HIP events on the execution stream.
+ * cfk_plan_read_timing waits for the recorded events, returns the summed kernel time in ms and
+ * the number of launches, and clears the record. */
+int cfk_plan_set_timing(cfk_plan plan, int32_t enable);
+int cfk_plan_read_timing(cfk_plan plan, double *total_ms, int64_t *launches);
+
+/* dense variant on device: Q_dev[b] += C f_b f_b for B distributions of length N (layout natural). */
+int cfk_dense_execute(double *Q_dev, const double *f_dev, const double *C_dev, int64_t N, int64_t B, void *stream);
+
+/* Conservation projection: set the five collision invariants (mass, three momenta, energy) of each
+ * distribution to exactly zero: q=0,1,2,3 at k=0 and q=0 at k=1 (0-based), cf.
+ * tutorial_collisions_spectral_properties.m:138-144. */
+int cfk_conservation_project(double *Q_dev, int64_t B, int32_t N_Q, int32_t K, int32_t layout, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFK_H */

@@ -1,0 +1,159 @@
+"""GPU parity of the contraction path (libcfk, HIP) against the CPU oracle and the golden vectors.
+
+Bar: FP64 variants within 1e-12 normwise relative per distribution (max_b |Q-Q_ref|_inf /
+|Q_ref|_inf); the mixed path within 1e-6 with the five invariants exactly zero.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cfk
+import oracle
+from cfk.inputs import perturbed_maxwellians
+from conftest import GOLDEN, load_precalc
+from oracle import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL_MIXED = 1e-6
+
+
+def rel_err(Q, Qref, nq, K):
+    Q = np.asarray(Q).reshape(-1, nq * K)
+    Qref = np.asarray(Qref).reshape(-1, nq * K)
+    return float(np.max(np.max(np.abs(Q - Qref), axis=1) / np.max(np.abs(Qref), axis=1)))
+
+
+def test_config0_mex_entry_points_match_golden():
+    d = np.load(os.path.join(GOLDEN, "config0_maxwell_l4_k6.npz"))
+    L, K = 4, 6
+    nq = (L + 1) ** 2
+    lab, val, ic = cfk.real_gaunt(L)
+    ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
+    f = d["f"].reshape(nq, K, order="F")
+    for fn, args in ((cfk.angular_first_collision_kernel_mex, (ls, vs, ics)),
+                     (cfk.radial_first_collision_kernel_mex, (ls, vs, ics)),
+                     (cfk.naive_collision_kernel_mex, (lab, val, ic))):
+        print("config0", fn.__name__, flush=True)
+        Q = np.zeros((nq, K), order="F")
+        fn(Q, f, *args, d["R"], nq, K)
+        assert rel_err(Q.ravel(order="F"), d["Q_naive"], nq, K) <= TOL64
+        assert rel_err(Q.ravel(order="F"), d["Q_af"], nq, K) <= TOL64
+        fn(Q, f, *args, d["R"], nq, K)          # accumulates in place, as the MEX
+        assert rel_err(Q.ravel(order="F"), 2 * d["Q_naive"], nq, K) <= TOL64
+    N = nq * K
+    C = G.assemble_dense(lab.astype(np.int64), val, ic.astype(np.int64), d["R"], nq, K)
+    Qd = np.zeros(N, order="F")
+    cfk.dense_tensor_kernel_mex(Qd, d["f"], C, N)
+    assert rel_err(Qd, d["Q_naive"], nq, K) <= TOL64
+
+
+def test_config0_sumfac_on_gpu_matches_golden():
+    d = np.load(os.path.join(GOLDEN, "config0_maxwell_l4_k6.npz"))
+    R = cfk.build_rtensor(5, 4, 0.0, tuple(int(v) for v in d["sizes"]), geometry=cfk.real_gaunt(4))
+    assert np.max(np.abs(R - d["R"])) <= TOL64 * np.max(np.abs(d["R"]))
+
+
+@pytest.mark.parametrize("L,gamma", [(2, 0.0), (4, 1.0)])
+def test_sumfac_gpu_reproduces_reference_fixtures(L, gamma):
+    # the reference's own R (K_max=4, pad 20) through the native prep + GPU quadrature
+    R = cfk.build_rtensor(4, L, gamma, cfk.sumfac_grid_sizes(4, L, 20, 20), geometry=cfk.real_gaunt(L))
+    Rf = load_precalc(L, gamma)[3]
+    assert np.max(np.abs(R - Rf)) <= 1e-13 * np.max(np.abs(Rf))
+
+
+def test_sumfac_mex_entry_point_matches_oracle_on_x_ranges():
+    K_max, L, gamma = 3, 4, 1.0
+    lab, val, ic = G.real_gaunt_geometry(L)
+    grids = G.sumfac_grids(gamma, G.sumfac_grid_sizes(K_max, L, 6, 6))
+    args = G.sumfac_prep(K_max, L, gamma, lab, val, ic, grids)
+    prhs = [args[k] for k in oracle.SUMFAC_ORDER]
+    n_x = args["x"].size
+    full = cfk.compute_rtensor_sumfac_mex(*prhs)
+    ref = oracle.sumfac(args, nthreads=1)
+    assert full.shape == (K_max + 1,) * 3 + (int(args["N_L"]),)
+    assert np.max(np.abs(full.ravel(order="F") - ref)) <= TOL64 * np.max(np.abs(ref))
+    part = cfk.compute_rtensor_sumfac_mex(*prhs, x_range=(2, 5))
+    refp = oracle.sumfac(args, 2, 5, nthreads=1)
+    assert np.max(np.abs(part.ravel(order="F") - refp)) <= TOL64 * np.max(np.abs(refp))
+    assert n_x > 5
+
+
+def config1_R():
+    path = os.path.join(GOLDEN, "sumfac_hs_l8_k10_g48.npz")
+    if os.path.exists(path):
+        return np.load(path)["R"]
+    return cfk.build_rtensor(9, 8, 1.0, (48,) * 7, geometry=cfk.real_gaunt(8))
+
+
+@pytest.fixture(scope="module")
+def config1(torch_cuda):
+    L, K = 8, 10
+    nq = (L + 1) ** 2
+    lab, val, ic = cfk.real_gaunt(L)
+    ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
+    R = config1_R()
+    plan = cfk.CollisionPlan(ls, vs, ics, R, nq, K)
+    return dict(L=L, K=K, nq=nq, lab=ls, val=vs, ic=ics, R=R, plan=plan)
+
+
+@pytest.mark.parametrize("B", [1, 256])
+def test_config1_batched_parity(config1, torch_cuda, B):
+    torch = torch_cuda
+    c = config1
+    nq, K = c["nq"], c["K"]
+    f = perturbed_maxwellians(c["L"], K, B, seed=11 + B)
+    Qref = oracle.angular_first_batch(np.zeros(B * nq * K), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
+                                      nthreads=8)
+    fd = torch.from_numpy(f).cuda()
+    for variant in ("angular_first", "radial_first", "naive"):
+        print("config1 B=%d %s" % (B, variant), flush=True)
+        Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
+        c["plan"].execute(Qd, fd, B, variant=variant)
+        torch.cuda.synchronize()
+        assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= TOL64, variant
+    # symmetrized FP64 (half the routing table) is still the FP64 operator
+    Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
+    c["plan"].execute(Qd, fd, B, precision="fp64_sym")
+    torch.cuda.synchronize()
+    assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= TOL64
+    # mixed precision: error bound and exact conservation
+    Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
+    c["plan"].execute(Qd, fd, B, precision="mixed")
+    torch.cuda.synchronize()
+    Qm = Qd.cpu().numpy().reshape(B, K, nq)
+    assert rel_err(Qm, Qref, nq, K) <= TOL_MIXED
+    assert np.all(Qm[:, 0, 0:4] == 0.0) and np.all(Qm[:, 1, 0] == 0.0)
+
+
+def test_config1_layouts_and_accumulation(config1, torch_cuda):
+    torch = torch_cuda
+    c = config1
+    nq, K, B = c["nq"], c["K"], 100
+    f = perturbed_maxwellians(c["L"], K, B, seed=5)
+    fd = torch.from_numpy(f).cuda()
+    base = torch.from_numpy(np.random.default_rng(0).standard_normal(B * K * nq)).cuda()
+    Qn = base.clone()
+    c["plan"].execute(Qn, fd, B)
+    # batch-fastest layout: [N_Q*K][B]
+    fT = fd.reshape(B, K * nq).t().contiguous()
+    QT = base.reshape(B, K * nq).t().contiguous()
+    c["plan"].execute(QT, fT, B, layout="batch_fastest")
+    torch.cuda.synchronize()
+    a = Qn.cpu().numpy()
+    b = QT.t().contiguous().cpu().numpy().ravel()
+    assert np.array_equal(a, b)
+    Qref = oracle.angular_first_batch(base.cpu().numpy().copy(), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
+                                      nthreads=8)
+    assert rel_err(a, Qref, nq, K) <= TOL64
+
+
+def test_config1_sumfac_on_gpu_matches_golden():
+    path = os.path.join(GOLDEN, "sumfac_hs_l8_k10_g48.npz")
+    if not os.path.exists(path):
+        pytest.skip("golden tensor not generated yet (tests/golden/make_sumfac_golden.py)")
+    d = np.load(path)
+    R = cfk.build_rtensor(9, 8, 1.0, tuple(int(v) for v in d["sizes"]), geometry=cfk.real_gaunt(8))
+    assert np.max(np.abs(R - d["R"])) <= TOL64 * np.max(np.abs(d["R"]))

@@ -1,0 +1,57 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point include/cfk.h declares,
+and rejects bad arguments on the host before touching a device.  No compute call is made here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cfk
+from cfk._lib import SIGNATURES, CfkError, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "cfk.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(cfk_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_what_the_binding_knows():
+    assert declared_symbols() == sorted(SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(cfk.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert lib().cfk_version() >= 1
+
+
+def test_host_validation_rejects_bad_indices_without_a_device():
+    nq, K = 4, 2
+    Q = np.zeros((nq, K), order="F")
+    f = np.ones((nq, K), order="F")
+    lab = np.array([[1.0, 1.0, 1.0], [5.0, 1.0, 1.0]])   # q1 = 5 > N_Q
+    with pytest.raises(CfkError) as e:
+        cfk.angular_first_collision_kernel_mex(Q, f, lab, [0.5, 0.5], [1.0, 1.0], np.ones(K ** 3), nq, K)
+    assert e.value.code == -1 and "labels" in str(e.value)
+    lab = np.array([[1.0, 1.0, 1.0]])
+    with pytest.raises(CfkError) as e:   # ic_map beyond N_L = numel(R)/K^3
+        cfk.naive_collision_kernel_mex(Q, f, lab, [0.5], [2.0], np.ones(K ** 3), nq, K)
+    assert "ic_map" in str(e.value)
+    with pytest.raises(CfkError):        # R not a multiple of K^3
+        cfk.radial_first_collision_kernel_mex(Q, f, lab, [0.5], [1.0], np.ones(K ** 3 + 1), nq, K)
+    with pytest.raises(CfkError):        # non-integral index
+        cfk.angular_first_collision_kernel_mex(Q, f, np.array([[1.5, 1.0, 1.0]]), [0.5], [1.0], np.ones(K ** 3), nq, K)
+    with pytest.raises(ValueError):      # Q not in MATLAB layout cannot be updated in place
+        cfk.angular_first_collision_kernel_mex(np.zeros((nq, K)), f, lab, [0.5], [1.0], np.ones(K ** 3), nq, K)
+
+
+def test_sumfac_arity_error_mirrors_the_mex():
+    # compute_rtensor_sumfac_mex.cpp:82-84
+    with pytest.raises(cfk.MexError) as e:
+        cfk.compute_rtensor_sumfac_mex(*([1.0] * 25))
+    assert e.value.identifier == "R_tensor:InvalidInput"
