@@ -76,6 +76,7 @@ def main():
     R = cfk.build_rtensor(K_max, L, gamma, grid, geometry=(lab, val, ic))
     t2 = time.perf_counter()
     plan = cfk.CollisionPlan(ls, vs, ics, R, nq, K)
+    tiles = plan.autotune(B)
     F, n_g, S = work_model(ls, ics, K)
 
     f_host = perturbed_maxwellians(L, K, B, seed=0)
@@ -144,6 +145,7 @@ def main():
         "speedup_vs_fp64": t64 / tmix,
         "kernel_ms": {"mixed": kmix * 1e3, "fp64": k64 * 1e3, "fp64_sym": ksym * 1e3},
         "flop_per_eval": F,
+        "tiles_kc": tiles,
         "setup_s": {"geometry": t1 - t0, "rtensor_sumfac_gpu": t2 - t1},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp32"], "traffic": None,

@@ -56,6 +56,7 @@ SIGNATURES = {
     "cfk_plan_create": ([_dp, _dp, _dp, _i64, _dp, _i64, _i32, _i32, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_plan_destroy": ([_vp], ctypes.c_int),
     "cfk_plan_execute": ([_vp, _i32, _i32, _i32, _vp, _vp, _i64, _vp], ctypes.c_int),
+    "cfk_plan_autotune": ([_vp, _i64, _vp, ctypes.POINTER(_i32)], ctypes.c_int),
     "cfk_plan_set_timing": ([_vp, _i32], ctypes.c_int),
     "cfk_plan_read_timing": ([_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_dense_execute": ([_vp, _vp, _vp, _i64, _i64, _vp], ctypes.c_int),
@@ -65,10 +66,32 @@ SIGNATURES = {
 _LIB = None
 
 
+def _bind_hip_runtime():
+    """Make sure one HIP runtime serves both libcfk and torch.
+
+    torch ships its own libamdhip64.so.7 (same SONAME as ROCm's).  The dynamic linker binds
+    libcfk's dependency to whichever copy is already loaded, so torch (when installed) is imported
+    first and libcfk runs on torch's runtime; loading libcfk first would pull in the system copy and
+    leave torch without a usable device.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
+def hip_runtimes_loaded():
+    """Paths of the libamdhip64 copies mapped into this process (should be exactly one)."""
+    import re
+    with open("/proc/self/maps") as fh:
+        return sorted(set(re.findall(r"(/\S*libamdhip64\S*)", fh.read())))
+
+
 def lib():
     """Load libcfk.so once; raise if it has not been built."""
     global _LIB
     if _LIB is None:
+        _bind_hip_runtime()
         if not os.path.exists(LIB_PATH):
             raise CfkError(CFK_ENODEV, "libcfk.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)

@@ -52,6 +52,12 @@ class CollisionPlan:
                                      _ptr(f), int(B), _stream_ptr(stream)))
         return Q
 
+    def autotune(self, B, stream=None):
+        """Pick the fastest tile (k3 chunk KC, batch tiles per workgroup) per precision; {precision: (KC, WPB)}."""
+        t = (ctypes.c_int32 * 6)()
+        check(lib().cfk_plan_autotune(self._h, int(B), _stream_ptr(stream), t))
+        return {"fp64": (t[0], t[3]), "mixed": (t[1], t[4]), "fp64_sym": (t[2], t[5])}
+
     def set_timing(self, enable=True):
         """Bracket each execute's main kernel with HIP events (see cfk_plan_set_timing)."""
         check(lib().cfk_plan_set_timing(self._h, 1 if enable else 0))

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <array>
 #include <map>
@@ -37,16 +38,17 @@ int check_device() {
     return CFK_OK;
 }
 
-int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, const double *fT, double *QT, int64_t B,
+int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, int wpb, const double *fT, double *QT, int64_t B,
                        int accumulate, hipStream_t st);
 int launch_transpose(const double *in, double *out, int64_t rows, int64_t cols, bool add, hipStream_t st);
 int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t B, double *part, int nsplit,
                  hipStream_t st);
 int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStream_t st);
 int default_kc(int K);
-int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, const double *fT, double *QT, int64_t B, int accumulate,
+int half_kc(int K);
+int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, int wpb, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st);
-int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, const float *fT, double *QT, int64_t B, int accumulate,
+int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, int wpb, const float *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st);
 int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t cols, hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
@@ -108,21 +110,34 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
                     Rs[(size_t)t * K3 + k1 + k2 * K + k3 * K * K] =
                         Rt[k1 + k2 * K + k3 * K * K] + Rm[k1 + k3 * K + k2 * K * K];
     }
+    // kept entries in (t,q1) order, then sorted by q2 inside each (t,q1) slice; an entry that
+    // starts a new q2 group stores ~q2 (negative) so the kernel reloads f[q2,:] only then.
+    std::vector<int32_t> kept;
+    for (int64_t i = 0; i < N_G; ++i) {
+        const int32_t z = perm[i];
+        if (labels[z + N_G] <= labels[z + 2 * N_G]) kept.push_back(z);
+    }
     std::vector<int2> zq;
     std::vector<double> zg;
     std::vector<Slice> tq;
-    for (int64_t i = 0; i < N_G; ++i) {
-        const int32_t z = perm[i];
-        const int q2 = (int)labels[z + N_G] - 1, q3 = (int)labels[z + 2 * N_G] - 1;
-        if (q2 > q3) continue;
-        const int t = (int)ic_map[z] - 1, q1 = (int)labels[z] - 1;
-        const int32_t j = (int32_t)zq.size();
-        zq.push_back(make_int2(q2, q3));
-        zg.push_back(q2 == q3 ? 0.5 * vals[z] : vals[z]);
-        if (tq.empty() || tq.back().t != t || tq.back().q1 != q1)
-            tq.push_back(Slice{t, j, j + 1, q1});
-        else
-            tq.back().z1 = j + 1;
+    for (size_t i = 0; i < kept.size();) {
+        const int32_t z0 = kept[i];
+        const int t = (int)ic_map[z0] - 1, q1 = (int)labels[z0] - 1;
+        size_t e = i;
+        while (e < kept.size() && (int)ic_map[kept[e]] - 1 == t && (int)labels[kept[e]] - 1 == q1) ++e;
+        std::stable_sort(kept.begin() + i, kept.begin() + e,
+                         [&](int32_t a_, int32_t b_) { return labels[a_ + N_G] < labels[b_ + N_G]; });
+        const int32_t j0 = (int32_t)zq.size();
+        int prev_q2 = -1;
+        for (size_t r = i; r < e; ++r) {
+            const int32_t z = kept[r];
+            const int q2 = (int)labels[z + N_G] - 1, q3 = (int)labels[z + 2 * N_G] - 1;
+            zq.push_back(make_int2(q2 != prev_q2 ? ~q2 : q2, q3));
+            zg.push_back(q2 == q3 ? 0.5 * vals[z] : vals[z]);
+            prev_q2 = q2;
+        }
+        tq.push_back(Slice{t, j0, (int32_t)zq.size(), q1});
+        i = e;
     }
     std::vector<int32_t> row_ptr(N_Q + 1, 0), order(N_Q);
     for (const Slice &s : tq) row_ptr[s.q1 + 1]++;
@@ -136,6 +151,8 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
     for (const Slice &s : tq) work[s.q1] += (double)(s.z1 - s.z0) * K * K + (double)K3;
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    // device copy: last field = 1 on the slices whose partner block is l = 0 (kept with l2 <= l3, so l2 == 0)
+    for (Slice &s : sl) s.q1 = (trip[s.t][1] == 0) ? 1 : 0;
     p->sym_N_G = (int64_t)zq.size();
     p->sym_S = (int64_t)tq.size();
     int rc;
@@ -249,6 +266,8 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if (dev != p->device) return set_error(CFK_EINVAL, "plan lives on device %d, current device is %d", p->device, dev);
     const int64_t NK = (int64_t)p->N_Q * p->K;
     const bool natural = layout == CFK_LAYOUT_NATURAL;
+    const int kc = p->kc[precision] > 0 ? p->kc[precision] : default_kc(p->K);
+    const int wpb = p->wpb[precision] > 0 ? p->wpb[precision] : 1;
     int rc;
     if (precision != CFK_PREC_FP64) {
         if (variant != CFK_VARIANT_ANGULAR_FIRST)
@@ -261,21 +280,21 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
             if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
             if ((rc = launch_transpose_f32(f, p->fT32.p, B, NK, st))) return rc;
             KernelTimer tm(p, st);
-            if ((rc = launch_af_sym_f32(p, true, default_kc(p->K), p->fT32.p, p->QT.p, B, 0, st))) return rc;
+            if ((rc = launch_af_sym_f32(p, true, kc, wpb, p->fT32.p, p->QT.p, B, 0, st))) return rc;
             tm.stop();
             return launch_transpose(p->QT.p, Q, NK, B, true, st);
         }
         if ((rc = launch_to_f32(f, p->fT32.p, NK * B, st))) return rc;
         KernelTimer tm(p, st);
-        if ((rc = launch_af_sym_f32(p, true, default_kc(p->K), p->fT32.p, Q, B, 1, st))) return rc;
+        if ((rc = launch_af_sym_f32(p, true, kc, wpb, p->fT32.p, Q, B, 1, st))) return rc;
         tm.stop();
         return CFK_OK;
     }
     const bool sym = precision == CFK_PREC_FP64_SYM;
     if (!natural) {
         KernelTimer tm(p, st);
-        rc = sym ? launch_af_sym_f64(p, false, default_kc(p->K), f, Q, B, 1, st)
-                 : launch_sparse_fp64(p, variant, default_kc(p->K), f, Q, B, 1, st);
+        rc = sym ? launch_af_sym_f64(p, false, kc, wpb, f, Q, B, 1, st)
+                 : launch_sparse_fp64(p, variant, kc, wpb, f, Q, B, 1, st);
         if (rc) return rc;
         tm.stop();
         return CFK_OK;
@@ -285,8 +304,8 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if ((rc = launch_transpose(f, p->fT.p, B, NK, false, st))) return rc;
     {
         KernelTimer tm(p, st);
-        rc = sym ? launch_af_sym_f64(p, false, default_kc(p->K), p->fT.p, p->QT.p, B, 0, st)
-                 : launch_sparse_fp64(p, variant, default_kc(p->K), p->fT.p, p->QT.p, B, 0, st);
+        rc = sym ? launch_af_sym_f64(p, false, kc, wpb, p->fT.p, p->QT.p, B, 0, st)
+                 : launch_sparse_fp64(p, variant, kc, wpb, p->fT.p, p->QT.p, B, 0, st);
         if (rc) return rc;
         tm.stop();
     }
@@ -405,6 +424,63 @@ int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t 
                      const double *f_dev, int64_t B, void *stream) {
     if (!plan) return set_error(CFK_EINVAL, "null plan");
     return plan_execute(plan, variant, precision, layout, Q_dev, f_dev, B, (hipStream_t)stream);
+}
+
+int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {  // kc_out = tiles_out
+    if (!plan) return set_error(CFK_EINVAL, "null plan");
+    if (B < 1) return set_error(CFK_EINVAL, "B must be >= 1");
+    cfk_plan_s *p = plan;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = (int64_t)p->N_Q * p->K * B;
+    DevBuf<double> f, Q;
+    int rc;
+    if ((rc = f.alloc((size_t)n)) || (rc = Q.alloc((size_t)n))) return rc;
+    CFK_HIP(hipMemsetAsync(f.p, 0, (size_t)n * sizeof(double), st));
+    CFK_HIP(hipMemsetAsync(Q.p, 0, (size_t)n * sizeof(double), st));
+    const int kcs[2] = {p->K, half_kc(p->K)};
+    const int wpbs[3] = {1, 2, 4};
+    const bool was_timing = p->timing;
+    for (int prec = 0; prec < 3; ++prec) {
+        if (prec != CFK_PREC_FP64 && !p->sym_ok) continue;
+        double best = 1e300;
+        int best_kc = default_kc(p->K), best_w = 1;
+        for (int c = 0; c < 2; ++c) {
+            if (c == 1 && kcs[1] == kcs[0]) continue;
+            for (int w = 0; w < 3; ++w) {
+                p->kc[prec] = kcs[c];
+                p->wpb[prec] = wpbs[w];
+                p->timing = false;
+                if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p, B, st)))
+                    return rc;  // warm-up
+                p->timing = true;
+                for (int rep = 0; rep < 3; ++rep)
+                    if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p, B,
+                                           st)))
+                        return rc;
+                double ms = 0.0;
+                int64_t nl = 0;
+                if ((rc = cfk_plan_read_timing(p, &ms, &nl))) return rc;
+                if (std::getenv("CFK_AUTOTUNE_VERBOSE"))
+                    fprintf(stderr, "cfk autotune: precision %d KC %d WPB %d: %.4f ms\n", prec, kcs[c], wpbs[w],
+                            nl > 0 ? ms / nl : -1.0);
+                if (nl > 0 && ms / nl < best) {
+                    best = ms / nl;
+                    best_kc = kcs[c];
+                    best_w = wpbs[w];
+                }
+            }
+        }
+        p->kc[prec] = best_kc;
+        p->wpb[prec] = best_w;
+    }
+    p->timing = was_timing;
+    CFK_HIP(hipStreamSynchronize(st));
+    if (kc_out)
+        for (int i = 0; i < 3; ++i) {
+            kc_out[i] = p->kc[i] > 0 ? p->kc[i] : default_kc(p->K);
+            kc_out[3 + i] = p->wpb[i] > 0 ? p->wpb[i] : 1;
+        }
+    return CFK_OK;
 }
 
 int cfk_plan_set_timing(cfk_plan plan, int32_t enable) {

@@ -56,7 +56,9 @@ struct DevBuf {
     int ensure(size_t count) { return (count <= n && p) ? CFK_OK : alloc(count); }
 };
 
-// One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.
+// One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.  In the device copy
+// of the symmetrized tables the last field instead flags the precise slices (partner block l = 0)
+// for the mixed-precision kernel; the kernels never need q1 (it is the row being computed).
 struct Slice {
     int32_t t, z0, z1, q1;
 };
@@ -94,6 +96,9 @@ struct cfk_plan_s {
     // scratch for the natural layout (batch-fastest staging)
     cfk::DevBuf<double> fT, QT;
     cfk::DevBuf<float> fT32;
+    // tile choice per precision (index = CFK_PREC_*), set by the autotuner; 0 = default
+    int kc[3] = {0, 0, 0};
+    int wpb[3] = {0, 0, 0};  // waves (batch tiles) per workgroup
     // kernel timing (cfk_plan_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -4,8 +4,8 @@
 // batch (one per lane) and one output row q1.  Every lane walks the SAME routing-table entries
 // (the (t,q1) slices of that row and their Gaunt nonzeros), so the table entries, Gaunt values and
 // the radial block R_t are wave-uniform and are fetched with scalar loads into SGPRs; each lane
-// only loads its own distribution's coefficients, which sit batch-fastest ([N_Q][K][B]) so a
-// wave's load of one (q,k) coefficient is one contiguous 512-byte row.  Each row's output
+// only loads its own distribution's coefficients, which sit batch-fastest so a
+// wave's load of one (q,k) coefficient is one contiguous 512-byte row ([K][N_Q][B] layout).  Each row's output
 // Q[q1,:,b] is owned by exactly one lane, so no atomics are needed and results are deterministic.
 #include <hip/hip_runtime.h>
 
@@ -27,72 +27,180 @@ struct SparseTables {
     const int32_t *row_order;
     const int2 *zq;
     const T *zg;
-    const T *R;
+    const double *R;
+    const float *R32;  // FP32 copy of R (mixed path); may be null for double tables
 };
 
 // ---------------------------------------------------------------------------------------------
 // Angular-first (angular_first_collision_kernel_mex.cpp:24-75).  Per slice (t,q1):
-//   Phi[k2,k3] = sum_z g_z f[q2_z,k2] f[q3_z,k3]   (VALU, per lane)
-//   Q[q1,k1]  += sum_{k2,k3} R_t[k1,k2,k3] Phi[k2,k3]   (R_t from SGPRs)
-// k3 is processed in chunks of KC columns so Phi fits in registers for large K.
-// T is the arithmetic type of Phi and of the per-slice R.Phi product (double for the FP64
-// operator, float for the mixed path); the per-row sum over slices is kept in double.
+//   Phi[k2,k3] = sum_z g_z f[q2_z,k2] f[q3_z,k3]            (type T: double or float, per lane)
+//   Q[q1,k1]  += sum_{k2,k3} R_t[k1,k2,k3] Phi[k2,k3]         (always FP64)
+// The R.Phi stage streams one contiguous row R_t[0:K,k2,k3] (wave-uniform, scalar loads) per
+// (k2,k3) and FMAs it into the K double accumulators of the output row, so only K values of R are
+// live at a time.  k3 is processed in chunks of KC columns so Phi fits in registers.
+//
+// GROUPED tables (symmetrized path) store the entries of a slice sorted by q2; an entry whose q2
+// is stored bit-inverted (negative) starts a new q2 group.  Per group the kernel accumulates
+//   h[k3] = sum_{entries of the group} g f[q3,k3]   and then   Phi += f[q2,:] (x) h,
+// one outer product per distinct (t,q1,q2) instead of one per nonzero.
 // PROJ forces the five collision invariants of the contribution to exactly zero.
+//
+// Mixed precision (T = float): Phi is built in FP32; the R.Phi stage runs in FP32 with an FP32
+// per-slice sum (R32, one FP32 row per (k2,k3)) except on slices flagged precise (Slice.q1 != 0
+// in the symmetrized device tables: the blocks whose partner is the l=0 block, which carry the
+// large, cancelling Maxwellian-times-perturbation terms), where it runs in FP64 as for T = double.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int K, int KC, bool PROJ>
-__global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
-                                                  const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
-                                                  const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
-                                                  const T *__restrict__ zg, const T *__restrict__ R, int64_t B,
-                                                  int accumulate) {
+template <typename T, int K, int KC, bool GROUPED, bool PROJ, int WPB>
+__global__ __launch_bounds__(WAVE *WPB) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
+                                                       const Slice *__restrict__ slices,
+                                                       const int32_t *__restrict__ row_ptr,
+                                                       const int32_t *__restrict__ row_order,
+                                                       const int2 *__restrict__ zq, const T *__restrict__ zg,
+                                                       const double *__restrict__ R, const float *__restrict__ R32,
+                                                       int64_t B, int N_Q, int accumulate) {
     constexpr int K2 = K * K, K3 = K2 * K;
-    const int64_t b = (int64_t)blockIdx.x * WAVE + threadIdx.x;
+    // WPB waves of one workgroup take WPB consecutive batch tiles of the SAME output row, so they
+    // walk the same slices in step and share R_t and the table entries in the scalar cache.
+    const int64_t b = ((int64_t)blockIdx.x * WPB + (threadIdx.x / WAVE)) * WAVE + (threadIdx.x % WAVE);
     const bool active = b < B;
     const int64_t bb = active ? b : 0;
     const int row = row_order[blockIdx.y];
     const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
-    const int64_t rowstride = (int64_t)K * B;
+    const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
     double q[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) q[k] = 0.0;
     for (int s = s0; s < s1; ++s) {
         const Slice sl = slices[s];
-        const T *Rt = R + (int64_t)sl.t * K3;
-#pragma unroll
+        const double *Rt = R + (int64_t)sl.t * K3;
+        const float *Rt32 = R32 + (int64_t)sl.t * K3;
+        const bool precise = !std::is_same<T, float>::value || sl.q1 != 0;
+#pragma unroll 1
         for (int c0 = 0; c0 < K; c0 += KC) {
             T phi[K][KC];
 #pragma unroll
             for (int i = 0; i < K; ++i)
 #pragma unroll
                 for (int j = 0; j < KC; ++j) phi[i][j] = T(0);
-            for (int z = sl.z0; z < sl.z1; ++z) {
-                const int2 qq = zq[z];
-                const T g = zg[z];
-                const T *f2p = fT + (int64_t)qq.x * rowstride + bb;
-                const T *f3p = fT + (int64_t)qq.y * rowstride + bb;
-                T gf2[K];
+            if (GROUPED) {
+                // one-entry-ahead prefetch of the table entry and of its f rows
+                T f2[K], h[KC], f2n[K], f3n[KC];
 #pragma unroll
-                for (int k2 = 0; k2 < K; ++k2) gf2[k2] = g * f2p[(int64_t)k2 * B];
+                for (int j = 0; j < KC; ++j) h[j] = T(0);
 #pragma unroll
-                for (int j = 0; j < KC; ++j) {
-                    if (c0 + j < K) {
-                        const T f3 = f3p[(int64_t)(c0 + j) * B];
+                for (int k = 0; k < K; ++k) f2[k] = T(0);
+                int2 qn = zq[sl.z0];
+                T gn = zg[sl.z0];
+                {
+                    const T *f3p = fT + (int64_t)qn.y * B + bb;
 #pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(gf2[k2], f3, phi[k2][j]);
+                    for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                    const T *f2p = fT + (int64_t)(~qn.x) * B + bb;   // the first entry always starts a group
+#pragma unroll
+                    for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
+                }
+                for (int z = sl.z0; z < sl.z1; ++z) {
+                    const int2 qq = qn;
+                    const T g = gn;
+                    T f3[KC];
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) f3[j] = f3n[j];
+                    if (qq.x < 0) {  // new q2 group: flush the previous one, take the prefetched row
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                            for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(f2[k2], h[j], phi[k2][j]);
+                            h[j] = T(0);
+                        }
+#pragma unroll
+                        for (int k = 0; k < K; ++k) f2[k] = f2n[k];
+                    }
+                    if (z + 1 < sl.z1) {
+                        qn = zq[z + 1];
+                        gn = zg[z + 1];
+                        const T *f3p = fT + (int64_t)qn.y * B + bb;
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                        if (qn.x < 0) {
+                            const T *f2p = fT + (int64_t)(~qn.x) * B + bb;
+#pragma unroll
+                            for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) h[j] = fma(g, f3[j], h[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < KC; ++j)
+#pragma unroll
+                    for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(f2[k2], h[j], phi[k2][j]);
+            } else {
+                int2 qn = zq[sl.z0];
+                T gn = zg[sl.z0];
+                T f2n[K], f3n[KC];
+                {
+                    const T *f2p = fT + (int64_t)qn.x * B + bb;
+                    const T *f3p = fT + (int64_t)qn.y * B + bb;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                }
+                for (int z = sl.z0; z < sl.z1; ++z) {
+                    const T g = gn;
+                    T gf2[K], f3[KC];
+#pragma unroll
+                    for (int k2 = 0; k2 < K; ++k2) gf2[k2] = g * f2n[k2];
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) f3[j] = f3n[j];
+                    if (z + 1 < sl.z1) {
+                        qn = zq[z + 1];
+                        gn = zg[z + 1];
+                        const T *f2p = fT + (int64_t)qn.x * B + bb;
+                        const T *f3p = fT + (int64_t)qn.y * B + bb;
+#pragma unroll
+                        for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                    }
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(gf2[k2], f3[j], phi[k2][j]);
                     }
                 }
             }
-#pragma unroll
-            for (int k1 = 0; k1 < K; ++k1) {
-                T acc = T(0);
+            if (precise) {
 #pragma unroll
                 for (int j = 0; j < KC; ++j) {
                     if (c0 + j < K) {
 #pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) acc = fma(Rt[k1 + k2 * K + (c0 + j) * K2], phi[k2][j], acc);
+                        for (int k2 = 0; k2 < K; ++k2) {
+                            const double p = (double)phi[k2][j];
+                            const double *r = Rt + k2 * K + (c0 + j) * K2;
+#pragma unroll
+                            for (int k1 = 0; k1 < K; ++k1) q[k1] = fma(r[k1], p, q[k1]);
+                        }
                     }
                 }
-                q[k1] += (double)acc;
+            } else {
+                float a[K];
+#pragma unroll
+                for (int k1 = 0; k1 < K; ++k1) a[k1] = 0.0f;
+#pragma unroll
+                for (int j = 0; j < KC; ++j) {
+                    if (c0 + j < K) {
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) {
+                            const float p = (float)phi[k2][j];
+                            const float *r = Rt32 + k2 * K + (c0 + j) * K2;
+#pragma unroll
+                            for (int k1 = 0; k1 < K; ++k1) a[k1] = fmaf(r[k1], p, a[k1]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k1 = 0; k1 < K; ++k1) q[k1] += (double)a[k1];
             }
         }
     }
@@ -101,7 +209,7 @@ __global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, doub
         for (int k1 = 0; k1 < K; ++k1) {
             double v = q[k1];
             if (PROJ && ((row < 4 && k1 == 0) || (row == 0 && k1 == 1))) v = 0.0;
-            double *o = QT + ((int64_t)row * K + k1) * B + b;
+            double *o = QT + ((int64_t)k1 * N_Q + row) * B + b;
             *o = accumulate ? *o + v : v;
         }
     }
@@ -120,14 +228,15 @@ __global__ __launch_bounds__(WAVE) void k3_fp64_kernel(const double *__restrict_
                                                        const int32_t *__restrict__ row_ptr,
                                                        const int32_t *__restrict__ row_order,
                                                        const int2 *__restrict__ zq, const double *__restrict__ zg,
-                                                       const double *__restrict__ R, int64_t B, int accumulate) {
+                                                       const double *__restrict__ R, int64_t B, int N_Q,
+                                                       int accumulate) {
     constexpr int K2 = K * K, K3 = K2 * K;
     const int64_t b = (int64_t)blockIdx.x * WAVE + threadIdx.x;
     const bool active = b < B;
     const int64_t bb = active ? b : 0;
     const int row = row_order[blockIdx.y];
     const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
-    const int64_t rowstride = (int64_t)K * B;
+    const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
     double q[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) q[k] = 0.0;
@@ -140,8 +249,8 @@ __global__ __launch_bounds__(WAVE) void k3_fp64_kernel(const double *__restrict_
             double f2[K], f3[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                f2[k] = fT[(int64_t)qq.x * rowstride + (int64_t)k * B + bb];
-                f3[k] = fT[(int64_t)qq.y * rowstride + (int64_t)k * B + bb];
+                f2[k] = fT[(int64_t)qq.x * B + k * kstride + bb];
+                f3[k] = fT[(int64_t)qq.y * B + k * kstride + bb];
             }
             if (RADIAL_FIRST) {
 #pragma unroll
@@ -168,7 +277,7 @@ __global__ __launch_bounds__(WAVE) void k3_fp64_kernel(const double *__restrict_
     if (active) {
 #pragma unroll
         for (int k1 = 0; k1 < K; ++k1) {
-            double *o = QT + ((int64_t)row * K + k1) * B + b;
+            double *o = QT + ((int64_t)k1 * N_Q + row) * B + b;
             *o = accumulate ? *o + q[k1] : q[k1];
         }
     }
@@ -267,25 +376,38 @@ __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int la
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-template <typename T, int K, int KC>
-void launch_af(const SparseTables<T> &tb, int N_Q, bool proj, const T *fT, double *QT, int64_t B, int acc,
-               hipStream_t st) {
-    dim3 grid((unsigned)((B + WAVE - 1) / WAVE), (unsigned)N_Q);
+template <typename T, int K, int KC, bool GROUPED, int WPB>
+void launch_af_w(const SparseTables<T> &tb, int N_Q, const T *fT, double *QT, int64_t B, int acc, hipStream_t st) {
+    dim3 grid((unsigned)((B + WAVE * WPB - 1) / (WAVE * WPB)), (unsigned)N_Q);
     // double runs the plain operator (no projection), float the mixed path (always projected)
     constexpr bool PROJ = std::is_same<T, float>::value;
-    (void)proj;
-    hipLaunchKernelGGL((af_kernel<T, K, KC, PROJ>), grid, dim3(WAVE), 0, st, fT, QT, tb.slices, tb.row_ptr,
-                       tb.row_order, tb.zq, tb.zg, tb.R, B, acc);
+    hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, WPB>), grid, dim3(WAVE * WPB), 0, st, fT, QT, tb.slices,
+                       tb.row_ptr, tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc);
 }
 
-// Default k3 chunk: keep the Phi accumulators (K*KC values) within ~100 registers' worth.
-constexpr int kc_default(int K) { return (K * K <= 100) ? K : ((K * ((K + 1) / 2) <= 100) ? (K + 1) / 2 : (K + 3) / 4); }
+template <typename T, int K, int KC, bool GROUPED>
+void launch_af(const SparseTables<T> &tb, int N_Q, int wpb, const T *fT, double *QT, int64_t B, int acc,
+               hipStream_t st) {
+    if (wpb >= 4)
+        launch_af_w<T, K, KC, GROUPED, 4>(tb, N_Q, fT, QT, B, acc, st);
+    else if (wpb >= 2)
+        launch_af_w<T, K, KC, GROUPED, 2>(tb, N_Q, fT, QT, B, acc, st);
+    else
+        launch_af_w<T, K, KC, GROUPED, 1>(tb, N_Q, fT, QT, B, acc, st);
+}
 
-template <typename T, int K>
-void launch_af_k(const SparseTables<T> &tb, int N_Q, bool proj, int kc, const T *fT, double *QT, int64_t B, int acc,
+// k3 chunk candidates: the full width and half of it.
+constexpr int kc_half(int K) { return (K + 1) / 2; }
+// Default chunk: full width while the Phi accumulators stay within ~100 values.
+constexpr int kc_default(int K) { return (K * K <= 100) ? K : kc_half(K); }
+
+template <typename T, int K, bool GROUPED>
+void launch_af_k(const SparseTables<T> &tb, int N_Q, int kc, int wpb, const T *fT, double *QT, int64_t B, int acc,
                  hipStream_t st) {
-    (void)kc;
-    launch_af<T, K, kc_default(K)>(tb, N_Q, proj, fT, QT, B, acc, st);
+    if (kc >= K)
+        launch_af<T, K, K, GROUPED>(tb, N_Q, wpb, fT, QT, B, acc, st);
+    else
+        launch_af<T, K, kc_half(K), GROUPED>(tb, N_Q, wpb, fT, QT, B, acc, st);
 }
 
 template <int K>
@@ -293,27 +415,28 @@ void launch_k3(const cfk_plan_s *p, bool rf, const double *fT, double *QT, int64
     dim3 grid((unsigned)((B + WAVE - 1) / WAVE), (unsigned)p->N_Q);
     if (rf)
         hipLaunchKernelGGL((k3_fp64_kernel<K, true>), grid, dim3(WAVE), 0, st, fT, QT, p->slices.p, p->row_ptr.p,
-                           p->row_order.p, p->zq.p, p->zg.p, p->R.p, B, acc);
+                           p->row_order.p, p->zq.p, p->zg.p, p->R.p, B, p->N_Q, acc);
     else
         hipLaunchKernelGGL((k3_fp64_kernel<K, false>), grid, dim3(WAVE), 0, st, fT, QT, p->slices.p, p->row_ptr.p,
-                           p->row_order.p, p->zq.p, p->zg.p, p->R.p, B, acc);
+                           p->row_order.p, p->zq.p, p->zg.p, p->R.p, B, p->N_Q, acc);
 }
 
 }  // namespace
 
 int default_kc(int K) { return kc_default(K); }
+int half_kc(int K) { return kc_half(K); }
 
-int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, const double *fT, double *QT, int64_t B,
+int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, int wpb, const double *fT, double *QT, int64_t B,
                        int accumulate, hipStream_t st) {
     if (p->K < 1 || p->K > 16) return set_error(CFK_EUNSUP, "K=%d unsupported (1..16)", p->K);
     if (B <= 0) return CFK_OK;
     if ((B + WAVE - 1) / WAVE > 0x7fffffffLL) return set_error(CFK_EINVAL, "batch too large");
-    const SparseTables<double> tb{p->slices.p, p->row_ptr.p, p->row_order.p, p->zq.p, p->zg.p, p->R.p};
+    const SparseTables<double> tb{p->slices.p, p->row_ptr.p, p->row_order.p, p->zq.p, p->zg.p, p->R.p, nullptr};
     switch (p->K) {
 #define X(KK)                                                                                  \
     case KK:                                                                                   \
         if (variant == CFK_VARIANT_ANGULAR_FIRST)                                              \
-            launch_af_k<double, KK>(tb, p->N_Q, false, kc, fT, QT, B, accumulate, st);         \
+            launch_af_k<double, KK, false>(tb, p->N_Q, kc, wpb, fT, QT, B, accumulate, st);         \
         else                                                                                   \
             launch_k3<KK>(p, variant == CFK_VARIANT_RADIAL_FIRST, fT, QT, B, accumulate, st);  \
         break;
@@ -326,14 +449,14 @@ int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, const double *f
 
 // Angular-first over the symmetrized tables (q2 <= q3 entries only, merged R blocks).
 template <typename T>
-int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, bool proj, int kc, const T *fT, double *QT,
+int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, bool proj, int kc, int wpb, const T *fT, double *QT,
                   int64_t B, int accumulate, hipStream_t st) {
     if (p->K < 1 || p->K > 16) return set_error(CFK_EUNSUP, "K=%d unsupported (1..16)", p->K);
     if (B <= 0) return CFK_OK;
     switch (p->K) {
 #define X(KK)                                                                   \
     case KK:                                                                    \
-        launch_af_k<T, KK>(tb, p->N_Q, proj, kc, fT, QT, B, accumulate, st);    \
+        launch_af_k<T, KK, true>(tb, p->N_Q, kc, wpb, fT, QT, B, accumulate, st);    \
         break;
         CFK_K_CASES(X)
 #undef X
@@ -342,18 +465,18 @@ int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, bool proj, int
     return CFK_OK;
 }
 
-int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, const double *fT, double *QT, int64_t B, int accumulate,
+int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, int wpb, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st) {
     const SparseTables<double> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg.p,
-                                  p->sym_R.p};
-    return launch_af_sym<double>(p, tb, proj, kc, fT, QT, B, accumulate, st);
+                                  p->sym_R.p, nullptr};
+    return launch_af_sym<double>(p, tb, proj, kc, wpb, fT, QT, B, accumulate, st);
 }
 
-int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, const float *fT, double *QT, int64_t B, int accumulate,
+int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, int wpb, const float *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st) {
     const SparseTables<float> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg32.p,
-                                 p->sym_R32.p};
-    return launch_af_sym<float>(p, tb, proj, kc, fT, QT, B, accumulate, st);
+                                 p->sym_R.p, p->sym_R32.p};
+    return launch_af_sym<float>(p, tb, proj, kc, wpb, fT, QT, B, accumulate, st);
 }
 
 int launch_transpose(const double *in, double *out, int64_t rows, int64_t cols, bool add, hipStream_t st) {

@@ -19,7 +19,7 @@
  * entry points fail with CFK_ENODEV.
  *
  * Batched device API: a distribution batch is B consecutive N_Q x K arrays ([B][K][N_Q],
- * CFK_LAYOUT_NATURAL) or the batch-fastest array [N_Q][K][B] (CFK_LAYOUT_BATCH_FASTEST).
+ * CFK_LAYOUT_NATURAL) or the batch-fastest array [K][N_Q][B] (CFK_LAYOUT_BATCH_FASTEST).
  * Device pointers must come from hipMalloc (or torch) on the current device.
  */
 #ifndef CFK_H
@@ -51,7 +51,7 @@ extern "C" {
 
 /* batch layouts */
 #define CFK_LAYOUT_NATURAL 0        /* [B][K][N_Q]  (B MATLAB N_Q x K arrays back to back) */
-#define CFK_LAYOUT_BATCH_FASTEST 1  /* [N_Q][K][B]  (cells fastest, as a spatial solver stores them) */
+#define CFK_LAYOUT_BATCH_FASTEST 1  /* [K][N_Q][B]  (transpose of natural: cells fastest, as a spatial solver stores them) */
 
 const char *cfk_last_error(void);
 int cfk_version(void);
@@ -135,6 +135,13 @@ int cfk_plan_destroy(cfk_plan plan);
  * Asynchronous.  variant: CFK_VARIANT_*; precision: CFK_PREC_*; layout: CFK_LAYOUT_*. */
 int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t layout, double *Q_dev,
                      const double *f_dev, int64_t B, void *stream);
+
+/* Autotuner: times every tile variant of the angular-first kernels -- k3 chunk width KC in
+ * {K, ceil(K/2)} x batch tiles per workgroup in {1,2,4} -- for each precision on a synthetic batch
+ * of B distributions (batch-fastest layout) on `stream` and keeps the fastest per precision in the
+ * plan.  When tiles_out is not NULL it receives 6 ints: KC for (FP64, MIXED, FP64_SYM), then the
+ * tiles per workgroup for the same three.  Allocates temporary device buffers; synchronizes. */
+int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *tiles_out);
 
 /* Kernel timing: when enabled, every execute brackets its main contraction kernel (not the
  * layout/precision conversions around it) with a pair of HIP events on the execution stream.

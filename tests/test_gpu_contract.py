@@ -157,3 +157,21 @@ def test_config1_sumfac_on_gpu_matches_golden():
     d = np.load(path)
     R = cfk.build_rtensor(9, 8, 1.0, tuple(int(v) for v in d["sizes"]), geometry=cfk.real_gaunt(8))
     assert np.max(np.abs(R - d["R"])) <= TOL64 * np.max(np.abs(d["R"]))
+
+
+def test_autotuned_tiles_keep_parity(config1, torch_cuda):
+    torch = torch_cuda
+    c = config1
+    nq, K, B = c["nq"], c["K"], 192
+    tiles = c["plan"].autotune(B)
+    assert set(tiles) == {"fp64", "mixed", "fp64_sym"}
+    assert all(kc in (K, (K + 1) // 2) and w in (1, 2, 4) for kc, w in tiles.values()), tiles
+    f = perturbed_maxwellians(c["L"], K, B, seed=99)
+    Qref = oracle.angular_first_batch(np.zeros(B * nq * K), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
+                                      nthreads=8)
+    fd = torch.from_numpy(f).cuda()
+    for prec, tol in (("fp64", TOL64), ("fp64_sym", TOL64), ("mixed", TOL_MIXED)):
+        Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
+        c["plan"].execute(Qd, fd, B, precision=prec)
+        torch.cuda.synchronize()
+        assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= tol, prec

@@ -1,6 +1,5 @@
 """CPU checks of the C-ABI library: it loads, exports every entry point include/cfk.h declares,
 and rejects bad arguments on the host before touching a device.  No compute call is made here."""
-import ctypes
 import os
 import re
 
@@ -24,7 +23,7 @@ def test_header_declares_what_the_binding_knows():
 
 
 def test_library_exports_every_declared_symbol():
-    L = ctypes.CDLL(cfk.LIB_PATH)
+    L = lib()   # loads torch's HIP runtime first, then libcfk
     for name in declared_symbols():
         assert hasattr(L, name), name
     assert lib().cfk_version() >= 1
@@ -55,3 +54,10 @@ def test_sumfac_arity_error_mirrors_the_mex():
     with pytest.raises(cfk.MexError) as e:
         cfk.compute_rtensor_sumfac_mex(*([1.0] * 25))
     assert e.value.identifier == "R_tensor:InvalidInput"
+
+
+def test_single_hip_runtime_in_process():
+    # libcfk must bind to the same libamdhip64 as torch (one runtime per process)
+    from cfk._lib import hip_runtimes_loaded
+    lib()
+    assert len(hip_runtimes_loaded()) == 1, hip_runtimes_loaded()
