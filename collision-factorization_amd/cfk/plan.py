@@ -53,10 +53,10 @@ class CollisionPlan:
         return Q
 
     def autotune(self, B, stream=None):
-        """Pick the fastest tile (k3 chunk KC, batch tiles per workgroup) per precision; {precision: (KC, WPB)}."""
-        t = (ctypes.c_int32 * 6)()
+        """Pick the fastest tile per precision for batches of B; returns {precision: (KC, NB, PF)}."""
+        t = (ctypes.c_int32 * 9)()
         check(lib().cfk_plan_autotune(self._h, int(B), _stream_ptr(stream), t))
-        return {"fp64": (t[0], t[3]), "mixed": (t[1], t[4]), "fp64_sym": (t[2], t[5])}
+        return {"fp64": tuple(t[0:3]), "mixed": tuple(t[3:6]), "fp64_sym": tuple(t[6:9])}
 
     def set_timing(self, enable=True):
         """Bracket each execute's main kernel with HIP events (see cfk_plan_set_timing)."""

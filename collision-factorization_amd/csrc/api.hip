@@ -38,7 +38,7 @@ int check_device() {
     return CFK_OK;
 }
 
-int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, int wpb, const double *fT, double *QT, int64_t B,
+int launch_sparse_fp64(const cfk_plan_s *p, int variant, const Tile &tl, const double *fT, double *QT, int64_t B,
                        int accumulate, hipStream_t st);
 int launch_transpose(const double *in, double *out, int64_t rows, int64_t cols, bool add, hipStream_t st);
 int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t B, double *part, int nsplit,
@@ -46,9 +46,9 @@ int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t
 int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStream_t st);
 int default_kc(int K);
 int half_kc(int K);
-int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, int wpb, const double *fT, double *QT, int64_t B, int accumulate,
+int launch_af_sym_f64(const cfk_plan_s *p, const Tile &tl, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st);
-int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, int wpb, const float *fT, double *QT, int64_t B, int accumulate,
+int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st);
 int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t cols, hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
@@ -266,8 +266,8 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if (dev != p->device) return set_error(CFK_EINVAL, "plan lives on device %d, current device is %d", p->device, dev);
     const int64_t NK = (int64_t)p->N_Q * p->K;
     const bool natural = layout == CFK_LAYOUT_NATURAL;
-    const int kc = p->kc[precision] > 0 ? p->kc[precision] : default_kc(p->K);
-    const int wpb = p->wpb[precision] > 0 ? p->wpb[precision] : 1;
+    const Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_kc(p->K), p->nb[precision] > 0 ? p->nb[precision] : 1,
+                  p->pf[precision]};
     int rc;
     if (precision != CFK_PREC_FP64) {
         if (variant != CFK_VARIANT_ANGULAR_FIRST)
@@ -280,21 +280,21 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
             if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
             if ((rc = launch_transpose_f32(f, p->fT32.p, B, NK, st))) return rc;
             KernelTimer tm(p, st);
-            if ((rc = launch_af_sym_f32(p, true, kc, wpb, p->fT32.p, p->QT.p, B, 0, st))) return rc;
+            if ((rc = launch_af_sym_f32(p, tl, p->fT32.p, p->QT.p, B, 0, st))) return rc;
             tm.stop();
             return launch_transpose(p->QT.p, Q, NK, B, true, st);
         }
         if ((rc = launch_to_f32(f, p->fT32.p, NK * B, st))) return rc;
         KernelTimer tm(p, st);
-        if ((rc = launch_af_sym_f32(p, true, kc, wpb, p->fT32.p, Q, B, 1, st))) return rc;
+        if ((rc = launch_af_sym_f32(p, tl, p->fT32.p, Q, B, 1, st))) return rc;
         tm.stop();
         return CFK_OK;
     }
     const bool sym = precision == CFK_PREC_FP64_SYM;
     if (!natural) {
         KernelTimer tm(p, st);
-        rc = sym ? launch_af_sym_f64(p, false, kc, wpb, f, Q, B, 1, st)
-                 : launch_sparse_fp64(p, variant, kc, wpb, f, Q, B, 1, st);
+        rc = sym ? launch_af_sym_f64(p, tl, f, Q, B, 1, st)
+                 : launch_sparse_fp64(p, variant, tl, f, Q, B, 1, st);
         if (rc) return rc;
         tm.stop();
         return CFK_OK;
@@ -304,8 +304,8 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if ((rc = launch_transpose(f, p->fT.p, B, NK, false, st))) return rc;
     {
         KernelTimer tm(p, st);
-        rc = sym ? launch_af_sym_f64(p, false, kc, wpb, p->fT.p, p->QT.p, B, 0, st)
-                 : launch_sparse_fp64(p, variant, kc, wpb, p->fT.p, p->QT.p, B, 0, st);
+        rc = sym ? launch_af_sym_f64(p, tl, p->fT.p, p->QT.p, B, 0, st)
+                 : launch_sparse_fp64(p, variant, tl, p->fT.p, p->QT.p, B, 0, st);
         if (rc) return rc;
         tm.stop();
     }
@@ -438,47 +438,50 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
     CFK_HIP(hipMemsetAsync(f.p, 0, (size_t)n * sizeof(double), st));
     CFK_HIP(hipMemsetAsync(Q.p, 0, (size_t)n * sizeof(double), st));
     const int kcs[2] = {p->K, half_kc(p->K)};
-    const int wpbs[3] = {1, 2, 4};
     const bool was_timing = p->timing;
     for (int prec = 0; prec < 3; ++prec) {
         if (prec != CFK_PREC_FP64 && !p->sym_ok) continue;
         double best = 1e300;
-        int best_kc = default_kc(p->K), best_w = 1;
+        Tile bt{default_kc(p->K), 1, 0};
         for (int c = 0; c < 2; ++c) {
             if (c == 1 && kcs[1] == kcs[0]) continue;
-            for (int w = 0; w < 3; ++w) {
-                p->kc[prec] = kcs[c];
-                p->wpb[prec] = wpbs[w];
-                p->timing = false;
-                if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p, B, st)))
-                    return rc;  // warm-up
-                p->timing = true;
-                for (int rep = 0; rep < 3; ++rep)
+            for (int nb = 1; nb <= 2; ++nb)
+                for (int pf = 0; pf <= (prec == CFK_PREC_FP64 ? 1 : 0); ++pf) {
+                    p->kc[prec] = kcs[c];
+                    p->nb[prec] = nb;
+                    p->pf[prec] = pf;
+                    p->timing = false;
                     if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p, B,
                                            st)))
-                        return rc;
-                double ms = 0.0;
-                int64_t nl = 0;
-                if ((rc = cfk_plan_read_timing(p, &ms, &nl))) return rc;
-                if (std::getenv("CFK_AUTOTUNE_VERBOSE"))
-                    fprintf(stderr, "cfk autotune: precision %d KC %d WPB %d: %.4f ms\n", prec, kcs[c], wpbs[w],
-                            nl > 0 ? ms / nl : -1.0);
-                if (nl > 0 && ms / nl < best) {
-                    best = ms / nl;
-                    best_kc = kcs[c];
-                    best_w = wpbs[w];
+                        return rc;  // warm-up
+                    p->timing = true;
+                    for (int rep = 0; rep < 3; ++rep)
+                        if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p,
+                                               B, st)))
+                            return rc;
+                    double ms = 0.0;
+                    int64_t nl = 0;
+                    if ((rc = cfk_plan_read_timing(p, &ms, &nl))) return rc;
+                    if (std::getenv("CFK_AUTOTUNE_VERBOSE"))
+                        fprintf(stderr, "cfk autotune: precision %d KC %d NB %d PF %d: %.4f ms\n", prec, kcs[c], nb, pf,
+                                nl > 0 ? ms / nl : -1.0);
+                    if (nl > 0 && ms / nl < best) {
+                        best = ms / nl;
+                        bt = Tile{kcs[c], nb, pf};
+                    }
                 }
-            }
         }
-        p->kc[prec] = best_kc;
-        p->wpb[prec] = best_w;
+        p->kc[prec] = bt.kc;
+        p->nb[prec] = bt.nb;
+        p->pf[prec] = bt.pf;
     }
     p->timing = was_timing;
     CFK_HIP(hipStreamSynchronize(st));
     if (kc_out)
         for (int i = 0; i < 3; ++i) {
-            kc_out[i] = p->kc[i] > 0 ? p->kc[i] : default_kc(p->K);
-            kc_out[3 + i] = p->wpb[i] > 0 ? p->wpb[i] : 1;
+            kc_out[3 * i] = p->kc[i] > 0 ? p->kc[i] : default_kc(p->K);
+            kc_out[3 * i + 1] = p->nb[i] > 0 ? p->nb[i] : 1;
+            kc_out[3 * i + 2] = p->pf[i];
         }
     return CFK_OK;
 }

@@ -56,6 +56,12 @@ struct DevBuf {
     int ensure(size_t count) { return (count <= n && p) ? CFK_OK : alloc(count); }
 };
 
+// Tile variant of the angular-first kernel (chosen by the autotuner): k3 chunk width, distributions
+// per lane, one-entry-ahead prefetch of the next nonzero's f rows (plain tables only).
+struct Tile {
+    int kc, nb, pf;
+};
+
 // One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.  In the device copy
 // of the symmetrized tables the last field instead flags the precise slices (partner block l = 0)
 // for the mixed-precision kernel; the kernels never need q1 (it is the row being computed).
@@ -98,7 +104,8 @@ struct cfk_plan_s {
     cfk::DevBuf<float> fT32;
     // tile choice per precision (index = CFK_PREC_*), set by the autotuner; 0 = default
     int kc[3] = {0, 0, 0};
-    int wpb[3] = {0, 0, 0};  // waves (batch tiles) per workgroup
+    int nb[3] = {0, 0, 0};   // distributions per lane
+    int pf[3] = {0, 0, 0};   // prefetch next nonzero (plain tables)
     // kernel timing (cfk_plan_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

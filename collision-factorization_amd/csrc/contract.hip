@@ -50,26 +50,31 @@ struct SparseTables {
 // in the symmetrized device tables: the blocks whose partner is the l=0 block, which carry the
 // large, cancelling Maxwellian-times-perturbation terms), where it runs in FP64 as for T = double.
 // ---------------------------------------------------------------------------------------------
-template <typename T, int K, int KC, bool GROUPED, bool PROJ, int WPB>
-__global__ __launch_bounds__(WAVE *WPB) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
-                                                       const Slice *__restrict__ slices,
-                                                       const int32_t *__restrict__ row_ptr,
-                                                       const int32_t *__restrict__ row_order,
-                                                       const int2 *__restrict__ zq, const T *__restrict__ zg,
-                                                       const double *__restrict__ R, const float *__restrict__ R32,
-                                                       int64_t B, int N_Q, int accumulate) {
+template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
+__global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
+                                                  const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
+                                                  const T *__restrict__ zg, const double *__restrict__ R,
+                                                  const float *__restrict__ R32, int64_t B, int N_Q, int accumulate) {
     constexpr int K2 = K * K, K3 = K2 * K;
-    // WPB waves of one workgroup take WPB consecutive batch tiles of the SAME output row, so they
-    // walk the same slices in step and share R_t and the table entries in the scalar cache.
-    const int64_t b = ((int64_t)blockIdx.x * WPB + (threadIdx.x / WAVE)) * WAVE + (threadIdx.x % WAVE);
-    const bool active = b < B;
-    const int64_t bb = active ? b : 0;
+    // Each lane owns NB distributions b_i = (tile*NB + i)*64 + lane, so every wave-uniform value
+    // (table entry, Gaunt value, R row) fetched through the scalar path feeds NB FMAs.
+    int64_t bi[NB];
+    bool act[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int64_t b = ((int64_t)blockIdx.x * NB + i) * WAVE + threadIdx.x;
+        act[i] = b < B;
+        bi[i] = act[i] ? b : 0;
+    }
     const int row = row_order[blockIdx.y];
     const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
     const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
-    double q[K];
+    double q[NB][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) q[k] = 0.0;
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int k = 0; k < K; ++k) q[i][k] = 0.0;
     for (int s = s0; s < s1; ++s) {
         const Slice sl = slices[s];
         const double *Rt = R + (int64_t)sl.t * K3;
@@ -77,139 +82,190 @@ __global__ __launch_bounds__(WAVE *WPB) void af_kernel(const T *__restrict__ fT,
         const bool precise = !std::is_same<T, float>::value || sl.q1 != 0;
 #pragma unroll 1
         for (int c0 = 0; c0 < K; c0 += KC) {
-            T phi[K][KC];
+            T phi[NB][K][KC];
 #pragma unroll
-            for (int i = 0; i < K; ++i)
+            for (int i = 0; i < NB; ++i)
 #pragma unroll
-                for (int j = 0; j < KC; ++j) phi[i][j] = T(0);
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) phi[i][k][j] = T(0);
             if (GROUPED) {
-                // one-entry-ahead prefetch of the table entry and of its f rows
-                T f2[K], h[KC], f2n[K], f3n[KC];
+                T f2[NB][K], h[NB][KC];
 #pragma unroll
-                for (int j = 0; j < KC; ++j) h[j] = T(0);
+                for (int i = 0; i < NB; ++i) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) f2[k] = T(0);
-                int2 qn = zq[sl.z0];
-                T gn = zg[sl.z0];
-                {
-                    const T *f3p = fT + (int64_t)qn.y * B + bb;
+                    for (int j = 0; j < KC; ++j) h[i][j] = T(0);
 #pragma unroll
-                    for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
-                    const T *f2p = fT + (int64_t)(~qn.x) * B + bb;   // the first entry always starts a group
-#pragma unroll
-                    for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
+                    for (int k = 0; k < K; ++k) f2[i][k] = T(0);
                 }
                 for (int z = sl.z0; z < sl.z1; ++z) {
-                    const int2 qq = qn;
-                    const T g = gn;
-                    T f3[KC];
+                    const int2 qq = zq[z];
+                    const T g = zg[z];
+                    if (qq.x < 0) {  // new q2 group: flush the previous one, load f[q2,:]
 #pragma unroll
-                    for (int j = 0; j < KC; ++j) f3[j] = f3n[j];
-                    if (qq.x < 0) {  // new q2 group: flush the previous one, take the prefetched row
+                        for (int i = 0; i < NB; ++i)
+#pragma unroll
+                            for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                                for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
+                                h[i][j] = T(0);
+                            }
+#pragma unroll
+                        for (int i = 0; i < NB; ++i) {
+                            const T *f2p = fT + (int64_t)(~qq.x) * B + bi[i];
+#pragma unroll
+                            for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = f2p[k2 * kstride];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) {
+                        const T *f3p = fT + (int64_t)qq.y * B + bi[i];
+#pragma unroll
+                        for (int j = 0; j < KC; ++j)
+                            if (c0 + j < K) h[i][j] = fma(g, f3p[(c0 + j) * kstride], h[i][j]);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int j = 0; j < KC; ++j)
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
+            } else if (PF) {
+                // one-entry-ahead prefetch of the next nonzero's f rows
+                int2 qn = zq[sl.z0];
+                T gn = zg[sl.z0];
+                T f2n[NB][K], f3n[NB][KC];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    const T *f2p = fT + (int64_t)qn.x * B + bi[i];
+                    const T *f3p = fT + (int64_t)qn.y * B + bi[i];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) f2n[i][k] = f2p[k * kstride];
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) f3n[i][j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                }
+                for (int z = sl.z0; z < sl.z1; ++z) {
+                    const T g = gn;
+                    T gf2[NB][K], f3[NB][KC];
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) {
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) gf2[i][k2] = g * f2n[i][k2];
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) f3[i][j] = f3n[i][j];
+                    }
+                    if (z + 1 < sl.z1) {
+                        qn = zq[z + 1];
+                        gn = zg[z + 1];
+#pragma unroll
+                        for (int i = 0; i < NB; ++i) {
+                            const T *f2p = fT + (int64_t)qn.x * B + bi[i];
+                            const T *f3p = fT + (int64_t)qn.y * B + bi[i];
+#pragma unroll
+                            for (int k = 0; k < K; ++k) f2n[i][k] = f2p[k * kstride];
+#pragma unroll
+                            for (int j = 0; j < KC; ++j) f3n[i][j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < NB; ++i)
+#pragma unroll
+                        for (int j = 0; j < KC; ++j)
+#pragma unroll
+                            for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[i][k2], f3[i][j], phi[i][k2][j]);
+                }
+            } else {
+                for (int z = sl.z0; z < sl.z1; ++z) {
+                    const int2 qq = zq[z];
+                    const T g = zg[z];
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) {
+                        const T *f2p = fT + (int64_t)qq.x * B + bi[i];
+                        const T *f3p = fT + (int64_t)qq.y * B + bi[i];
+                        T gf2[K];
+#pragma unroll
+                        for (int k2 = 0; k2 < K; ++k2) gf2[k2] = g * f2p[k2 * kstride];
 #pragma unroll
                         for (int j = 0; j < KC; ++j) {
+                            if (c0 + j < K) {
+                                const T f3 = f3p[(c0 + j) * kstride];
 #pragma unroll
-                            for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(f2[k2], h[j], phi[k2][j]);
-                            h[j] = T(0);
+                                for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[k2], f3, phi[i][k2][j]);
+                            }
                         }
-#pragma unroll
-                        for (int k = 0; k < K; ++k) f2[k] = f2n[k];
-                    }
-                    if (z + 1 < sl.z1) {
-                        qn = zq[z + 1];
-                        gn = zg[z + 1];
-                        const T *f3p = fT + (int64_t)qn.y * B + bb;
-#pragma unroll
-                        for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
-                        if (qn.x < 0) {
-                            const T *f2p = fT + (int64_t)(~qn.x) * B + bb;
-#pragma unroll
-                            for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
-                        }
-                    }
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) h[j] = fma(g, f3[j], h[j]);
-                }
-#pragma unroll
-                for (int j = 0; j < KC; ++j)
-#pragma unroll
-                    for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(f2[k2], h[j], phi[k2][j]);
-            } else {
-                int2 qn = zq[sl.z0];
-                T gn = zg[sl.z0];
-                T f2n[K], f3n[KC];
-                {
-                    const T *f2p = fT + (int64_t)qn.x * B + bb;
-                    const T *f3p = fT + (int64_t)qn.y * B + bb;
-#pragma unroll
-                    for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
-                }
-                for (int z = sl.z0; z < sl.z1; ++z) {
-                    const T g = gn;
-                    T gf2[K], f3[KC];
-#pragma unroll
-                    for (int k2 = 0; k2 < K; ++k2) gf2[k2] = g * f2n[k2];
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) f3[j] = f3n[j];
-                    if (z + 1 < sl.z1) {
-                        qn = zq[z + 1];
-                        gn = zg[z + 1];
-                        const T *f2p = fT + (int64_t)qn.x * B + bb;
-                        const T *f3p = fT + (int64_t)qn.y * B + bb;
-#pragma unroll
-                        for (int k = 0; k < K; ++k) f2n[k] = f2p[k * kstride];
-#pragma unroll
-                        for (int j = 0; j < KC; ++j) f3n[j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
-                    }
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) {
-#pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fma(gf2[k2], f3[j], phi[k2][j]);
                     }
                 }
             }
+            // R.Phi: one k3 plane of R_t (K*K contiguous values) per k3, fetched through the scalar path
+            // in batches of RB values that are all in flight together (scalar loads return out of
+            // order, so each batch costs one wait), then consumed by RB FMAs per distribution.
             if (precise) {
+                constexpr int RB = (K * K + 1) / 2 < 25 ? (K * K + 1) / 2 : 25;  // doubles per batch
 #pragma unroll
                 for (int j = 0; j < KC; ++j) {
                     if (c0 + j < K) {
+                        const double *plane = Rt + (c0 + j) * K2;
 #pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) {
-                            const double p = (double)phi[k2][j];
-                            const double *r = Rt + k2 * K + (c0 + j) * K2;
+                        for (int e0 = 0; e0 < K2; e0 += RB) {
+                            double rr[RB];
 #pragma unroll
-                            for (int k1 = 0; k1 < K; ++k1) q[k1] = fma(r[k1], p, q[k1]);
+                            for (int e = 0; e < RB; ++e)
+                                if (e0 + e < K2) rr[e] = plane[e0 + e];
+#pragma unroll
+                            for (int e = 0; e < RB; ++e) {
+                                if (e0 + e < K2) {
+                                    const int k2 = (e0 + e) / K, k1 = (e0 + e) % K;
+#pragma unroll
+                                    for (int i = 0; i < NB; ++i) q[i][k1] = fma(rr[e], (double)phi[i][k2][j], q[i][k1]);
+                                }
+                            }
                         }
                     }
                 }
             } else {
-                float a[K];
+                constexpr int RB = (K * K + 1) / 2 < 50 ? (K * K + 1) / 2 : 50;  // floats per batch
+                float a[NB][K];
 #pragma unroll
-                for (int k1 = 0; k1 < K; ++k1) a[k1] = 0.0f;
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int k1 = 0; k1 < K; ++k1) a[i][k1] = 0.0f;
 #pragma unroll
                 for (int j = 0; j < KC; ++j) {
                     if (c0 + j < K) {
+                        const float *plane = Rt32 + (c0 + j) * K2;
 #pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) {
-                            const float p = (float)phi[k2][j];
-                            const float *r = Rt32 + k2 * K + (c0 + j) * K2;
+                        for (int e0 = 0; e0 < K2; e0 += RB) {
+                            float rr[RB];
 #pragma unroll
-                            for (int k1 = 0; k1 < K; ++k1) a[k1] = fmaf(r[k1], p, a[k1]);
+                            for (int e = 0; e < RB; ++e)
+                                if (e0 + e < K2) rr[e] = plane[e0 + e];
+#pragma unroll
+                            for (int e = 0; e < RB; ++e) {
+                                if (e0 + e < K2) {
+                                    const int k2 = (e0 + e) / K, k1 = (e0 + e) % K;
+#pragma unroll
+                                    for (int i = 0; i < NB; ++i) a[i][k1] = fmaf(rr[e], (float)phi[i][k2][j], a[i][k1]);
+                                }
+                            }
                         }
                     }
                 }
 #pragma unroll
-                for (int k1 = 0; k1 < K; ++k1) q[k1] += (double)a[k1];
+                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                    for (int k1 = 0; k1 < K; ++k1) q[i][k1] += (double)a[i][k1];
             }
         }
     }
-    if (active) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        if (!act[i]) continue;
 #pragma unroll
         for (int k1 = 0; k1 < K; ++k1) {
-            double v = q[k1];
+            double v = q[i][k1];
             if (PROJ && ((row < 4 && k1 == 0) || (row == 0 && k1 == 1))) v = 0.0;
-            double *o = QT + ((int64_t)k1 * N_Q + row) * B + b;
+            double *o = QT + ((int64_t)k1 * N_Q + row) * B + bi[i];
             *o = accumulate ? *o + v : v;
         }
     }
@@ -376,24 +432,13 @@ __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int la
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-template <typename T, int K, int KC, bool GROUPED, int WPB>
-void launch_af_w(const SparseTables<T> &tb, int N_Q, const T *fT, double *QT, int64_t B, int acc, hipStream_t st) {
-    dim3 grid((unsigned)((B + WAVE * WPB - 1) / (WAVE * WPB)), (unsigned)N_Q);
+template <typename T, int K, int KC, bool GROUPED, int NB, bool PF>
+void launch_af_v(const SparseTables<T> &tb, int N_Q, const T *fT, double *QT, int64_t B, int acc, hipStream_t st) {
+    dim3 grid((unsigned)((B + WAVE * NB - 1) / (WAVE * NB)), (unsigned)N_Q);
     // double runs the plain operator (no projection), float the mixed path (always projected)
     constexpr bool PROJ = std::is_same<T, float>::value;
-    hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, WPB>), grid, dim3(WAVE * WPB), 0, st, fT, QT, tb.slices,
+    hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(WAVE), 0, st, fT, QT, tb.slices,
                        tb.row_ptr, tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc);
-}
-
-template <typename T, int K, int KC, bool GROUPED>
-void launch_af(const SparseTables<T> &tb, int N_Q, int wpb, const T *fT, double *QT, int64_t B, int acc,
-               hipStream_t st) {
-    if (wpb >= 4)
-        launch_af_w<T, K, KC, GROUPED, 4>(tb, N_Q, fT, QT, B, acc, st);
-    else if (wpb >= 2)
-        launch_af_w<T, K, KC, GROUPED, 2>(tb, N_Q, fT, QT, B, acc, st);
-    else
-        launch_af_w<T, K, KC, GROUPED, 1>(tb, N_Q, fT, QT, B, acc, st);
 }
 
 // k3 chunk candidates: the full width and half of it.
@@ -401,13 +446,40 @@ constexpr int kc_half(int K) { return (K + 1) / 2; }
 // Default chunk: full width while the Phi accumulators stay within ~100 values.
 constexpr int kc_default(int K) { return (K * K <= 100) ? K : kc_half(K); }
 
+// K values that get the full set of tile variants (the workloads' sizes); other K get the default
+// tile only, which keeps the build time bounded.
+#ifndef CFK_TUNED_K
+#define CFK_TUNED_K(K) ((K) == 6 || (K) == 8 || (K) == 10 || (K) == 12 || (K) == 14)
+#endif
+
 template <typename T, int K, bool GROUPED>
-void launch_af_k(const SparseTables<T> &tb, int N_Q, int kc, int wpb, const T *fT, double *QT, int64_t B, int acc,
+void launch_af_k(const SparseTables<T> &tb, int N_Q, const Tile &tl, const T *fT, double *QT, int64_t B, int acc,
                  hipStream_t st) {
-    if (kc >= K)
-        launch_af<T, K, K, GROUPED>(tb, N_Q, wpb, fT, QT, B, acc, st);
-    else
-        launch_af<T, K, kc_half(K), GROUPED>(tb, N_Q, wpb, fT, QT, B, acc, st);
+    constexpr int KCD = kc_default(K);
+    if constexpr (CFK_TUNED_K(K)) {
+        const bool full = tl.kc >= K;
+        const bool pf = tl.pf && !GROUPED;
+#define CFK_AF_NB(KCV, PFV)                                                             \
+    do {                                                                                \
+        if (tl.nb >= 2)                                                                 \
+            launch_af_v<T, K, KCV, GROUPED, 2, PFV>(tb, N_Q, fT, QT, B, acc, st);       \
+        else                                                                            \
+            launch_af_v<T, K, KCV, GROUPED, 1, PFV>(tb, N_Q, fT, QT, B, acc, st);       \
+    } while (0)
+        if constexpr (GROUPED) {
+            (void)pf;
+            if (full) CFK_AF_NB(K, false); else CFK_AF_NB(kc_half(K), false);
+        } else {
+            if (full) {
+                if (pf) CFK_AF_NB(K, true); else CFK_AF_NB(K, false);
+            } else {
+                if (pf) CFK_AF_NB(kc_half(K), true); else CFK_AF_NB(kc_half(K), false);
+            }
+        }
+#undef CFK_AF_NB
+    } else {
+        launch_af_v<T, K, KCD, GROUPED, 1, false>(tb, N_Q, fT, QT, B, acc, st);
+    }
 }
 
 template <int K>
@@ -426,7 +498,7 @@ void launch_k3(const cfk_plan_s *p, bool rf, const double *fT, double *QT, int64
 int default_kc(int K) { return kc_default(K); }
 int half_kc(int K) { return kc_half(K); }
 
-int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, int wpb, const double *fT, double *QT, int64_t B,
+int launch_sparse_fp64(const cfk_plan_s *p, int variant, const Tile &tl, const double *fT, double *QT, int64_t B,
                        int accumulate, hipStream_t st) {
     if (p->K < 1 || p->K > 16) return set_error(CFK_EUNSUP, "K=%d unsupported (1..16)", p->K);
     if (B <= 0) return CFK_OK;
@@ -436,7 +508,7 @@ int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, int wpb, const 
 #define X(KK)                                                                                  \
     case KK:                                                                                   \
         if (variant == CFK_VARIANT_ANGULAR_FIRST)                                              \
-            launch_af_k<double, KK, false>(tb, p->N_Q, kc, wpb, fT, QT, B, accumulate, st);         \
+            launch_af_k<double, KK, false>(tb, p->N_Q, tl, fT, QT, B, accumulate, st);         \
         else                                                                                   \
             launch_k3<KK>(p, variant == CFK_VARIANT_RADIAL_FIRST, fT, QT, B, accumulate, st);  \
         break;
@@ -449,14 +521,14 @@ int launch_sparse_fp64(const cfk_plan_s *p, int variant, int kc, int wpb, const 
 
 // Angular-first over the symmetrized tables (q2 <= q3 entries only, merged R blocks).
 template <typename T>
-int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, bool proj, int kc, int wpb, const T *fT, double *QT,
+int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, const Tile &tl, const T *fT, double *QT,
                   int64_t B, int accumulate, hipStream_t st) {
     if (p->K < 1 || p->K > 16) return set_error(CFK_EUNSUP, "K=%d unsupported (1..16)", p->K);
     if (B <= 0) return CFK_OK;
     switch (p->K) {
 #define X(KK)                                                                   \
     case KK:                                                                    \
-        launch_af_k<T, KK, true>(tb, p->N_Q, kc, wpb, fT, QT, B, accumulate, st);    \
+        launch_af_k<T, KK, true>(tb, p->N_Q, tl, fT, QT, B, accumulate, st);    \
         break;
         CFK_K_CASES(X)
 #undef X
@@ -465,18 +537,18 @@ int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, bool proj, int
     return CFK_OK;
 }
 
-int launch_af_sym_f64(const cfk_plan_s *p, bool proj, int kc, int wpb, const double *fT, double *QT, int64_t B, int accumulate,
+int launch_af_sym_f64(const cfk_plan_s *p, const Tile &tl, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st) {
     const SparseTables<double> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg.p,
                                   p->sym_R.p, nullptr};
-    return launch_af_sym<double>(p, tb, proj, kc, wpb, fT, QT, B, accumulate, st);
+    return launch_af_sym<double>(p, tb, tl, fT, QT, B, accumulate, st);
 }
 
-int launch_af_sym_f32(const cfk_plan_s *p, bool proj, int kc, int wpb, const float *fT, double *QT, int64_t B, int accumulate,
+int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st) {
     const SparseTables<float> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg32.p,
                                  p->sym_R.p, p->sym_R32.p};
-    return launch_af_sym<float>(p, tb, proj, kc, wpb, fT, QT, B, accumulate, st);
+    return launch_af_sym<float>(p, tb, tl, fT, QT, B, accumulate, st);
 }
 
 int launch_transpose(const double *in, double *out, int64_t rows, int64_t cols, bool add, hipStream_t st) {

@@ -137,10 +137,12 @@ int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t 
                      const double *f_dev, int64_t B, void *stream);
 
 /* Autotuner: times every tile variant of the angular-first kernels -- k3 chunk width KC in
- * {K, ceil(K/2)} x batch tiles per workgroup in {1,2,4} -- for each precision on a synthetic batch
- * of B distributions (batch-fastest layout) on `stream` and keeps the fastest per precision in the
- * plan.  When tiles_out is not NULL it receives 6 ints: KC for (FP64, MIXED, FP64_SYM), then the
- * tiles per workgroup for the same three.  Allocates temporary device buffers; synchronizes. */
+ * {K, ceil(K/2)} x distributions per lane NB in {1,2} x (FP64 only) prefetch of the next nonzero's
+ * rows on/off -- for each precision on a synthetic batch of B distributions (batch-fastest layout)
+ * on `stream` and keeps the fastest per precision in the plan.  When tiles_out is not NULL it
+ * receives 9 ints: (KC, NB, PF) for FP64, MIXED, FP64_SYM in that order.  Allocates temporary
+ * device buffers; synchronizes `stream`.  The full variant set exists for K in {6,8,10,12,14}; other
+ * K run the default tile. */
 int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *tiles_out);
 
 /* Kernel timing: when enabled, every execute brackets its main contraction kernel (not the
