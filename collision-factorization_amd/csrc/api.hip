@@ -45,6 +45,7 @@ int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t
                  hipStream_t st);
 int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStream_t st);
 int default_kc(int K);
+int table_padding();
 int half_kc(int K);
 int launch_af_sym_f64(const cfk_plan_s *p, const Tile &tl, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st);
@@ -155,6 +156,10 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
     for (Slice &s : sl) s.q1 = (trip[s.t][1] == 0) ? 1 : 0;
     p->sym_N_G = (int64_t)zq.size();
     p->sym_S = (int64_t)tq.size();
+    for (int i = 0; i < table_padding(); ++i) {  // batch-fetch padding (never used as entries)
+        zq.push_back(make_int2(0, 0));
+        zg.push_back(0.0);
+    }
     int rc;
     if ((rc = p->sym_slices.upload(sl.data(), sl.size()))) return rc;
     if ((rc = p->sym_row_ptr.upload(row_ptr.data(), row_ptr.size()))) return rc;
@@ -246,6 +251,11 @@ int build_plan(const double *labels, const double *vals, const double *ic_map, i
     if ((rc = p->slices.upload(p->h_slices.data(), p->h_slices.size()))) return rc;
     if ((rc = p->row_ptr.upload(p->h_row_ptr.data(), p->h_row_ptr.size()))) return rc;
     if ((rc = p->row_order.upload(p->h_row_order.data(), p->h_row_order.size()))) return rc;
+    // padding entries: the kernels fetch table entries in fixed-size batches
+    for (int i = 0; i < table_padding(); ++i) {
+        zq.push_back(make_int2(0, 0));
+        zg.push_back(0.0);
+    }
     if ((rc = p->zq.upload(zq.data(), zq.size()))) return rc;
     if ((rc = p->zg.upload(zg.data(), zg.size()))) return rc;
     return build_sym_tables(labels, vals, ic_map, N_G, R, perm, p);

@@ -16,6 +16,9 @@
 namespace cfk {
 
 constexpr int WAVE = 64;
+// routing-table entries fetched per scalar round trip; the device tables carry ZB padding entries
+constexpr int ZB = 8;
+int table_padding() { return ZB; }
 
 #define CFK_K_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
 
@@ -89,6 +92,8 @@ __global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, doub
                 for (int k = 0; k < K; ++k)
 #pragma unroll
                     for (int j = 0; j < KC; ++j) phi[i][k][j] = T(0);
+            // Table entries are fetched ZB at a time (one scalar round trip per batch; the device arrays
+            // carry ZB padding entries so the batch read never leaves them).
             if (GROUPED) {
                 T f2[NB][K], h[NB][KC];
 #pragma unroll
@@ -98,31 +103,44 @@ __global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, doub
 #pragma unroll
                     for (int k = 0; k < K; ++k) f2[i][k] = T(0);
                 }
-                for (int z = sl.z0; z < sl.z1; ++z) {
-                    const int2 qq = zq[z];
-                    const T g = zg[z];
-                    if (qq.x < 0) {  // new q2 group: flush the previous one, load f[q2,:]
+                for (int zb = sl.z0; zb < sl.z1; zb += ZB) {
+                    int2 qv[ZB];
+                    T gv[ZB];
 #pragma unroll
-                        for (int i = 0; i < NB; ++i)
-#pragma unroll
-                            for (int j = 0; j < KC; ++j) {
-#pragma unroll
-                                for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
-                                h[i][j] = T(0);
-                            }
-#pragma unroll
-                        for (int i = 0; i < NB; ++i) {
-                            const T *f2p = fT + (int64_t)(~qq.x) * B + bi[i];
-#pragma unroll
-                            for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = f2p[k2 * kstride];
-                        }
+                    for (int e = 0; e < ZB; ++e) {
+                        qv[e] = zq[zb + e];
+                        gv[e] = zg[zb + e];
                     }
 #pragma unroll
-                    for (int i = 0; i < NB; ++i) {
-                        const T *f3p = fT + (int64_t)qq.y * B + bi[i];
+                    for (int e = 0; e < ZB; ++e) {
+                        if (zb + e < sl.z1) {
+                            const int2 qq = qv[e];
+                            const T g = gv[e];
+                            if (qq.x < 0) {  // new q2 group: flush the previous one, load f[q2,:]
 #pragma unroll
-                        for (int j = 0; j < KC; ++j)
-                            if (c0 + j < K) h[i][j] = fma(g, f3p[(c0 + j) * kstride], h[i][j]);
+                                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                                    for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                                        for (int k2 = 0; k2 < K; ++k2)
+                                            phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
+                                        h[i][j] = T(0);
+                                    }
+#pragma unroll
+                                for (int i = 0; i < NB; ++i) {
+                                    const T *f2p = fT + (int64_t)(~qq.x) * B + bi[i];
+#pragma unroll
+                                    for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = f2p[k2 * kstride];
+                                }
+                            }
+#pragma unroll
+                            for (int i = 0; i < NB; ++i) {
+                                const T *f3p = fT + (int64_t)qq.y * B + bi[i];
+#pragma unroll
+                                for (int j = 0; j < KC; ++j)
+                                    if (c0 + j < K) h[i][j] = fma(g, f3p[(c0 + j) * kstride], h[i][j]);
+                            }
+                        }
                     }
                 }
 #pragma unroll
@@ -176,22 +194,32 @@ __global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, doub
                             for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[i][k2], f3[i][j], phi[i][k2][j]);
                 }
             } else {
-                for (int z = sl.z0; z < sl.z1; ++z) {
-                    const int2 qq = zq[z];
-                    const T g = zg[z];
+                for (int zb = sl.z0; zb < sl.z1; zb += ZB) {
+                    int2 qv[ZB];
+                    T gv[ZB];
 #pragma unroll
-                    for (int i = 0; i < NB; ++i) {
-                        const T *f2p = fT + (int64_t)qq.x * B + bi[i];
-                        const T *f3p = fT + (int64_t)qq.y * B + bi[i];
-                        T gf2[K];
+                    for (int e = 0; e < ZB; ++e) {
+                        qv[e] = zq[zb + e];
+                        gv[e] = zg[zb + e];
+                    }
 #pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) gf2[k2] = g * f2p[k2 * kstride];
+                    for (int e = 0; e < ZB; ++e) {
+                        if (zb + e < sl.z1) {
 #pragma unroll
-                        for (int j = 0; j < KC; ++j) {
-                            if (c0 + j < K) {
-                                const T f3 = f3p[(c0 + j) * kstride];
+                            for (int i = 0; i < NB; ++i) {
+                                const T *f2p = fT + (int64_t)qv[e].x * B + bi[i];
+                                const T *f3p = fT + (int64_t)qv[e].y * B + bi[i];
+                                T gf2[K];
 #pragma unroll
-                                for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[k2], f3, phi[i][k2][j]);
+                                for (int k2 = 0; k2 < K; ++k2) gf2[k2] = gv[e] * f2p[k2 * kstride];
+#pragma unroll
+                                for (int j = 0; j < KC; ++j) {
+                                    if (c0 + j < K) {
+                                        const T f3 = f3p[(c0 + j) * kstride];
+#pragma unroll
+                                        for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[k2], f3, phi[i][k2][j]);
+                                    }
+                                }
                             }
                         }
                     }
