@@ -20,6 +20,10 @@ sys.path.insert(0, os.path.join(ROOT, "collision-factorization_amd"))
 
 METRIC = "Q(f) evals/sec at HS Lmax=8 batch 4096 vs FP64 GPU path @<=1e-6 rel err"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
+# HBM bytes per launch of the mixed kernel at this workload, from two rocprofv3 PMC passes
+# (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration (factors 2.0 and 1.0); the commands and
+# raw figures are in profiles/af_kernel_mixed_hs_l8_k10_b4096.txt.  Re-measured when the kernel changes.
+MEASURED_TRAFFIC_BYTES = 77868 * 1024
 
 
 def work_model(lab, ic, K):
@@ -148,7 +152,7 @@ def main():
         "tiles_kc": tiles,
         "setup_s": {"geometry": t1 - t0, "rtensor_sumfac_gpu": t2 - t1},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "frac": achieved / PEAK["fp32"], "traffic": MEASURED_TRAFFIC_BYTES if B == 4096 else None,
                      "fp64_path_frac": F * B / k64 / PEAK["fp64"]},
         "cpu_baseline": None,
     }
