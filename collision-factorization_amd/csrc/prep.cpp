@@ -375,11 +375,11 @@ SumfacTables sumfac_tables(const RealGaunt &g, const SumfacGrids &gr) {
     T.L = L;
     T.trip.assign((size_t)n_l * 3, 0);
     T.qi_off.assign(n_l + 1, 0);
-    T.P_loss_p1.assign((size_t)n_l * Nu * Nt, 0.0);
-    T.P_loss_p2.assign((size_t)n_l * Ny, 0.0);
     std::vector<std::vector<size_t>> members(n_l);
     for (size_t z = 0; z < g.val.size(); ++z) members[g.t[z] - 1].push_back(z);
-    std::vector<double> gain1, gain2;
+    // per t: slot list and its (u,t) / y profiles, collected first, then laid out slot-fastest
+    std::vector<std::vector<double>> g1s, g2s;   // per slot
+    std::vector<std::vector<double>> l1s(n_l), l2s(n_l);
     for (int t = 0; t < n_l; ++t) {
         const auto &mz = members[t];
         size_t z0 = mz.front();
@@ -394,6 +394,7 @@ SumfacTables sumfac_tables(const RealGaunt &g, const SumfacGrids &gr) {
         double SCALE = (8.0 * M_PI * M_PI * Yj0) / D;
         std::vector<int> qlist;
         std::vector<std::vector<double>> g1, g2;
+        std::vector<double> lp1((size_t)Nu * Nt, 0.0), lp2((size_t)Ny, 0.0);
         for (size_t z : mz) {
             int qi = g.q1[z] - 1, qj = g.q2[z] - 1, qk = g.q3[z] - 1;
             if (qj != qj0) continue;
@@ -411,22 +412,39 @@ SumfacTables sumfac_tables(const RealGaunt &g, const SumfacGrids &gr) {
                 for (int tt = 0; tt < Nt; ++tt) {
                     double yk = Y1[((size_t)u * Nt + tt) * nq + qk];
                     g1[slot][(size_t)u * Nt + tt] += G * yk;
-                    if (qi == qi0) T.P_loss_p1[((size_t)t * Nu + u) * Nt + tt] += G * Yi0 * yk;
+                    if (qi == qi0) lp1[(size_t)u * Nt + tt] += G * Yi0 * yk;
                 }
             for (int i = 0; i < Ny; ++i) {
                 double yk = Y2[(size_t)i * nq + qk];
                 g2[slot][i] += G * yk;
-                if (qi == qi0) T.P_loss_p2[(size_t)t * Ny + i] += G * Yi0 * yk;
+                if (qi == qi0) lp2[i] += G * Yi0 * yk;
             }
         }
-        for (size_t i = 0; i < (size_t)Nu * Nt; ++i) T.P_loss_p1[(size_t)t * Nu * Nt + i] *= SCALE;
-        for (int i = 0; i < Ny; ++i) T.P_loss_p2[(size_t)t * Ny + i] *= SCALE;
+        for (double &v : lp1) v *= SCALE;
+        for (double &v : lp2) v *= SCALE;
+        l1s[t] = std::move(lp1);
+        l2s[t] = std::move(lp2);
         for (size_t s = 0; s < qlist.size(); ++s) {
             T.qi.push_back(qlist[s]);
-            for (double v : g1[s]) T.P_gain_p1.push_back(v * SCALE);
-            for (double v : g2[s]) T.P_gain_p2.push_back(v * SCALE);
+            for (double &v : g1[s]) v *= SCALE;
+            for (double &v : g2[s]) v *= SCALE;
+            g1s.push_back(std::move(g1[s]));
+            g2s.push_back(std::move(g2[s]));
         }
         T.qi_off[t + 1] = (int)T.qi.size();
+    }
+    const size_t ns = T.qi.size();
+    T.P_gain_p1.resize((size_t)Nu * Nt * ns);
+    T.P_gain_p2.resize((size_t)Ny * ns);
+    T.P_loss_p1.resize((size_t)Nu * Nt * n_l);
+    T.P_loss_p2.resize((size_t)Ny * n_l);
+    for (size_t s = 0; s < ns; ++s) {
+        for (size_t i = 0; i < (size_t)Nu * Nt; ++i) T.P_gain_p1[i * ns + s] = g1s[s][i];
+        for (size_t i = 0; i < (size_t)Ny; ++i) T.P_gain_p2[i * ns + s] = g2s[s][i];
+    }
+    for (int t = 0; t < n_l; ++t) {
+        for (size_t i = 0; i < (size_t)Nu * Nt; ++i) T.P_loss_p1[i * n_l + t] = l1s[t][i];
+        for (size_t i = 0; i < (size_t)Ny; ++i) T.P_loss_p2[i * n_l + t] = l2s[t][i];
     }
     return T;
 }
@@ -446,8 +464,6 @@ SumfacTables compact_mex_tables(int N_Q, int n_l, const double *L_triplets, cons
     T.L = L;
     T.trip.resize((size_t)n_l * 3);
     T.qi_off.assign(n_l + 1, 0);
-    T.P_loss_p1.resize((size_t)n_l * Nu * Nt);
-    T.P_loss_p2.resize((size_t)n_l * Ny);
     for (int t = 0; t < n_l; ++t) {
         for (int c = 0; c < 3; ++c) {
             double v = L_triplets[t + (size_t)n_l * c];
@@ -467,16 +483,27 @@ SumfacTables compact_mex_tables(int N_Q, int n_l, const double *L_triplets, cons
                 return T;
             }
             T.qi.push_back(q);
-            for (int u = 0; u < Nu; ++u)
-                for (int tt = 0; tt < Nt; ++tt)
-                    T.P_gain_p1.push_back(P_gain_p1[tt + (size_t)Nt * (u + (size_t)Nu * (q + (size_t)N_Q * t))]);
-            for (int i = 0; i < Ny; ++i) T.P_gain_p2.push_back(P_gain_p2[i + (size_t)Ny * (q + (size_t)N_Q * t)]);
         }
         T.qi_off[t + 1] = (int)T.qi.size();
+    }
+    const size_t ns = T.qi.size();
+    T.P_gain_p1.resize((size_t)Nu * Nt * ns);
+    T.P_gain_p2.resize((size_t)Ny * ns);
+    T.P_loss_p1.resize((size_t)Nu * Nt * n_l);
+    T.P_loss_p2.resize((size_t)Ny * n_l);
+    for (int t = 0; t < n_l; ++t) {
+        for (int s = T.qi_off[t]; s < T.qi_off[t + 1]; ++s) {
+            const int q = T.qi[s];
+            for (int u = 0; u < Nu; ++u)
+                for (int tt = 0; tt < Nt; ++tt)
+                    T.P_gain_p1[((size_t)u * Nt + tt) * ns + s] =
+                        P_gain_p1[tt + (size_t)Nt * (u + (size_t)Nu * (q + (size_t)N_Q * t))];
+            for (int i = 0; i < Ny; ++i) T.P_gain_p2[(size_t)i * ns + s] = P_gain_p2[i + (size_t)Ny * (q + (size_t)N_Q * t)];
+        }
         for (int u = 0; u < Nu; ++u)
             for (int tt = 0; tt < Nt; ++tt)
-                T.P_loss_p1[((size_t)t * Nu + u) * Nt + tt] = P_loss_p1[tt + (size_t)Nt * (u + (size_t)Nu * t)];
-        for (int i = 0; i < Ny; ++i) T.P_loss_p2[(size_t)t * Ny + i] = P_loss_p2[i + (size_t)Ny * t];
+                T.P_loss_p1[((size_t)u * Nt + tt) * n_l + t] = P_loss_p1[tt + (size_t)Nt * (u + (size_t)Nu * t)];
+        for (int i = 0; i < Ny; ++i) T.P_loss_p2[(size_t)i * n_l + t] = P_loss_p2[i + (size_t)Ny * t];
     }
     return T;
 }

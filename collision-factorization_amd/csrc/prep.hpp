@@ -55,7 +55,9 @@ struct SumfacTables {
     std::vector<int32_t> trip;        // [n_l][3] (l1,l2,l3)
     std::vector<int32_t> qi_off;      // [n_l+1] offsets into the compact valid list
     std::vector<int32_t> qi;          // 0-based q_i, in the reference's qi_valid_mat order
-    // patch 1: gain [slot][N_u1][N_t1], loss [t][N_u1][N_t1]; patch 2: gain [slot][N_y2], loss [t][N_y2]
+    // Device layouts, fastest index last: patch 1 gain [N_u1][N_t1][slot], loss [N_u1][N_t1][t];
+    // patch 2 gain [N_y2][slot], loss [N_y2][t].  For one quadrature step all slots (t_chans) are
+    // contiguous, so a workgroup loads its share with coalesced reads.
     std::vector<double> P_gain_p1, P_loss_p1, P_gain_p2, P_loss_p2;
 };
 

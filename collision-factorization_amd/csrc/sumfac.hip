@@ -33,8 +33,10 @@ namespace {
 
 constexpr int SF_THREADS = 256;
 constexpr int NPB = 4;        // gain points per LDS batch
-constexpr int MAXL = 16;      // L <= 16 (N_Q <= 289)
+constexpr int MAXL = 12;      // L <= 12 (N_Q <= 169)
 constexpr int MAXK = 16;      // N_K <= 16
+constexpr int MAXSLOT = 2 * MAXL + 1;  // valid q_i per t_chan (2 l1 + 1)
+constexpr int MAXG = 128;     // N_chi, N_eps <= 128
 constexpr double EPS_SAFE = 1e-15;
 
 struct SumfacDev {
@@ -47,17 +49,22 @@ struct SumfacDev {
     const int *trip;        // [NL][3]
     const int *qi_off;      // [NL+1]
     const int *qi;          // [nslot] 0-based q
-    const double *Pg1;      // [nslot][Nu1][Nt1]
-    const double *Pl1;      // [NL][Nu1][Nt1]
-    const double *Pg2;      // [nslot][Ny2]
-    const double *Pl2;      // [NL][Ny2]
+    const double *Pg1;      // [Nu1][Nt1][nslot]
+    const double *Pl1;      // [Nu1][Nt1][NL]
+    const double *Pg2;      // [Ny2][nslot]
+    const double *Pl2;      // [Ny2][NL]
+    int max_block_slots;    // max over t_chan blocks of their number of valid q_i
 };
 
 // eval_radial for all k at one (l, v): out[k] = norm[k,l] * L_k^{(l+1/2)}(v^2) * v^l
 // (compute_rtensor_sumfac_mex.cpp:14-34, same recursion and operation order).
-__device__ inline void radial_all_k(int NK, int l, double v, const double *radnorm, double *out) {
-    const double v2 = v * v, al = (double)l + 0.5, vl = pow(v, (double)l);
+template <int NK>
+__device__ inline void radial_all_k(int l, double v, const double *radnorm, double *out) {
+    double vl = 1.0;  // v^l by repeated multiplication (the reference calls pow(v, l))
+    for (int i = 0; i < l; ++i) vl *= v;
+    const double v2 = v * v, al = (double)l + 0.5;
     double L0 = 1.0, L1 = 1.0 + al - v2;
+#pragma unroll
     for (int k = 0; k < NK; ++k) {
         double Lk;
         if (k == 0)
@@ -75,7 +82,8 @@ __device__ inline void radial_all_k(int NK, int l, double v, const double *radno
 
 // Real SH of one order m >= 0 for all degrees l = m..L (eval_SH / legendre_P,
 // compute_rtensor_sumfac_mex.cpp:36-75): Y[l^2+l+m] = N P cos(m phi), Y[l^2+l-m] = N P sin(m phi).
-__device__ inline void sh_order_m(int L, int m, double x, double phi, const double *shnorm, double *Y) {
+// x = cos(theta); (cphi, sphi) = (cos phi, sin phi); cos/sin(m phi) come from the m-th complex power.
+__device__ inline void sh_order_m(int L, int m, double x, double cphi, double sphi, const double *shnorm, double *Y) {
     double pmm = 1.0;
     if (m > 0) {
         const double s = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
@@ -85,7 +93,12 @@ __device__ inline void sh_order_m(int L, int m, double x, double phi, const doub
             fact += 2.0;
         }
     }
-    const double cm = cos(m * phi), sm = sin(m * phi);
+    double cm = 1.0, sm = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double c = cm * cphi - sm * sphi;
+        sm = sm * cphi + cm * sphi;
+        cm = c;
+    }
     double p_lm2 = 0.0, p_lm1 = 0.0;
     for (int l = m; l <= L; ++l) {
         double P;
@@ -115,8 +128,12 @@ struct TGeom {
     double U_x2, U_z2, u_mag2, zx2, zz2;
 };
 
+// Post-collision speed and direction (compute_rtensor_sumfac_mex.cpp:199-213).  The reference
+// forms theta = acos(clamp(vz/|v|)) and phi = atan2(vy, vx) and evaluates cos(theta), cos(m phi),
+// sin(m phi); here cos(theta) = clamp(vz/|v|) and (cos phi, sin phi) = (vx, vy)/|(vx, vy)| directly
+// (equal up to rounding; atan2(0,0) = 0 gives (1, 0)).
 __device__ inline void post_collision(const TGeom &g, double c_chi, double s_chi, double c_eps, double s_eps, int side,
-                                      double &vp_mag, double &theta, double &phi) {
+                                      double &vp_mag, double &ctheta, double &cphi, double &sphi) {
     const double u_px = s_chi * c_eps, u_py = s_chi * s_eps, u_pz = c_chi;
     const double zx = side ? g.zx2 : g.zx1, zz = side ? g.zz2 : g.zz1;
     const double Ux = side ? g.U_x2 : g.U_x1, Uz = side ? g.U_z2 : g.U_z1, um = side ? g.u_mag2 : g.u_mag1;
@@ -126,8 +143,15 @@ __device__ inline void post_collision(const TGeom &g, double c_chi, double s_chi
     const double vy = 0.5 * (um * u_py);
     const double vz = 0.5 * (Uz + um * up_z);
     vp_mag = sqrt(fmax(vx * vx + vy * vy + vz * vz, EPS_SAFE));
-    theta = acos(fmax(fmin(vz / vp_mag, 1.0), -1.0));
-    phi = atan2(vy, vx);
+    ctheta = fmax(fmin(vz / vp_mag, 1.0), -1.0);
+    const double rho = sqrt(vx * vx + vy * vy);
+    if (rho > 0.0) {
+        cphi = vx / rho;
+        sphi = vy / rho;
+    } else {
+        cphi = 1.0;
+        sphi = 0.0;
+    }
 }
 
 __device__ inline void make_tgeom(TGeom &g, double x_i, double v_large, double v_small, double c_beta, double s_beta,
@@ -152,34 +176,64 @@ __device__ inline void make_tgeom(TGeom &g, double x_i, double v_large, double v
 }
 
 // Stage A.  Unit id u in [0, n_units): x = x0 + u / Ux; r = u % Ux; r < Nu1 -> patch 1 (u_idx=r),
-// else patch 2 (y_idx, t_idx) = divmod(r - Nu1, Nt2).  Grid: (n_units, ceil(NL / SF_THREADS)).
+// else patch 2 (y_idx, t_idx) = divmod(r - Nu1, Nt2).  Grid: (n_units, ceil(NL / SF_THREADS)); the
+// block's lanes are the t_chans [tc0, tc0 + SF_THREADS).
 // Outputs: net[(unit*NL + tc)*2*NK + side*NK + k1], rad[unit][side][l][k] (side 0 = v_large).
+//
+// Per inner-t step the block stages its t_chans' P_gain slots in LDS (one coalesced load; they do not
+// depend on the gain point).  Per batch of NPB gain points: (1) one thread per (point, side) computes
+// the post-collision speed and direction; (2) SH tasks (one per (point, side, order m)) and radial
+// tasks (one per (point, side, degree l)) fill LDS tables, each kind in its own half of the
+// workgroup so no wave mixes the two; (3) every t_chan lane does its sparse dot over its slots and
+// updates its Phi accumulators (registers) in the reference's gain-point order.
+template <int NK>
 __global__ __launch_bounds__(SF_THREADS) void sumfac_units_kernel(SumfacDev d, int x0, int64_t n_units,
                                                                   double *__restrict__ net, double *__restrict__ rad) {
     __shared__ double sY[NPB][2][(MAXL + 1) * (MAXL + 1)];
     __shared__ double sR[NPB][2][(MAXL + 1) * MAXK];
-    __shared__ double sW[NPB];              // gain_base * W_eps per point
+    __shared__ double sGeo[NPB][2][4];      // vp_mag, cos(theta), cos(phi), sin(phi)
+    __shared__ double sW[NPB];              // gain_base per point
     __shared__ double sRadU[2][(MAXL + 1) * MAXK];  // radial tables at v_large / v_small
+    __shared__ double sCE[MAXG], sSE[MAXG], sCC[MAXG], sSC[MAXG];
     __shared__ TGeom sG;
+    extern __shared__ double sDyn[];        // [nsb] P_gain slots of this block, then [nsb] int q_i
 
     const int64_t unit = blockIdx.x;
     const int Ux = d.Nu1 + d.Ny2 * d.Nt2;
     const int xi = x0 + (int)(unit / Ux);
     const int r = (int)(unit % Ux);
     const bool patch1 = r < d.Nu1;
-    const int tc = blockIdx.y * SF_THREADS + threadIdx.x;
+    const int tc0 = blockIdx.y * SF_THREADS;
+    const int tc1 = min(d.NL, tc0 + SF_THREADS);
+    const int tc = tc0 + threadIdx.x;
     const bool own = tc < d.NL;
-    const int NK = d.NK, L = d.L;
+    const int L = d.L;
+    const int ns_all = d.qi_off[d.NL];
+    const int sb0 = d.qi_off[tc0], nsb = d.qi_off[tc1] - sb0;
+    double *sPv = sDyn;
+    int *sQi = reinterpret_cast<int *>(sDyn + d.max_block_slots);
     const double x_i = d.x[xi];
 
-    int l1 = 0, s_begin = 0, s_end = 0;
+    for (int i = threadIdx.x; i < d.Neps; i += SF_THREADS) {
+        sCE[i] = cos(d.eps[i]);
+        sSE[i] = sin(d.eps[i]);
+    }
+    for (int i = threadIdx.x; i < d.Nchi; i += SF_THREADS) {
+        const double c = d.mu_chi[i];
+        sCC[i] = c;
+        sSC[i] = sqrt(fmax(1.0 - c * c, 0.0));
+    }
+    for (int i = threadIdx.x; i < nsb; i += SF_THREADS) sQi[i] = d.qi[sb0 + i];
+
+    int l1 = 0, s_lo = 0, s_hi = 0;
     if (own) {
         l1 = d.trip[3 * tc];
-        s_begin = d.qi_off[tc];
-        s_end = d.qi_off[tc + 1];
+        s_lo = d.qi_off[tc] - sb0;
+        s_hi = d.qi_off[tc + 1] - sb0;
     }
-    double Pg1[MAXK], Pg2[MAXK], Pl1[MAXK], Pl2[MAXK];
-    for (int k = 0; k < MAXK; ++k) Pg1[k] = Pg2[k] = Pl1[k] = Pl2[k] = 0.0;
+    double Pg1[NK], Pg2[NK], Pl1[NK], Pl2[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) Pg1[k] = Pg2[k] = Pl1[k] = Pl2[k] = 0.0;
 
     // unit-level quantities
     double v_large, v_small, Jac_unit;
@@ -202,14 +256,16 @@ __global__ __launch_bounds__(SF_THREADS) void sumfac_units_kernel(SumfacDev d, i
     // radial tables at v_large / v_small: one lane per (side, l)
     if (threadIdx.x < 2 * (L + 1)) {
         const int side = threadIdx.x / (L + 1), l = threadIdx.x % (L + 1);
-        double tmp[MAXK];
-        radial_all_k(NK, l, side ? v_small : v_large, d.radnorm, tmp);
+        double tmp[NK];
+        radial_all_k<NK>(l, side ? v_small : v_large, d.radnorm, tmp);
+#pragma unroll
         for (int k = 0; k < NK; ++k) sRadU[side][l * MAXK + k] = tmp[k];
     }
     const int n_inner_t = patch1 ? d.Nt1 : 1;
     const int n_gain = d.Nchi * d.Neps;
+    constexpr int HALF = SF_THREADS / 2;   // SH tasks in threads [0, HALF), radial tasks in [HALF, 2 HALF)
     for (int it = 0; it < n_inner_t; ++it) {
-        __syncthreads();  // sG reuse
+        __syncthreads();  // sG / sPv reuse
         if (threadIdx.x == 0) {
             if (patch1) {
                 const double u = d.u1[u_idx], t = d.t1[it];
@@ -229,11 +285,17 @@ __global__ __launch_bounds__(SF_THREADS) void sumfac_units_kernel(SumfacDev d, i
                 make_tgeom(sG, x_i, v_large, v_small, c_beta, s_beta, u_mag, Jac_t, d.alpha);
             }
         }
+        {
+            const double *src = patch1 ? d.Pg1 + ((int64_t)u_idx * d.Nt1 + it) * ns_all + sb0
+                                       : d.Pg2 + (int64_t)y_idx * ns_all + sb0;
+            for (int i = threadIdx.x; i < nsb; i += SF_THREADS) sPv[i] = src[i];
+        }
         __syncthreads();
         // loss (compute_rtensor_sumfac_mex.cpp:167-175 / :292-300)
         if (own) {
-            const double P = patch1 ? d.Pl1[((int64_t)tc * d.Nu1 + u_idx) * d.Nt1 + it] : d.Pl2[(int64_t)tc * d.Ny2 + y_idx];
+            const double P = patch1 ? d.Pl1[((int64_t)u_idx * d.Nt1 + it) * d.NL + tc] : d.Pl2[(int64_t)y_idx * d.NL + tc];
             const double li = sG.loss_weight * P;
+#pragma unroll
             for (int k = 0; k < NK; ++k) {
                 Pl1[k] += sRadU[0][l1 * MAXK + k] * li;
                 Pl2[k] += sRadU[1][l1 * MAXK + k] * li;
@@ -243,28 +305,33 @@ __global__ __launch_bounds__(SF_THREADS) void sumfac_units_kernel(SumfacDev d, i
         for (int p0 = 0; p0 < n_gain; p0 += NPB) {
             const int np = min(NPB, n_gain - p0);
             __syncthreads();
-            // tasks: (point j, side s, kind: 0 = SH order m, 1 = radial degree l), index over m/l in 0..L
-            const int ntask = np * 2 * 2 * (L + 1);
-            for (int task = threadIdx.x; task < ntask; task += SF_THREADS) {
-                const int j = task / (4 * (L + 1));
-                const int rem = task % (4 * (L + 1));
-                const int side = rem / (2 * (L + 1));
-                const int kind = (rem / (L + 1)) % 2;
-                const int ml = rem % (L + 1);
+            if (threadIdx.x < 2 * np) {
+                const int j = threadIdx.x >> 1, side = threadIdx.x & 1;
                 const int p = p0 + j;
                 const int n = p / d.Neps, e = p % d.Neps;
-                const double c_chi = d.mu_chi[n];
-                const double s_chi = sqrt(fmax(1.0 - c_chi * c_chi, 0.0));
-                double vp_mag, theta, phi;
-                post_collision(sG, c_chi, s_chi, cos(d.eps[e]), sin(d.eps[e]), side, vp_mag, theta, phi);
-                if (kind == 0) {
-                    sh_order_m(L, ml, cos(theta), phi, d.shnorm, sY[j][side]);
-                } else {
-                    double tmp[MAXK];
-                    radial_all_k(NK, ml, vp_mag, d.radnorm, tmp);
-                    for (int k = 0; k < NK; ++k) sR[j][side][ml * MAXK + k] = tmp[k];
+                double vp_mag, ct, cp, sp;
+                post_collision(sG, sCC[n], sSC[n], sCE[e], sSE[e], side, vp_mag, ct, cp, sp);
+                sGeo[j][side][0] = vp_mag;
+                sGeo[j][side][1] = ct;
+                sGeo[j][side][2] = cp;
+                sGeo[j][side][3] = sp;
+                if (side == 0) sW[j] = d.W_chi[n] * sG.B_val * sG.Jac_t;
+            }
+            __syncthreads();
+            {
+                const int kind = threadIdx.x / HALF, idx = threadIdx.x % HALF;
+                const int j = idx / (2 * (L + 1)), rem = idx % (2 * (L + 1));
+                const int side = rem / (L + 1), ml = rem % (L + 1);
+                if (j < np) {
+                    if (kind == 0) {
+                        sh_order_m(L, ml, sGeo[j][side][1], sGeo[j][side][2], sGeo[j][side][3], d.shnorm, sY[j][side]);
+                    } else {
+                        double tmp[NK];
+                        radial_all_k<NK>(ml, sGeo[j][side][0], d.radnorm, tmp);
+#pragma unroll
+                        for (int k = 0; k < NK; ++k) sR[j][side][ml * MAXK + k] = tmp[k];
+                    }
                 }
-                if (side == 0 && kind == 0 && ml == 0) sW[j] = d.W_chi[n] * sG.B_val * sG.Jac_t;
             }
             __syncthreads();
             if (own) {
@@ -272,15 +339,15 @@ __global__ __launch_bounds__(SF_THREADS) void sumfac_units_kernel(SumfacDev d, i
                     const int p = p0 + j;
                     const int e = p % d.Neps;
                     double g1 = 0.0, g2 = 0.0;
-                    for (int s = s_begin; s < s_end; ++s) {
-                        const int q = d.qi[s];
-                        const double w = patch1 ? d.Pg1[((int64_t)s * d.Nu1 + u_idx) * d.Nt1 + it]
-                                                : d.Pg2[(int64_t)s * d.Ny2 + y_idx];
+                    for (int s = s_lo; s < s_hi; ++s) {
+                        const int q = sQi[s];
+                        const double w = sPv[s];
                         g1 += sY[j][0][q] * w;
                         g2 += sY[j][1][q] * w;
                     }
                     const double wg1 = g1 * sW[j] * d.W_eps[e];
                     const double wg2 = g2 * sW[j] * d.W_eps[e];
+#pragma unroll
                     for (int k = 0; k < NK; ++k) {
                         Pg1[k] += wg1 * sR[j][0][l1 * MAXK + k];
                         Pg2[k] += wg2 * sR[j][1][l1 * MAXK + k];
@@ -291,6 +358,7 @@ __global__ __launch_bounds__(SF_THREADS) void sumfac_units_kernel(SumfacDev d, i
     }
     if (own) {
         double *o = net + (unit * d.NL + tc) * 2 * NK;
+#pragma unroll
         for (int k = 0; k < NK; ++k) {
             o[k] = Jac_unit * (Pg1[k] - Pl1[k]);
             o[NK + k] = Jac_unit * (Pg2[k] - Pl2[k]);
@@ -351,6 +419,11 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     const int L = T.L, NL = T.n_l, NQ = T.N_Q;
     if (L > MAXL) return set_error(CFK_EUNSUP, "sumfac: L=%d > %d unsupported", L, MAXL);
     if (NK > MAXK || NK < 1) return set_error(CFK_EUNSUP, "sumfac: N_K=%d outside 1..%d", NK, MAXK);
+    if ((int)G.mu_chi.size() > MAXG || (int)G.eps.size() > MAXG)
+        return set_error(CFK_EUNSUP, "sumfac: N_chi and N_eps must be <= %d", MAXG);
+    for (int t = 0; t < NL; ++t)
+        if (T.qi_off[t + 1] - T.qi_off[t] > MAXSLOT)
+            return set_error(CFK_EINVAL, "sumfac: t_chan %d has more than %d valid q_i", t + 1, MAXSLOT);
     SumfacBuffers b;
     const std::vector<double> *gv[14] = {&G.x, &G.W_x, &G.u1, &G.W_u1, &G.t1, &G.W_t1, &G.y2,
                                          &G.W_y2, &G.t2, &G.W_t2, &G.mu_chi, &G.W_chi, &G.eps, &G.W_eps};
@@ -404,6 +477,13 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     d.Pl1 = b.Pl1.p;
     d.Pg2 = b.Pg2.p;
     d.Pl2 = b.Pl2.p;
+    d.max_block_slots = 1;
+    for (int t0 = 0; t0 < NL; t0 += SF_THREADS) {
+        const int t1 = std::min(NL, t0 + SF_THREADS);
+        d.max_block_slots = std::max(d.max_block_slots, T.qi_off[t1] - T.qi_off[t0]);
+    }
+    if ((size_t)d.max_block_slots * (sizeof(double) + sizeof(int)) > 64 * 1024)
+        return set_error(CFK_EUNSUP, "sumfac: %d valid q_i per block exceed the LDS budget", d.max_block_slots);
     const int64_t nR = (int64_t)NK * NK * NK * NL;
     if ((rc = b.R.alloc((size_t)nR))) return rc;
     CFK_HIP(hipMemset(b.R.p, 0, (size_t)nR * sizeof(double)));
@@ -423,7 +503,16 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
         const int64_t n_units = (int64_t)nx * Ux;
         if (n_units > 0x7fffffffLL) return set_error(CFK_EINVAL, "sumfac: too many units per chunk");
         dim3 grid((unsigned)n_units, (unsigned)((NL + SF_THREADS - 1) / SF_THREADS));
-        hipLaunchKernelGGL(sumfac_units_kernel, grid, dim3(SF_THREADS), 0, nullptr, d, xc, n_units, b.net.p, b.rad.p);
+        const size_t dyn = (size_t)d.max_block_slots * (sizeof(double) + sizeof(int));
+        switch (NK) {
+#define X(KK)                                                                                                   \
+    case KK:                                                                                                    \
+        hipLaunchKernelGGL(sumfac_units_kernel<KK>, grid, dim3(SF_THREADS), dyn, nullptr, d, xc, n_units, b.net.p, \
+                           b.rad.p);                                                                            \
+        break;
+            X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+#undef X
+        }
         CFK_HIP(hipGetLastError());
         hipLaunchKernelGGL(sumfac_fold_kernel, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, nullptr, NK, NL, L,
                            b.trip.p, n_units, b.net.p, b.rad.p, b.R.p);
