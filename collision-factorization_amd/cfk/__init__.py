@@ -9,4 +9,4 @@ from .geometry import (build_rtensor, gauss_gen_laguerre, gauss_legendre, real_g
                        sort_t_q1, sumfac_grid_sizes)
 from .mex import (MexError, angular_first_collision_kernel_mex, compute_rtensor_sumfac_mex,  # noqa: F401
                   dense_tensor_kernel_mex, naive_collision_kernel_mex, radial_first_collision_kernel_mex)
-from .plan import CollisionPlan, conservation_project, dense_execute  # noqa: F401
+from .plan import CollisionPlan, MultiPlan, conservation_project, dense_execute  # noqa: F401

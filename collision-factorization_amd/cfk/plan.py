@@ -89,3 +89,34 @@ def dense_execute(Q, f, C, N, B=1, stream=None):
 def conservation_project(Q, B, N_Q, K, layout="natural", stream=None):
     check(lib().cfk_conservation_project(_ptr(Q), int(B), int(N_Q), int(K), LAYOUT[layout], _stream_ptr(stream)))
     return Q
+
+
+class MultiPlan:
+    """Several CollisionPlans (different L, K) executed together: one persistent mixed-precision
+    kernel over heaviest-first work items, captured into a hipGraph and replayed (cfk_multi_*)."""
+
+    def __init__(self, plans):
+        self.plans = list(plans)   # keep the plans alive
+        arr = (_vp * len(self.plans))(*[p._h for p in self.plans])
+        h = _vp()
+        check(lib().cfk_multi_create(arr, len(self.plans), ctypes.byref(h)))
+        self._h = h
+
+    def execute(self, Qs, fs, Bs, stream=None):
+        """Q_g += Q(f_g) for every group g; buffers are device tensors in batch-fastest layout."""
+        n = len(self.plans)
+        Qa = (_vp * n)(*[int(q.data_ptr()) for q in Qs])
+        fa = (_vp * n)(*[int(f.data_ptr()) for f in fs])
+        Ba = (ctypes.c_int64 * n)(*[int(b) for b in Bs])
+        check(lib().cfk_multi_execute(self._h, Qa, fa, Ba, _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cfk_multi_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -152,6 +152,7 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
     for (const Slice &s : tq) work[s.q1] += (double)(s.z1 - s.z0) * K * K + (double)K3;
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    p->sym_row_work = work;
     // device copy: last field = 1 on the slices whose partner block is l = 0 (kept with l2 <= l3, so l2 == 0)
     for (Slice &s : sl) s.q1 = (trip[s.t][1] == 0) ? 1 : 0;
     p->sym_N_G = (int64_t)zq.size();

@@ -62,6 +62,27 @@ struct Tile {
     int kc, nb, pf;
 };
 
+// One group of the batched uneven-size kernel (mixed precision, symmetrized tables, batch-fastest).
+struct MultiGroupDev {
+    const float *fT;
+    double *Q;
+    const struct Slice *slices;
+    const int32_t *row_ptr;
+    const int2 *zq;
+    const float *zg;
+    const double *R;
+    const float *R32;
+    int64_t B;
+    int N_Q, K;
+};
+constexpr int MULTI_MAX_GROUPS = 8;
+struct MultiDesc {
+    MultiGroupDev g[MULTI_MAX_GROUPS];
+    const int4 *items;  // (group, row, batch tile, 0), heaviest first
+    int n_items;
+    int *counter;
+};
+
 // One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.  In the device copy
 // of the symmetrized tables the last field instead flags the precise slices (partner block l = 0)
 // for the mixed-precision kernel; the kernels never need q1 (it is the row being computed).
@@ -99,6 +120,7 @@ struct cfk_plan_s {
     cfk::DevBuf<int2> sym_zq;
     cfk::DevBuf<double> sym_zg, sym_R;
     cfk::DevBuf<float> sym_zg32, sym_R32;
+    std::vector<double> sym_row_work;  // cost model per output row (symmetrized tables)
     // scratch for the natural layout (batch-fastest staging)
     cfk::DevBuf<double> fT, QT;
     cfk::DevBuf<float> fT32;

@@ -53,12 +53,14 @@ struct SparseTables {
 // in the symmetrized device tables: the blocks whose partner is the l=0 block, which carry the
 // large, cancelling Maxwellian-times-perturbation terms), where it runs in FP64 as for T = double.
 // ---------------------------------------------------------------------------------------------
+// One work item of the angular-first contraction: output row `row` for batch tile `tile` (64*NB
+// distributions), lane = threadIdx.x % 64 of a one-wave workgroup.
 template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
-__global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
-                                                  const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
-                                                  const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
-                                                  const T *__restrict__ zg, const double *__restrict__ R,
-                                                  const float *__restrict__ R32, int64_t B, int N_Q, int accumulate) {
+__device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__restrict__ QT,
+                                        const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
+                                        const int2 *__restrict__ zq, const T *__restrict__ zg,
+                                        const double *__restrict__ R, const float *__restrict__ R32, int64_t B,
+                                        int N_Q, int accumulate, int row, int64_t tile) {
     constexpr int K2 = K * K, K3 = K2 * K;
     // Each lane owns NB distributions b_i = (tile*NB + i)*64 + lane, so every wave-uniform value
     // (table entry, Gaunt value, R row) fetched through the scalar path feeds NB FMAs.
@@ -66,11 +68,10 @@ __global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, doub
     bool act[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        const int64_t b = ((int64_t)blockIdx.x * NB + i) * WAVE + threadIdx.x;
+        const int64_t b = (tile * NB + i) * WAVE + (threadIdx.x % WAVE);
         act[i] = b < B;
         bi[i] = act[i] ? b : 0;
     }
-    const int row = row_order[blockIdx.y];
     const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
     const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
     double q[NB][K];
@@ -297,6 +298,67 @@ __global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, doub
             *o = accumulate ? *o + v : v;
         }
     }
+}
+
+template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
+__global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
+                                                  const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
+                                                  const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
+                                                  const T *__restrict__ zg, const double *__restrict__ R,
+                                                  const float *__restrict__ R32, int64_t B, int N_Q, int accumulate) {
+    af_item<T, K, KC, GROUPED, PROJ, NB, PF>(fT, QT, slices, row_ptr, zq, zg, R, R32, B, N_Q, accumulate,
+                                             row_order[blockIdx.y], (int64_t)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched uneven sizes: one persistent kernel for several geometry groups (different L and K).
+// Each wave repeatedly takes the next work item (group, output row, batch tile) from a device
+// counter -- the items are ordered heaviest first by the host's cost model -- and runs the mixed
+// angular-first item for that group's K.  Items own disjoint outputs, so the result does not depend
+// on which wave takes which item.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(WAVE) void multi_af_kernel(MultiDesc md) {
+    for (;;) {
+        int v = 0;
+        if (threadIdx.x == 0) v = atomicAdd(md.counter, 1);
+        const int it = __shfl(v, 0);
+        if (it >= md.n_items) break;
+        const int4 w = md.items[it];
+        const MultiGroupDev &G = md.g[w.x];
+        switch (G.K) {
+#define CFK_MULTI_CASE(KK, KCC)                                                                                    \
+    case KK:                                                                                                       \
+        af_item<float, KK, KCC, true, true, 1, false>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.R32, G.B, \
+                                                      G.N_Q, 1, w.y, (int64_t)w.z);                                \
+        break;
+            CFK_MULTI_CASE(6, 6)
+            CFK_MULTI_CASE(8, 8)
+            CFK_MULTI_CASE(10, 10)
+            CFK_MULTI_CASE(12, 6)
+            CFK_MULTI_CASE(14, 7)
+#undef CFK_MULTI_CASE
+            default:
+                break;
+        }
+    }
+}
+
+__global__ void reset_counter_kernel(int *c) { *c = 0; }
+
+bool multi_supports_k(int K) { return K == 6 || K == 8 || K == 10 || K == 12 || K == 14; }
+
+int launch_multi(const MultiDesc &md, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, md.counter);
+    CFK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(multi_af_kernel, dim3((unsigned)grid), dim3(WAVE), 0, st, md);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+int multi_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, multi_af_kernel, WAVE, 0) != hipSuccess || n < 1) n = 4;
+    return n;
 }
 
 // ---------------------------------------------------------------------------------------------

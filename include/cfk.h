@@ -152,6 +152,26 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *tiles_out
 int cfk_plan_set_timing(cfk_plan plan, int32_t enable);
 int cfk_plan_read_timing(cfk_plan plan, double *total_ms, int64_t *launches);
 
+/* ---------------------------------------------------------------------------------------
+ * Batched uneven sizes: distributions of different (L, K) -- e.g. the cells of a spatial solver --
+ * grouped by geometry, one plan per group, executed together in mixed precision.
+ * ------------------------------------------------------------------------------------- */
+typedef struct cfk_multi_s *cfk_multi;
+
+/* Groups plans (K in {6,8,10,12,14}, symmetrized tables available, same device, at most 8). The
+ * plans must outlive the multi object; their autotuned tiles are not used (the fused kernel has one
+ * tile per K). */
+int cfk_multi_create(const cfk_plan *plans, int32_t n_groups, cfk_multi *out);
+int cfk_multi_destroy(cfk_multi multi);
+
+/* Q_dev[g] += Q(f_dev[g]) for B[g] distributions of group g (batch-fastest layout [K][N_Q][B],
+ * FP64 in and out, mixed precision inside, invariants projected).  One persistent kernel takes
+ * (group, row, batch tile) work items heaviest first from a device counter.  The conversions, the
+ * counter reset and the kernel are captured into a hipGraph on the first call for a given set of
+ * pointers and sizes; later calls with the same set replay the graph on `stream`. */
+int cfk_multi_execute(cfk_multi multi, double *const *Q_dev, const double *const *f_dev, const int64_t *B,
+                      void *stream);
+
 /* dense variant on device: Q_dev[b] += C f_b f_b for B distributions of length N (layout natural). */
 int cfk_dense_execute(double *Q_dev, const double *f_dev, const double *C_dev, int64_t N, int64_t B, void *stream);
 

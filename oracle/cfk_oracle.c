@@ -349,87 +349,120 @@ static void loss_accum(const sumfac_args *a, double loss_weight, const double *P
     }
 }
 
-/* Contribution of x nodes [i0, i1) accumulated into R_local (compute_rtensor_sumfac_mex.cpp:135-388). */
-static void sumfac_nodes(const sumfac_args *a, int i0, int i1, double *Rl)
+/* Work buffers of one thread. */
+typedef struct {
+    double *Pl1, *Pl2, *Pg1, *Pg2, *Y1, *Y2, *rl, *rs, *rv1, *rv2;
+    size_t nphi;
+} sumfac_ws;
+
+static void ws_alloc(sumfac_ws *w, const sumfac_args *a, int L_max)
 {
-    const double eps_safe = 1e-15;
-    int N_K = a->K_max + 1, N_L = a->N_L, N_Q = a->N_Q;
-    int L_max = (int)lround(sqrt((double)N_Q)) - 1;
-    size_t nphi = (size_t)N_K * N_L, nrad = (size_t)N_K * (L_max + 1);
-    double *Pl1 = calloc(nphi, sizeof(double)), *Pl2 = calloc(nphi, sizeof(double));
-    double *Pg1 = calloc(nphi, sizeof(double)), *Pg2 = calloc(nphi, sizeof(double));
-    double *Y1 = calloc(N_Q, sizeof(double)), *Y2 = calloc(N_Q, sizeof(double));
-    double *rl = calloc(nrad, sizeof(double)), *rs = calloc(nrad, sizeof(double));
-    double *rv1 = calloc(nrad, sizeof(double)), *rv2 = calloc(nrad, sizeof(double));
-    int N_t1 = a->N_t1, N_u1 = a->N_u1, N_y2 = a->N_y2;
-    for (int i = i0; i < i1; ++i) {
-        double x_i = a->x[i];
-        /* Patch 1 */
-        for (int ui = 0; ui < N_u1; ++ui) {
-            double u = a->u1[ui];
-            double v_large = sqrt(x_i / 2.0) * (1.0 + u);
-            double v_small = sqrt(x_i / 2.0) * (1.0 - u);
-            double Jac_u = 0.5 * a->W_x[i] * a->W_u1[ui] * u * exp(-x_i * u * u) *
-                           (v_large * v_large * v_small * v_small);
-            memset(Pl1, 0, nphi * sizeof(double));
-            memset(Pl2, 0, nphi * sizeof(double));
-            memset(Pg1, 0, nphi * sizeof(double));
-            memset(Pg2, 0, nphi * sizeof(double));
-            radial_table(v_large, N_K, L_max, a->RadialNorm, rl);
-            radial_table(v_small, N_K, L_max, a->RadialNorm, rs);
-            for (int ti = 0; ti < N_t1; ++ti) {
-                double t = a->t1[ti];
-                double y = u * t;
-                double Jac_t = a->W_t1[ti] * (4.0 * u * t);
-                double c_beta = 1.0 - 2.0 * y * y;
-                double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, eps_safe));
-                double u_mag = sqrt(2.0 * x_i) * u * sqrt(1.0 + (1.0 - u * u) * t * t);
-                double B_val = pow(u_mag * sqrt(2.0), a->alpha) / pow(x_i, a->alpha / 2.0);
-                double loss_weight = B_val * 2.0 * M_PI * 2.0 * Jac_t;
-                loss_accum(a, loss_weight, a->P_loss_p1, ti + (int64_t)N_t1 * ui, (int64_t)N_t1 * N_u1, rl, rs,
-                           Pl1, Pl2);
-                gain_sweep(a, L_max, v_large, v_small, c_beta, s_beta, B_val, Jac_t, a->P_gain_p1,
-                           ti + (int64_t)N_t1 * ui, (int64_t)N_t1 * N_u1, (int64_t)N_t1 * N_u1 * N_Q, Y1, Y2,
-                           rv1, rv2, Pg1, Pg2);
-            }
-            factor_contract(a, Jac_u, rl, rs, Pg1, Pl1, Pg2, Pl2, Rl);
-        }
-        /* Patch 2 */
-        for (int yi = 0; yi < N_y2; ++yi) {
-            double y = a->y2[yi];
-            double Jac_y = 0.5 * a->W_x[i] * a->W_y2[yi] * y * (4.0 * y);
-            double c_beta = 1.0 - 2.0 * y * y;
-            double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, eps_safe));
-            for (int ti = 0; ti < a->N_t2; ++ti) {
-                double t = a->t2[ti];
-                double u = y * t;
-                double v_large = sqrt(x_i / 2.0) * (1.0 + u);
-                double v_small = sqrt(x_i / 2.0) * (1.0 - u);
-                double Jac_t = a->W_t2[ti] * exp(-x_i * u * u) * (v_large * v_large * v_small * v_small);
-                double u_mag = sqrt(2.0 * x_i) * y * sqrt(t * t + 1.0 - y * y * t * t);
-                double B_val = pow(u_mag * sqrt(2.0), a->alpha) / pow(x_i, a->alpha / 2.0);
-                double loss_weight = B_val * 2.0 * M_PI * 2.0 * Jac_t;
-                memset(Pl1, 0, nphi * sizeof(double));
-                memset(Pl2, 0, nphi * sizeof(double));
-                memset(Pg1, 0, nphi * sizeof(double));
-                memset(Pg2, 0, nphi * sizeof(double));
-                radial_table(v_large, N_K, L_max, a->RadialNorm, rl);
-                radial_table(v_small, N_K, L_max, a->RadialNorm, rs);
-                loss_accum(a, loss_weight, a->P_loss_p2, yi, N_y2, rl, rs, Pl1, Pl2);
-                gain_sweep(a, L_max, v_large, v_small, c_beta, s_beta, B_val, Jac_t, a->P_gain_p2, yi, N_y2,
-                           (int64_t)N_y2 * N_Q, Y1, Y2, rv1, rv2, Pg1, Pg2);
-                factor_contract(a, Jac_y, rl, rs, Pg1, Pl1, Pg2, Pl2, Rl);
-            }
-        }
-    }
-    free(Pl1); free(Pl2); free(Pg1); free(Pg2); free(Y1); free(Y2);
-    free(rl); free(rs); free(rv1); free(rv2);
+    int N_K = a->K_max + 1;
+    size_t nrad = (size_t)N_K * (L_max + 1);
+    w->nphi = (size_t)N_K * a->N_L;
+    w->Pl1 = calloc(w->nphi, sizeof(double)); w->Pl2 = calloc(w->nphi, sizeof(double));
+    w->Pg1 = calloc(w->nphi, sizeof(double)); w->Pg2 = calloc(w->nphi, sizeof(double));
+    w->Y1 = calloc(a->N_Q, sizeof(double)); w->Y2 = calloc(a->N_Q, sizeof(double));
+    w->rl = calloc(nrad, sizeof(double)); w->rs = calloc(nrad, sizeof(double));
+    w->rv1 = calloc(nrad, sizeof(double)); w->rv2 = calloc(nrad, sizeof(double));
 }
 
-/* R_out[N_K,N_K,N_K,N_L] += contribution of x nodes [x_begin, x_end).  With nthreads > 1 each
- * thread takes a contiguous block of nodes and the per-thread partial tensors are summed in
- * thread order (deterministic for a given thread count; the reference merges under an OpenMP
- * critical section in arrival order, compute_rtensor_sumfac_mex.cpp:391-396). */
+static void ws_free(sumfac_ws *w)
+{
+    free(w->Pl1); free(w->Pl2); free(w->Pg1); free(w->Pg2); free(w->Y1); free(w->Y2);
+    free(w->rl); free(w->rs); free(w->rv1); free(w->rv2);
+}
+
+/* Patch-1 unit (x node i, u node ui): compute_rtensor_sumfac_mex.cpp:141-261. */
+static void patch1_unit(const sumfac_args *a, int L_max, int i, int ui, sumfac_ws *w, double *Rl)
+{
+    const double eps_safe = 1e-15;
+    int N_K = a->K_max + 1, N_t1 = a->N_t1, N_u1 = a->N_u1, N_Q = a->N_Q;
+    double x_i = a->x[i];
+    double u = a->u1[ui];
+    double v_large = sqrt(x_i / 2.0) * (1.0 + u);
+    double v_small = sqrt(x_i / 2.0) * (1.0 - u);
+    double Jac_u = 0.5 * a->W_x[i] * a->W_u1[ui] * u * exp(-x_i * u * u) * (v_large * v_large * v_small * v_small);
+    memset(w->Pl1, 0, w->nphi * sizeof(double));
+    memset(w->Pl2, 0, w->nphi * sizeof(double));
+    memset(w->Pg1, 0, w->nphi * sizeof(double));
+    memset(w->Pg2, 0, w->nphi * sizeof(double));
+    radial_table(v_large, N_K, L_max, a->RadialNorm, w->rl);
+    radial_table(v_small, N_K, L_max, a->RadialNorm, w->rs);
+    for (int ti = 0; ti < N_t1; ++ti) {
+        double t = a->t1[ti];
+        double y = u * t;
+        double Jac_t = a->W_t1[ti] * (4.0 * u * t);
+        double c_beta = 1.0 - 2.0 * y * y;
+        double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, eps_safe));
+        double u_mag = sqrt(2.0 * x_i) * u * sqrt(1.0 + (1.0 - u * u) * t * t);
+        double B_val = pow(u_mag * sqrt(2.0), a->alpha) / pow(x_i, a->alpha / 2.0);
+        double loss_weight = B_val * 2.0 * M_PI * 2.0 * Jac_t;
+        loss_accum(a, loss_weight, a->P_loss_p1, ti + (int64_t)N_t1 * ui, (int64_t)N_t1 * N_u1, w->rl, w->rs, w->Pl1,
+                   w->Pl2);
+        gain_sweep(a, L_max, v_large, v_small, c_beta, s_beta, B_val, Jac_t, a->P_gain_p1, ti + (int64_t)N_t1 * ui,
+                   (int64_t)N_t1 * N_u1, (int64_t)N_t1 * N_u1 * N_Q, w->Y1, w->Y2, w->rv1, w->rv2, w->Pg1, w->Pg2);
+    }
+    factor_contract(a, Jac_u, w->rl, w->rs, w->Pg1, w->Pl1, w->Pg2, w->Pl2, Rl);
+}
+
+/* Patch-2 row (x node i, y node yi, all inner t): compute_rtensor_sumfac_mex.cpp:267-386. */
+static void patch2_row(const sumfac_args *a, int L_max, int i, int yi, sumfac_ws *w, double *Rl)
+{
+    const double eps_safe = 1e-15;
+    int N_K = a->K_max + 1, N_y2 = a->N_y2, N_Q = a->N_Q;
+    double x_i = a->x[i];
+    double y = a->y2[yi];
+    double Jac_y = 0.5 * a->W_x[i] * a->W_y2[yi] * y * (4.0 * y);
+    double c_beta = 1.0 - 2.0 * y * y;
+    double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, eps_safe));
+    for (int ti = 0; ti < a->N_t2; ++ti) {
+        double t = a->t2[ti];
+        double u = y * t;
+        double v_large = sqrt(x_i / 2.0) * (1.0 + u);
+        double v_small = sqrt(x_i / 2.0) * (1.0 - u);
+        double Jac_t = a->W_t2[ti] * exp(-x_i * u * u) * (v_large * v_large * v_small * v_small);
+        double u_mag = sqrt(2.0 * x_i) * y * sqrt(t * t + 1.0 - y * y * t * t);
+        double B_val = pow(u_mag * sqrt(2.0), a->alpha) / pow(x_i, a->alpha / 2.0);
+        double loss_weight = B_val * 2.0 * M_PI * 2.0 * Jac_t;
+        memset(w->Pl1, 0, w->nphi * sizeof(double));
+        memset(w->Pl2, 0, w->nphi * sizeof(double));
+        memset(w->Pg1, 0, w->nphi * sizeof(double));
+        memset(w->Pg2, 0, w->nphi * sizeof(double));
+        radial_table(v_large, N_K, L_max, a->RadialNorm, w->rl);
+        radial_table(v_small, N_K, L_max, a->RadialNorm, w->rs);
+        loss_accum(a, loss_weight, a->P_loss_p2, yi, N_y2, w->rl, w->rs, w->Pl1, w->Pl2);
+        gain_sweep(a, L_max, v_large, v_small, c_beta, s_beta, B_val, Jac_t, a->P_gain_p2, yi, N_y2,
+                   (int64_t)N_y2 * N_Q, w->Y1, w->Y2, w->rv1, w->rv2, w->Pg1, w->Pg2);
+        factor_contract(a, Jac_y, w->rl, w->rs, w->Pg1, w->Pl1, w->Pg2, w->Pl2, Rl);
+    }
+}
+
+/* Contribution of work units [w0, w1) accumulated into R_local, where the units of x node i are, in
+ * the reference's order, its N_u1 patch-1 units followed by its N_y2 patch-2 rows. */
+static void sumfac_units(const sumfac_args *a, int x_begin, int64_t w0, int64_t w1, double *Rl)
+{
+    int L_max = (int)lround(sqrt((double)a->N_Q)) - 1;
+    int per_x = a->N_u1 + a->N_y2;
+    sumfac_ws w;
+    ws_alloc(&w, a, L_max);
+    for (int64_t wu = w0; wu < w1; ++wu) {
+        int i = x_begin + (int)(wu / per_x);
+        int r = (int)(wu % per_x);
+        if (r < a->N_u1)
+            patch1_unit(a, L_max, i, r, &w, Rl);
+        else
+            patch2_row(a, L_max, i, r - a->N_u1, &w, Rl);
+    }
+    ws_free(&w);
+}
+
+/* R_out[N_K,N_K,N_K,N_L] += contribution of x nodes [x_begin, x_end).  The work units (per x node:
+ * patch-1 u units, then patch-2 y rows) are split into nthreads contiguous blocks; each thread sums
+ * its block into a partial tensor in the reference's order and the partials are added in thread order
+ * (deterministic for a given thread count; with one thread the order is the reference's single-thread
+ * order).  The reference splits over x nodes only and merges under an OpenMP critical section in
+ * arrival order (compute_rtensor_sumfac_mex.cpp:126-134, :391-396). */
 void oracle_sumfac(double *R_out, int K_max, int N_L, int N_Q, double alpha,
                    const double *x, const double *W_x, int N_x,
                    const double *u1, const double *W_u1, int N_u1,
@@ -452,15 +485,16 @@ void oracle_sumfac(double *R_out, int K_max, int N_L, int N_Q, double alpha,
     if (nthreads < 1) nthreads = 1;
     int nx = x_end - x_begin;
     if (nx <= 0) return;
-    if (nthreads > nx) nthreads = nx;
+    int64_t nunits = (int64_t)nx * (N_u1 + N_y2);
+    if (nthreads > nunits) nthreads = (int)nunits;
     double *parts = calloc(nR * (size_t)nthreads, sizeof(double));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static, 1) num_threads(nthreads)
 #endif
     for (int th = 0; th < nthreads; ++th) {
-        int c0 = x_begin + (int)((int64_t)nx * th / nthreads);
-        int c1 = x_begin + (int)((int64_t)nx * (th + 1) / nthreads);
-        sumfac_nodes(&a, c0, c1, parts + nR * th);
+        int64_t c0 = nunits * th / nthreads;
+        int64_t c1 = nunits * (th + 1) / nthreads;
+        sumfac_units(&a, x_begin, c0, c1, parts + nR * th);
     }
     for (int th = 0; th < nthreads; ++th)
         for (size_t i = 0; i < nR; ++i) R_out[i] += parts[nR * th + i];
