@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor"],
+                    help="hs-l8-k10: the metric; uneven: 4096 distributions with L in {4,6,8,12}, K=L+2 "
+                         "(batched persistent kernel); rtensor: GPU tensor generation, L=12, K=12, 48^7 grids")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
@@ -67,6 +70,11 @@ def main():
     import torch
     import cfk
     from cfk.inputs import perturbed_maxwellians
+
+    if args.workload == "uneven":
+        return bench_uneven(args, torch, cfk, perturbed_maxwellians)
+    if args.workload == "rtensor":
+        return bench_rtensor(args, cfk)
 
     L, K_max, gamma, B = 8, 9, 1.0, args.batch
     K = K_max + 1
@@ -161,6 +169,90 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ls, vs, ics, R, nq, K, f_host, args.cpu_seconds)
     plan.close()
+    print(json.dumps(out))
+
+
+def bench_uneven(args, torch, cfk, perturbed_maxwellians):
+    """4096 hard-sphere distributions with L drawn uniformly from {4,6,8,12}, K = L+2, one multi plan."""
+    B = args.batch
+    rng = np.random.Generator(np.random.PCG64(7))
+    Ls = [4, 6, 8, 12]
+    counts = np.bincount(rng.integers(0, 4, size=B), minlength=4)
+    plans, fs, Qs, refs, F = [], [], [], [], 0.0
+    setup0 = time.perf_counter()
+    for i, L in enumerate(Ls):
+        K = L + 2
+        nq = (L + 1) ** 2
+        lab, val, ic = cfk.real_gaunt(L)
+        ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
+        R = cfk.build_rtensor(K - 1, L, 1.0, cfk.sumfac_grid_sizes(K - 1, L, 0, 0), geometry=(lab, val, ic))
+        plan = cfk.CollisionPlan(ls, vs, ics, R, nq, K)
+        n = int(counts[i])
+        f = torch.from_numpy(perturbed_maxwellians(L, K, n, seed=10 + i)).cuda().reshape(n, -1).t().contiguous()
+        ref = torch.zeros_like(f)
+        plan.execute(ref, f, n, precision="fp64", layout="batch_fastest")
+        plans.append(plan)
+        fs.append(f)
+        Qs.append(torch.zeros_like(f))
+        refs.append(ref)
+        F += n * work_model(ls, ics, K)[0]
+    setup = time.perf_counter() - setup0
+    mp = cfk.MultiPlan(plans)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        for Q in Qs:
+            Q.zero_()
+        mp.execute(Qs, fs, list(counts), stream=stream)
+
+    def separate():
+        for p, Q, f, n in zip(plans, Qs, fs, counts):
+            Q.zero_()
+            p.execute(Q, f, int(n), precision="mixed", layout="batch_fastest", stream=stream)
+
+    for fn in (step, separate):
+        for _ in range(args.warmup):
+            fn()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        separate()
+    torch.cuda.synchronize()
+    t_sep = (time.perf_counter() - ts) / args.steps
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - ts) / args.steps
+    errs = {}
+    for L, Q, ref in zip(Ls, Qs, refs):
+        errs["L%d" % L] = float(((Q - ref).abs().amax(dim=0) / ref.abs().amax(dim=0)).max())
+    out = {"metric": "Q(f) evals/sec, batched uneven sizes (L in {4,6,8,12}, K=L+2), batch 4096",
+           "value": B / t, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "uneven L in {4,6,8,12}, K=L+2, hard spheres, batch %d" % B,
+                      "counts": [int(c) for c in counts]},
+           "max_rel_err_vs_fp64_per_L": errs, "separate_launches_evals_per_s": B / t_sep,
+           "algorithmic_tflops": F / t / 1e12, "setup_s": setup}
+    print(json.dumps(out))
+
+
+def bench_rtensor(args, cfk):
+    """GPU tensor generation at L=12, K=12 radial modes, 48 nodes per axis, gamma = 0 and 1."""
+    L, K_max = 12, 11
+    geo = cfk.real_gaunt(L)
+    res = {}
+    for gamma in (0.0, 1.0):
+        t0 = time.perf_counter()
+        R = cfk.build_rtensor(K_max, L, gamma, (48,) * 7, geometry=geo)
+        res["gamma%d_s" % int(gamma)] = time.perf_counter() - t0
+        res["gamma%d_max_abs_R" % int(gamma)] = float(np.max(np.abs(R)))
+    gain_points = 48 * (48 * 48 + 48 * 48) * 48 * 48
+    out = {"metric": "tensor generation seconds, L=12, K=12, 48 nodes per axis", "value": res["gamma1_s"],
+           "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "compute_rtensor_sumfac, L=12, K=12, 48^7", "gain_points": gain_points},
+           "gain_points_per_s": gain_points / res["gamma1_s"], **res}
     print(json.dumps(out))
 
 
