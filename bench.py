@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor"],
                     help="hs-l8-k10: the metric; uneven: 4096 distributions with L in {4,6,8,12}, K=L+2 "
-                         "(batched persistent kernel); rtensor: GPU tensor generation, L=12, K=12, 48^7 grids")
+                         "(batched item kernels in a hipGraph); rtensor: GPU tensor generation, L=12, K=12, 48^7 grids")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
@@ -210,20 +210,21 @@ def bench_uneven(args, torch, cfk, perturbed_maxwellians):
             Q.zero_()
             p.execute(Q, f, int(n), precision="mixed", layout="batch_fastest", stream=stream)
 
-    for fn in (step, separate):
+    def timed(fn):
         for _ in range(args.warmup):
             fn()
-    torch.cuda.synchronize()
-    ts = time.perf_counter()
-    for _ in range(args.steps):
-        separate()
-    torch.cuda.synchronize()
-    t_sep = (time.perf_counter() - ts) / args.steps
-    ts = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - ts) / args.steps
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - ts) / args.steps
+
+    t_sep = timed(separate)
+    mp.set_graph(False)
+    t_eager = timed(step)
+    mp.set_graph(True)
+    t = timed(step)
     errs = {}
     for L, Q, ref in zip(Ls, Qs, refs):
         errs["L%d" % L] = float(((Q - ref).abs().amax(dim=0) / ref.abs().amax(dim=0)).max())
@@ -234,6 +235,7 @@ def bench_uneven(args, torch, cfk, perturbed_maxwellians):
            "config": {"workload": "uneven L in {4,6,8,12}, K=L+2, hard spheres, batch %d" % B,
                       "counts": [int(c) for c in counts]},
            "max_rel_err_vs_fp64_per_L": errs, "separate_launches_evals_per_s": B / t_sep,
+           "no_graph_evals_per_s": B / t_eager,
            "algorithmic_tflops": F / t / 1e12, "setup_s": setup}
     print(json.dumps(out))
 

@@ -32,9 +32,16 @@ class CfkError(RuntimeError):
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32
 _vp = ctypes.c_void_p
+
+class RelaxStats(ctypes.Structure):
+    """cfk_relax_stats of include/cfk.h."""
+    _fields_ = [("steps_network", ctypes.c_int64), ("steps_exact", ctypes.c_int64), ("exact_evals", ctypes.c_int64),
+                ("max_residual", ctypes.c_double), ("last_residual", ctypes.c_double)]
+
 
 # (name, argtypes) of every exported entry point of include/cfk.h
 SIGNATURES = {
@@ -61,9 +68,30 @@ SIGNATURES = {
     "cfk_plan_read_timing": ([_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_multi_create": ([ctypes.POINTER(_vp), _i32, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_multi_destroy": ([_vp], ctypes.c_int),
+    "cfk_multi_set_graph": ([_vp, _i32], ctypes.c_int),
     "cfk_multi_execute": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "cfk_dense_execute": ([_vp, _vp, _vp, _i64, _i64, _vp], ctypes.c_int),
     "cfk_conservation_project": ([_vp, _i64, _i32, _i32, _i32, _vp], ctypes.c_int),
+    "cfk_mlp_create": ([_i32, _i32, _i32, _i32, ctypes.c_uint64, ctypes.POINTER(_vp)], ctypes.c_int),
+    "cfk_mlp_destroy": ([_vp], ctypes.c_int),
+    "cfk_mlp_num_params": ([_vp, ctypes.POINTER(_i64)], ctypes.c_int),
+    "cfk_mlp_get_params": ([_vp, _fp], ctypes.c_int),
+    "cfk_mlp_set_params": ([_vp, _fp], ctypes.c_int),
+    "cfk_mlp_get_grads": ([_vp, _fp], ctypes.c_int),
+    "cfk_mlp_forward": ([_vp, _vp, _vp, _i64, _vp], ctypes.c_int),
+    "cfk_mlp_loss_grad": ([_vp, _vp, _vp, _i64, _dp, _vp], ctypes.c_int),
+    "cfk_mlp_adam_step": ([_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp], ctypes.c_int),
+    "cfk_project_maxwellians": ([_vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _vp], ctypes.c_int),
+    "cfk_surrogate_create": ([_vp, _i32, _i32, ctypes.c_uint64, ctypes.POINTER(_vp)], ctypes.c_int),
+    "cfk_surrogate_destroy": ([_vp], ctypes.c_int),
+    "cfk_surrogate_mlp": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
+    "cfk_surrogate_linear_blocks": ([_vp, _dp], ctypes.c_int),
+    "cfk_surrogate_get_normalization": ([_vp, _fp, _fp, _fp, _fp], ctypes.c_int),
+    "cfk_surrogate_fit": ([_vp, _vp, _vp, _i64, _i32, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _dp, _vp],
+                          ctypes.c_int),
+    "cfk_surrogate_apply": ([_vp, _vp, _vp, _i64, _i32, _vp], ctypes.c_int),
+    "cfk_surrogate_relax": ([_vp, _vp, _i64, ctypes.c_double, _i32, _i32, ctypes.c_double, _i32, _dp,
+                             ctypes.POINTER(RelaxStats), _vp], ctypes.c_int),
 }
 
 _LIB = None

@@ -92,8 +92,9 @@ def conservation_project(Q, B, N_Q, K, layout="natural", stream=None):
 
 
 class MultiPlan:
-    """Several CollisionPlans (different L, K) executed together: one persistent mixed-precision
-    kernel over heaviest-first work items, captured into a hipGraph and replayed (cfk_multi_*)."""
+    """Several CollisionPlans (different L, K) executed together in mixed precision: heaviest-first
+    work items, one item kernel per radial size K on parallel graph branches, captured into a
+    hipGraph and replayed (cfk_multi_*)."""
 
     def __init__(self, plans):
         self.plans = list(plans)   # keep the plans alive
@@ -102,13 +103,29 @@ class MultiPlan:
         check(lib().cfk_multi_create(arr, len(self.plans), ctypes.byref(h)))
         self._h = h
 
+    def set_graph(self, use_graph):
+        """Replay a captured hipGraph (True, default) or enqueue the launches directly (False)."""
+        check(lib().cfk_multi_set_graph(self._h, 1 if use_graph else 0))
+
     def execute(self, Qs, fs, Bs, stream=None):
         """Q_g += Q(f_g) for every group g; buffers are device tensors in batch-fastest layout."""
         n = len(self.plans)
+        if len(Qs) != n or len(fs) != n or len(Bs) != n:
+            raise ValueError("expected %d groups" % n)
+        for p, q, f, b in zip(self.plans, Qs, fs, Bs):
+            need = p.N_Q * p.K * int(b)
+            for t in (q, f):
+                if t.dtype != torch_float64() or not t.is_cuda or not t.is_contiguous() or t.numel() < need:
+                    raise ValueError("group buffers must be contiguous float64 device tensors of >= %d values" % need)
         Qa = (_vp * n)(*[int(q.data_ptr()) for q in Qs])
         fa = (_vp * n)(*[int(f.data_ptr()) for f in fs])
         Ba = (ctypes.c_int64 * n)(*[int(b) for b in Bs])
         check(lib().cfk_multi_execute(self._h, Qa, fa, Ba, _stream_ptr(stream)))
+
+
+def torch_float64():
+    import torch
+    return torch.float64
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -76,12 +76,6 @@ struct MultiGroupDev {
     int N_Q, K;
 };
 constexpr int MULTI_MAX_GROUPS = 8;
-struct MultiDesc {
-    MultiGroupDev g[MULTI_MAX_GROUPS];
-    const int4 *items;  // (group, row, batch tile, 0), heaviest first
-    int n_items;
-    int *counter;
-};
 
 // One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.  In the device copy
 // of the symmetrized tables the last field instead flags the precise slices (partner block l = 0)

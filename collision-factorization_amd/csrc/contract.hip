@@ -55,13 +55,42 @@ struct SparseTables {
 // ---------------------------------------------------------------------------------------------
 // One work item of the angular-first contraction: output row `row` for batch tile `tile` (64*NB
 // distributions), lane = threadIdx.x % 64 of a one-wave workgroup.
+// The routing tables and R are read-only for the whole launch and read at wave-uniform addresses.
+// They are read through the constant address space, which lets the compiler fetch them with scalar
+// loads even where their pointers were themselves loaded from memory (the batched kernel's group
+// descriptors) and it could not otherwise rule out aliasing with the output stores.
+// Per-lane data (f in, Q out) is read and written through the global address space, so the
+// compiler emits global rather than flat memory instructions for it even where the pointers were
+// loaded from memory: flat loads also count against the scalar-load counter, and every wait for a
+// batch of table or R values would then wait for the outstanding f loads too.
+// (The host pass of this file only needs the types to exist.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+#else
+template <typename T>
+using cptr = const T *;
+template <typename T>
+using gptr = T *;
+#endif
+
 template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
-__device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__restrict__ QT,
+__device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__restrict__ QT_in,
                                         const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
                                         const int2 *__restrict__ zq, const T *__restrict__ zg,
                                         const double *__restrict__ R, const float *__restrict__ R32, int64_t B,
                                         int N_Q, int accumulate, int row, int64_t tile) {
     constexpr int K2 = K * K, K3 = K2 * K;
+    const gptr<const T> fT = (gptr<const T>)fT_in;
+    const gptr<double> QT = (gptr<double>)QT_in;
+    const cptr<Slice> slices_c = (cptr<Slice>)slices;
+    const cptr<int32_t> row_ptr_c = (cptr<int32_t>)row_ptr;
+    const cptr<int2> zq_c = (cptr<int2>)zq;
+    const cptr<T> zg_c = (cptr<T>)zg;
+    const cptr<double> R_c = (cptr<double>)R;
+    const cptr<float> R32_c = (cptr<float>)R32;
     // Each lane owns NB distributions b_i = (tile*NB + i)*64 + lane, so every wave-uniform value
     // (table entry, Gaunt value, R row) fetched through the scalar path feeds NB FMAs.
     int64_t bi[NB];
@@ -72,7 +101,7 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
         act[i] = b < B;
         bi[i] = act[i] ? b : 0;
     }
-    const int s0 = row_ptr[row], s1 = row_ptr[row + 1];
+    const int s0 = row_ptr_c[row], s1 = row_ptr_c[row + 1];
     const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
     double q[NB][K];
 #pragma unroll
@@ -80,9 +109,9 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
 #pragma unroll
         for (int k = 0; k < K; ++k) q[i][k] = 0.0;
     for (int s = s0; s < s1; ++s) {
-        const Slice sl = slices[s];
-        const double *Rt = R + (int64_t)sl.t * K3;
-        const float *Rt32 = R32 + (int64_t)sl.t * K3;
+        const Slice sl = slices_c[s];
+        const cptr<double> Rt = R_c + (int64_t)sl.t * K3;
+        const cptr<float> Rt32 = R32_c + (int64_t)sl.t * K3;
         const bool precise = !std::is_same<T, float>::value || sl.q1 != 0;
 #pragma unroll 1
         for (int c0 = 0; c0 < K; c0 += KC) {
@@ -109,8 +138,8 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
                     T gv[ZB];
 #pragma unroll
                     for (int e = 0; e < ZB; ++e) {
-                        qv[e] = zq[zb + e];
-                        gv[e] = zg[zb + e];
+                        qv[e] = zq_c[zb + e];
+                        gv[e] = zg_c[zb + e];
                     }
 #pragma unroll
                     for (int e = 0; e < ZB; ++e) {
@@ -129,14 +158,14 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
                                     }
 #pragma unroll
                                 for (int i = 0; i < NB; ++i) {
-                                    const T *f2p = fT + (int64_t)(~qq.x) * B + bi[i];
+                                    const gptr<const T> f2p = fT + (int64_t)(~qq.x) * B + bi[i];
 #pragma unroll
                                     for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = f2p[k2 * kstride];
                                 }
                             }
 #pragma unroll
                             for (int i = 0; i < NB; ++i) {
-                                const T *f3p = fT + (int64_t)qq.y * B + bi[i];
+                                const gptr<const T> f3p = fT + (int64_t)qq.y * B + bi[i];
 #pragma unroll
                                 for (int j = 0; j < KC; ++j)
                                     if (c0 + j < K) h[i][j] = fma(g, f3p[(c0 + j) * kstride], h[i][j]);
@@ -152,13 +181,13 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
                         for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
             } else if (PF) {
                 // one-entry-ahead prefetch of the next nonzero's f rows
-                int2 qn = zq[sl.z0];
-                T gn = zg[sl.z0];
+                int2 qn = zq_c[sl.z0];
+                T gn = zg_c[sl.z0];
                 T f2n[NB][K], f3n[NB][KC];
 #pragma unroll
                 for (int i = 0; i < NB; ++i) {
-                    const T *f2p = fT + (int64_t)qn.x * B + bi[i];
-                    const T *f3p = fT + (int64_t)qn.y * B + bi[i];
+                    const gptr<const T> f2p = fT + (int64_t)qn.x * B + bi[i];
+                    const gptr<const T> f3p = fT + (int64_t)qn.y * B + bi[i];
 #pragma unroll
                     for (int k = 0; k < K; ++k) f2n[i][k] = f2p[k * kstride];
 #pragma unroll
@@ -175,12 +204,12 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
                         for (int j = 0; j < KC; ++j) f3[i][j] = f3n[i][j];
                     }
                     if (z + 1 < sl.z1) {
-                        qn = zq[z + 1];
-                        gn = zg[z + 1];
+                        qn = zq_c[z + 1];
+                        gn = zg_c[z + 1];
 #pragma unroll
                         for (int i = 0; i < NB; ++i) {
-                            const T *f2p = fT + (int64_t)qn.x * B + bi[i];
-                            const T *f3p = fT + (int64_t)qn.y * B + bi[i];
+                            const gptr<const T> f2p = fT + (int64_t)qn.x * B + bi[i];
+                            const gptr<const T> f3p = fT + (int64_t)qn.y * B + bi[i];
 #pragma unroll
                             for (int k = 0; k < K; ++k) f2n[i][k] = f2p[k * kstride];
 #pragma unroll
@@ -200,16 +229,16 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
                     T gv[ZB];
 #pragma unroll
                     for (int e = 0; e < ZB; ++e) {
-                        qv[e] = zq[zb + e];
-                        gv[e] = zg[zb + e];
+                        qv[e] = zq_c[zb + e];
+                        gv[e] = zg_c[zb + e];
                     }
 #pragma unroll
                     for (int e = 0; e < ZB; ++e) {
                         if (zb + e < sl.z1) {
 #pragma unroll
                             for (int i = 0; i < NB; ++i) {
-                                const T *f2p = fT + (int64_t)qv[e].x * B + bi[i];
-                                const T *f3p = fT + (int64_t)qv[e].y * B + bi[i];
+                                const gptr<const T> f2p = fT + (int64_t)qv[e].x * B + bi[i];
+                                const gptr<const T> f3p = fT + (int64_t)qv[e].y * B + bi[i];
                                 T gf2[K];
 #pragma unroll
                                 for (int k2 = 0; k2 < K; ++k2) gf2[k2] = gv[e] * f2p[k2 * kstride];
@@ -234,7 +263,7 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
 #pragma unroll
                 for (int j = 0; j < KC; ++j) {
                     if (c0 + j < K) {
-                        const double *plane = Rt + (c0 + j) * K2;
+                        const cptr<double> plane = Rt + (c0 + j) * K2;
 #pragma unroll
                         for (int e0 = 0; e0 < K2; e0 += RB) {
                             double rr[RB];
@@ -262,7 +291,7 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
 #pragma unroll
                 for (int j = 0; j < KC; ++j) {
                     if (c0 + j < K) {
-                        const float *plane = Rt32 + (c0 + j) * K2;
+                        const cptr<float> plane = Rt32 + (c0 + j) * K2;
 #pragma unroll
                         for (int e0 = 0; e0 < K2; e0 += RB) {
                             float rr[RB];
@@ -294,7 +323,7 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT, double *__rest
         for (int k1 = 0; k1 < K; ++k1) {
             double v = q[i][k1];
             if (PROJ && ((row < 4 && k1 == 0) || (row == 0 && k1 == 1))) v = 0.0;
-            double *o = QT + ((int64_t)k1 * N_Q + row) * B + bi[i];
+            const gptr<double> o = QT + ((int64_t)k1 * N_Q + row) * B + bi[i];
             *o = accumulate ? *o + v : v;
         }
     }
@@ -311,54 +340,46 @@ __global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, doub
 }
 
 // ---------------------------------------------------------------------------------------------
-// Batched uneven sizes: one persistent kernel for several geometry groups (different L and K).
-// Each wave repeatedly takes the next work item (group, output row, batch tile) from a device
-// counter -- the items are ordered heaviest first by the host's cost model -- and runs the mixed
-// angular-first item for that group's K.  Items own disjoint outputs, so the result does not depend
-// on which wave takes which item.
+// Batched uneven sizes: work items (group, output row, batch tile) of several geometry groups, one
+// kernel per radial size K so that each is register-allocated for its own K (a single kernel that
+// switches over K is allocated for the largest case and runs at one wave per SIMD).  The items of a
+// K-class are ordered heaviest first by the host's cost model and workgroup i runs item i: the
+// hardware dispatcher, which hands out workgroups in index order as slots free up, is the work
+// queue.  (A persistent variant whose waves loop over a device counter measured slower: the loop
+// around the item body raises register pressure enough that at K = 14 the compiler serializes the
+// per-lane f loads.)  Items own disjoint outputs, so the result does not depend on which wave runs
+// which item.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(WAVE) void multi_af_kernel(MultiDesc md) {
-    for (;;) {
-        int v = 0;
-        if (threadIdx.x == 0) v = atomicAdd(md.counter, 1);
-        const int it = __shfl(v, 0);
-        if (it >= md.n_items) break;
-        const int4 w = md.items[it];
-        const MultiGroupDev &G = md.g[w.x];
-        switch (G.K) {
-#define CFK_MULTI_CASE(KK, KCC)                                                                                    \
-    case KK:                                                                                                       \
-        af_item<float, KK, KCC, true, true, 1, false>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.R32, G.B, \
-                                                      G.N_Q, 1, w.y, (int64_t)w.z);                                \
-        break;
-            CFK_MULTI_CASE(6, 6)
-            CFK_MULTI_CASE(8, 8)
-            CFK_MULTI_CASE(10, 10)
-            CFK_MULTI_CASE(12, 6)
-            CFK_MULTI_CASE(14, 7)
-#undef CFK_MULTI_CASE
-            default:
-                break;
-        }
-    }
+template <int K, int KC>
+__global__ __launch_bounds__(WAVE) void multi_af_kernel(const MultiGroupDev *__restrict__ groups,
+                                                        const int4 *__restrict__ items, int i0, int i1) {
+    const int it = i0 + (int)blockIdx.x;
+    if (it >= i1) return;
+    const int4 w = items[it];
+    const MultiGroupDev &G = groups[w.x];
+    af_item<float, K, KC, true, true, 1, false>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.R32, G.B, G.N_Q, 1,
+                                                w.y, (int64_t)w.z);
 }
-
-__global__ void reset_counter_kernel(int *c) { *c = 0; }
 
 bool multi_supports_k(int K) { return K == 6 || K == 8 || K == 10 || K == 12 || K == 14; }
 
-int launch_multi(const MultiDesc &md, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, md.counter);
-    CFK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(multi_af_kernel, dim3((unsigned)grid), dim3(WAVE), 0, st, md);
+// Items [i0, i1) of one K-class; KC is K or ceil(K/2) (any other value selects ceil(K/2)).
+int launch_multi(const MultiGroupDev *groups, const int4 *items, int K, int KC, int i0, int i1, hipStream_t st) {
+    if (i1 <= i0) return CFK_OK;
+    const dim3 grid((unsigned)(i1 - i0));
+#define CFK_MULTI_LAUNCH(KK, KCC) \
+    { hipLaunchKernelGGL((multi_af_kernel<KK, KCC>), grid, dim3(WAVE), 0, st, groups, items, i0, i1); }
+    switch (K) {
+        case 6: if (KC == 6) CFK_MULTI_LAUNCH(6, 6) else CFK_MULTI_LAUNCH(6, 3) break;
+        case 8: if (KC == 8) CFK_MULTI_LAUNCH(8, 8) else CFK_MULTI_LAUNCH(8, 4) break;
+        case 10: if (KC == 10) CFK_MULTI_LAUNCH(10, 10) else CFK_MULTI_LAUNCH(10, 5) break;
+        case 12: if (KC == 12) CFK_MULTI_LAUNCH(12, 12) else CFK_MULTI_LAUNCH(12, 6) break;
+        case 14: if (KC == 14) CFK_MULTI_LAUNCH(14, 14) else CFK_MULTI_LAUNCH(14, 7) break;
+        default: return set_error(CFK_EUNSUP, "multi kernel: K=%d", K);
+    }
+#undef CFK_MULTI_LAUNCH
     CFK_HIP(hipGetLastError());
     return CFK_OK;
-}
-
-int multi_blocks_per_cu() {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, multi_af_kernel, WAVE, 0) != hipSuccess || n < 1) n = 4;
-    return n;
 }
 
 // ---------------------------------------------------------------------------------------------

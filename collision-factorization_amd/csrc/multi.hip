@@ -1,5 +1,7 @@
-// Batched uneven-size execution: several plans (different L and K) in one persistent kernel launch,
-// captured once into a hipGraph and replayed (cfk_multi_* in include/cfk.h).
+// Batched uneven-size execution: several plans (different L and K) in one hipGraph, captured once
+// and replayed (cfk_multi_* in include/cfk.h).  The graph converts each group's f to FP32, then forks
+// one item kernel per radial size K onto parallel branches, so the kernels of different K share the
+// GPU and fill each other's tails.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,10 +12,10 @@
 #include "common.hpp"
 
 namespace cfk {
-int launch_multi(const MultiDesc &md, int grid, hipStream_t st);
+int launch_multi(const MultiGroupDev *groups, const int4 *items, int K, int KC, int i0, int i1, hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
-int multi_blocks_per_cu();
 bool multi_supports_k(int K);
+int default_kc(int K);
 }  // namespace cfk
 
 using namespace cfk;
@@ -22,6 +24,8 @@ struct cfk_multi_s {
     std::vector<cfk_plan_s *> plans;
     int device = 0;
     hipStream_t cap = nullptr;
+    hipStream_t branch[MULTI_MAX_GROUPS] = {};   // fork streams used during capture only
+    hipEvent_t ev[MULTI_MAX_GROUPS + 1] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // the pointer/size set the current graph was captured for
@@ -29,10 +33,23 @@ struct cfk_multi_s {
     std::vector<double *> key_Q;
     std::vector<int64_t> key_B;
     DevBuf<int4> items;
-    DevBuf<int> counter;
+    DevBuf<MultiGroupDev> groups;
+    bool use_graph = true;
+    bool prepared = false;
+    struct Launch {
+        int K, KC, i0, i1;
+    };
+    std::vector<Launch> launches;  // one item kernel per K-class
     std::vector<std::unique_ptr<DevBuf<float>>> f32;
     int n_items = 0;
-    ~cfk_multi_s() { reset(); if (cap) (void)hipStreamDestroy(cap); }
+    ~cfk_multi_s() {
+        reset();
+        if (cap) (void)hipStreamDestroy(cap);
+        for (auto &b : branch)
+            if (b) (void)hipStreamDestroy(b);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
     void reset() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -41,62 +58,117 @@ struct cfk_multi_s {
     }
 };
 
-static int multi_capture(cfk_multi_s *m, double *const *Q, const double *const *f, const int64_t *B) {
+static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *f, const int64_t *B) {
     const int G = (int)m->plans.size();
     m->reset();
-    // work items, heaviest first: cost of (group,row) from the plan's cost model, one item per batch tile
+    m->prepared = false;
+    // K-classes: groups that share (K, KC) run in one item kernel
+    struct Cls {
+        int K, KC;
+        double work = 0.0;
+        std::vector<int4> items;
+    };
+    std::vector<Cls> cls;
+    std::vector<int> cls_of(G, -1);
+    for (int g = 0; g < G; ++g) {
+        if (B[g] <= 0) continue;
+        const cfk_plan_s *p = m->plans[g];
+        const int kc = p->kc[CFK_PREC_MIXED] > 0 ? p->kc[CFK_PREC_MIXED] : default_kc(p->K);
+        int c = 0;
+        while (c < (int)cls.size() && !(cls[c].K == p->K && cls[c].KC == kc)) ++c;
+        if (c == (int)cls.size()) cls.push_back(Cls{p->K, kc});
+        cls_of[g] = c;
+    }
+    // work items of each class, heaviest first: cost of (group,row) from the plan's cost model, one
+    // item per 64-distribution batch tile
     struct Row {
         int g, r;
         double w;
     };
-    std::vector<Row> rows;
-    for (int g = 0; g < G; ++g)
-        if (B[g] > 0)
-            for (int r = 0; r < m->plans[g]->N_Q; ++r) rows.push_back(Row{g, r, m->plans[g]->sym_row_work[r]});
-    std::stable_sort(rows.begin(), rows.end(), [](const Row &a, const Row &b) { return a.w > b.w; });
-    std::vector<int4> items;
-    for (const Row &rw : rows) {
-        const int64_t tiles = (B[rw.g] + 63) / 64;
-        for (int64_t t = 0; t < tiles; ++t) items.push_back(make_int4(rw.g, rw.r, (int)t, 0));
+    for (int c = 0; c < (int)cls.size(); ++c) {
+        std::vector<Row> rows;
+        for (int g = 0; g < G; ++g)
+            if (cls_of[g] == c)
+                for (int r = 0; r < m->plans[g]->N_Q; ++r) rows.push_back(Row{g, r, m->plans[g]->sym_row_work[r]});
+        std::stable_sort(rows.begin(), rows.end(), [](const Row &a, const Row &b) { return a.w > b.w; });
+        for (const Row &rw : rows) {
+            const int64_t tiles = (B[rw.g] + 63) / 64;
+            for (int64_t t = 0; t < tiles; ++t) cls[c].items.push_back(make_int4(rw.g, rw.r, (int)t, 0));
+            cls[c].work += rw.w * (double)tiles;
+        }
     }
-    if (items.size() > (size_t)0x7fffffff) return set_error(CFK_EINVAL, "too many work items");
+    // heaviest class first: it is launched first and so gets the GPU first
+    std::stable_sort(cls.begin(), cls.end(), [](const Cls &a, const Cls &b) { return a.work > b.work; });
+    std::vector<int4> items;
+    std::vector<int> off(1, 0);
+    for (const Cls &c : cls) {
+        items.insert(items.end(), c.items.begin(), c.items.end());
+        if (items.size() > (size_t)0x7fffffff) return set_error(CFK_EINVAL, "too many work items");
+        off.push_back((int)items.size());
+    }
     m->n_items = (int)items.size();
     int rc;
     if ((rc = m->items.upload(items.data(), items.size()))) return rc;
-    if ((rc = m->counter.ensure(1))) return rc;
     m->f32.clear();
-    MultiDesc md{};
+    std::vector<MultiGroupDev> gd(G);
     for (int g = 0; g < G; ++g) {
         cfk_plan_s *p = m->plans[g];
         const int64_t n = (int64_t)p->N_Q * p->K * B[g];
         m->f32.emplace_back(new DevBuf<float>());
         if ((rc = m->f32.back()->alloc((size_t)std::max<int64_t>(n, 1)))) return rc;
-        md.g[g] = MultiGroupDev{m->f32.back()->p, Q[g], p->sym_slices.p, p->sym_row_ptr.p, p->sym_zq.p,
-                                p->sym_zg32.p, p->sym_R.p, p->sym_R32.p, B[g], p->N_Q, p->K};
+        gd[g] = MultiGroupDev{m->f32.back()->p, Q[g], p->sym_slices.p, p->sym_row_ptr.p, p->sym_zq.p,
+                              p->sym_zg32.p, p->sym_R.p, p->sym_R32.p, B[g], p->N_Q, p->K};
     }
-    md.items = m->items.p;
-    md.n_items = m->n_items;
-    md.counter = m->counter.p;
-    int ncu = 0;
-    CFK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
-    const int grid = std::max(1, std::min(m->n_items, ncu * multi_blocks_per_cu()));
-    CFK_HIP(hipStreamBeginCapture(m->cap, hipStreamCaptureModeThreadLocal));
-    rc = CFK_OK;
-    for (int g = 0; g < G && rc == CFK_OK; ++g)
-        if (B[g] > 0) rc = launch_to_f32(f[g], m->f32[g]->p, (int64_t)m->plans[g]->N_Q * m->plans[g]->K * B[g], m->cap);
-    if (rc == CFK_OK && m->n_items > 0) rc = launch_multi(md, grid, m->cap);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(m->cap, &graph);
-    if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
+    if ((rc = m->groups.upload(gd.data(), gd.size()))) return rc;
+    const int nc = (int)cls.size();
+    m->launches.clear();
+    for (int c = 0; c < nc; ++c) {
+        if (!m->branch[c]) CFK_HIP(hipStreamCreateWithFlags(&m->branch[c], hipStreamNonBlocking));
+        if (!m->ev[c]) CFK_HIP(hipEventCreateWithFlags(&m->ev[c], hipEventDisableTiming));
+        m->launches.push_back(cfk_multi_s::Launch{cls[c].K, cls[c].KC, off[c], off[c + 1]});
     }
-    if (e != hipSuccess) return set_error(CFK_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    m->graph = graph;
-    CFK_HIP(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
+    if (!m->ev[MULTI_MAX_GROUPS]) CFK_HIP(hipEventCreateWithFlags(&m->ev[MULTI_MAX_GROUPS], hipEventDisableTiming));
     m->key_f.assign(f, f + G);
     m->key_Q.assign(Q, Q + G);
     m->key_B.assign(B, B + G);
+    m->prepared = true;
+    return CFK_OK;
+}
+
+// The launch sequence on stream st: conversions, then a fork with one item kernel per K-class on its
+// own branch stream, joined back into st.
+static int multi_enqueue(cfk_multi_s *m, hipStream_t st) {
+    const int G = (int)m->plans.size();
+    int rc = CFK_OK;
+    for (int g = 0; g < G && rc == CFK_OK; ++g)
+        if (m->key_B[g] > 0)
+            rc = launch_to_f32(m->key_f[g], m->f32[g]->p, (int64_t)m->plans[g]->N_Q * m->plans[g]->K * m->key_B[g], st);
+    const int nc = (int)m->launches.size();
+    if (rc || nc == 0) return rc;
+    CFK_HIP(hipEventRecord(m->ev[MULTI_MAX_GROUPS], st));
+    for (int c = 0; c < nc; ++c) {
+        const cfk_multi_s::Launch &l = m->launches[c];
+        CFK_HIP(hipStreamWaitEvent(m->branch[c], m->ev[MULTI_MAX_GROUPS], 0));
+        if ((rc = launch_multi(m->groups.p, m->items.p, l.K, l.KC, l.i0, l.i1, m->branch[c]))) return rc;
+        CFK_HIP(hipEventRecord(m->ev[c], m->branch[c]));
+        CFK_HIP(hipStreamWaitEvent(st, m->ev[c], 0));
+    }
+    return CFK_OK;
+}
+
+static int multi_capture(cfk_multi_s *m) {
+    m->reset();
+    CFK_HIP(hipStreamBeginCapture(m->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = multi_enqueue(m, m->cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(m->cap, &graph);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        return set_error(CFK_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    m->graph = graph;
+    CFK_HIP(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
     return CFK_OK;
 }
 
@@ -124,6 +196,12 @@ int cfk_multi_create(const cfk_plan *plans, int32_t n_groups, cfk_multi *out) {
     return CFK_OK;
 }
 
+int cfk_multi_set_graph(cfk_multi m, int32_t use_graph) {
+    if (!m) return set_error(CFK_EINVAL, "null multi plan");
+    m->use_graph = use_graph != 0;
+    return CFK_OK;
+}
+
 int cfk_multi_destroy(cfk_multi m) {
     delete m;
     return CFK_OK;
@@ -139,11 +217,13 @@ int cfk_multi_execute(cfk_multi m, double *const *Q_dev, const double *const *f_
     int dev = -1;
     CFK_HIP(hipGetDevice(&dev));
     if (dev != m->device) return set_error(CFK_EINVAL, "multi plan lives on device %d, current is %d", m->device, dev);
-    const bool same = m->exec && std::equal(m->key_f.begin(), m->key_f.end(), f_dev) &&
+    const bool same = m->prepared && std::equal(m->key_f.begin(), m->key_f.end(), f_dev) &&
                       std::equal(m->key_Q.begin(), m->key_Q.end(), Q_dev) &&
                       std::equal(m->key_B.begin(), m->key_B.end(), B);
     int rc;
-    if (!same && (rc = multi_capture(m, Q_dev, f_dev, B))) return rc;
+    if (!same && (rc = multi_prepare(m, Q_dev, f_dev, B))) return rc;
+    if (!m->use_graph) return multi_enqueue(m, (hipStream_t)stream);
+    if (!m->exec && (rc = multi_capture(m))) return rc;
     CFK_HIP(hipGraphLaunch(m->exec, (hipStream_t)stream));
     return CFK_OK;
 }

@@ -1,0 +1,514 @@
+// Learned surrogate of the collision operator, its training and its use in explicit relaxation
+// (cfk_surrogate_* in include/cfk.h).  No counterpart in the reference; the exact operator it
+// stands in for is the plan's FP64 contraction (symmetrized tables).
+//
+// Decomposition.  Write a distribution as f = n (e1 + d) with n = f[k=0,q=0] (the density
+// coefficient) and e1 the unit Maxwellian.  Q is quadratic, so
+//   Q(f) = n^2 ( Q(e1) + J d + Q(d) ),     J d = Q(e1 + d) - Q(e1 - d) over 2 (exact, linear).
+// J is the collision operator linearized about an isotropic Maxwellian: it commutes with rotations,
+// so it is block diagonal, one K x K block per l shared by all m.  The surrogate evaluates the
+// linear part exactly in FP64 and learns only the remainder with the network:
+//   Q~(f) = n^2 ( J d + N(d) ),   N(d) ~ r(d) = Q(f)/n^2 - J d,
+// then applies the FP64 conservation projection (the five invariant coefficients of Q set to 0).
+// Network inputs and targets are standardized per feature with statistics of the training set.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
+#include "mlp.hpp"
+#include "prep.hpp"
+
+namespace cfk {
+namespace {
+
+constexpr int ET = 256;
+inline unsigned nb(int64_t n) { return (unsigned)((n + ET - 1) / ET); }
+
+__device__ inline bool invariant_slot(int q, int k) { return (k == 0 && q < 4) || (k == 1 && q == 0); }
+
+// X[i][b] = bf16((d_i - mu[i]) * inv_s[i]), d = f/n - e1, and n_out[b] = n
+__global__ void prep_input_kernel(const double *__restrict__ f, const float *__restrict__ mu,
+                                  const float *__restrict__ inv_s, __bf16 *__restrict__ X, double *__restrict__ n_out,
+                                  int NK, int64_t B) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)NK * B) return;
+    const int i = (int)(t / B);
+    const int64_t b = t % B;
+    const double n = f[b];
+    const double d = f[t] / n - (i == 0 ? 1.0 : 0.0);
+    X[t] = (__bf16)((float)(d - (double)mu[i]) * inv_s[i]);
+    if (i == 0) n_out[b] = n;
+}
+
+// Q[k1][q][b] = n^2 (sum_k J_l[k1][k] d[k][q][b] + Y[k1][q][b] s[..] + mu[..]), invariants zeroed.
+// use_net = 0 keeps the linear part only.
+__global__ void post_kernel(const double *__restrict__ f, const double *__restrict__ nvec, const float *__restrict__ Y,
+                            const float *__restrict__ mu, const float *__restrict__ s, const double *__restrict__ JL,
+                            double *__restrict__ Q, int K, int N_Q, int64_t B, int use_net) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)N_Q * B) return;
+    const int q = (int)(t / B);
+    const int64_t b = t % B;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= q) ++l;
+    const double n = nvec[b];
+    const int64_t ks = (int64_t)N_Q * B;
+    double d[16];
+    for (int k = 0; k < K; ++k) d[k] = f[k * ks + t] / n - ((k == 0 && q == 0) ? 1.0 : 0.0);
+    const double *J = JL + (int64_t)l * K * K;
+    for (int k1 = 0; k1 < K; ++k1) {
+        double v = 0.0;
+        for (int k = 0; k < K; ++k) v = fma(J[k1 * K + k], d[k], v);
+        if (use_net) {
+            const int64_t i = (int64_t)k1 * N_Q + q;
+            v += (double)Y[k1 * ks + t] * (double)s[i] + (double)mu[i];
+        }
+        Q[k1 * ks + t] = invariant_slot(q, k1) ? 0.0 : n * n * v;
+    }
+}
+
+// Training pairs: d = f/n - e1 and r = Q/n^2 - J d (both FP64, [NK][n] like f and Q)
+__global__ void target_kernel(const double *__restrict__ f, const double *__restrict__ Qx, const double *__restrict__ JL,
+                              double *__restrict__ dout, double *__restrict__ rout, int K, int N_Q, int64_t B) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)N_Q * B) return;
+    const int q = (int)(t / B);
+    const int64_t b = t % B;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= q) ++l;
+    const double n = f[b];
+    const int64_t ks = (int64_t)N_Q * B;
+    double d[16];
+    for (int k = 0; k < K; ++k) {
+        d[k] = f[k * ks + t] / n - ((k == 0 && q == 0) ? 1.0 : 0.0);
+        dout[k * ks + t] = d[k];
+    }
+    const double *J = JL + (int64_t)l * K * K;
+    for (int k1 = 0; k1 < K; ++k1) {
+        double v = 0.0;
+        for (int k = 0; k < K; ++k) v = fma(J[k1 * K + k], d[k], v);
+        rout[k1 * ks + t] = Qx[k1 * ks + t] / (n * n) - v;
+    }
+}
+
+// Per-row mean and standard deviation of X [rows][n] (one workgroup per row, two passes, FP64)
+__global__ void row_stats_kernel(const double *__restrict__ X, double *__restrict__ mean, double *__restrict__ sd,
+                                 int64_t n) {
+    __shared__ double red[ET];
+    const double *x = X + (int64_t)blockIdx.x * n;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += ET) s += x[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = ET / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    const double m = red[0] / (double)n;
+    __syncthreads();
+    s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += ET) s += (x[i] - m) * (x[i] - m);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = ET / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        mean[blockIdx.x] = m;
+        sd[blockIdx.x] = sqrt(red[0] / (double)n);
+    }
+}
+
+template <typename TO>
+__global__ void normalize_kernel(const double *__restrict__ X, const float *__restrict__ mu,
+                                 const float *__restrict__ inv_s, TO *__restrict__ out, int rows, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)rows * n) return;
+    const int i = (int)(t / n);
+    out[t] = (TO)((float)(X[t] - (double)mu[i]) * inv_s[i]);
+}
+
+// dst[i][j] = src[i][perm[start + j]] for j < cols (rows x cols), src row length n
+template <typename T>
+__global__ void gather_cols_kernel(const T *__restrict__ src, const int32_t *__restrict__ perm, int64_t start,
+                                   T *__restrict__ dst, int rows, int64_t cols, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)rows * cols) return;
+    const int i = (int)(t / cols);
+    const int64_t j = t % cols;
+    dst[t] = src[(int64_t)i * n + perm[start + j]];
+}
+
+// dst[i][j] = src[i][off + j*stride] (rows x cols), src row length B
+__global__ void gather_strided_kernel(const double *__restrict__ src, double *__restrict__ dst, int rows, int64_t cols,
+                                      int64_t B, int64_t off, int64_t stride) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)rows * cols) return;
+    const int i = (int)(t / cols);
+    const int64_t j = t % cols;
+    dst[t] = src[(int64_t)i * B + off + j * stride];
+}
+
+// acc[0] += sum (Qs - Q[., off + j*stride])^2, acc[1] += sum Qs^2
+__global__ void residual_kernel(const double *__restrict__ Q, const double *__restrict__ Qs, double *__restrict__ acc,
+                                int rows, int64_t cols, int64_t B, int64_t off, int64_t stride) {
+    __shared__ double r0[ET], r1[ET];
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    double a = 0.0, c = 0.0;
+    if (t < (int64_t)rows * cols) {
+        const int i = (int)(t / cols);
+        const int64_t j = t % cols;
+        const double e = Qs[t], v = Q[(int64_t)i * B + off + j * stride];
+        a = (v - e) * (v - e);
+        c = e * e;
+    }
+    r0[threadIdx.x] = a;
+    r1[threadIdx.x] = c;
+    __syncthreads();
+    for (int k = ET / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            r0[threadIdx.x] += r0[threadIdx.x + k];
+            r1[threadIdx.x] += r1[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, r0[0]);
+        atomicAdd(acc + 1, r1[0]);
+    }
+}
+
+__global__ void axpy_kernel(double *__restrict__ c, const double *__restrict__ Q, double dt, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t < n) c[t] = fma(dt, Q[t], c[t]);
+}
+
+}  // namespace
+}  // namespace cfk
+
+using namespace cfk;
+
+struct cfk_surrogate_s {
+    cfk_plan_s *plan = nullptr;
+    int L = 0, K = 0, N_Q = 0, NK = 0;
+    cfk_mlp_s mlp;
+    std::vector<double> JL_host;        // [(L+1)][K][K]
+    DevBuf<double> JL;
+    std::vector<float> h_mu_x, h_is_x, h_mu_y, h_s_y;
+    DevBuf<float> mu_x, is_x, mu_y, s_y;  // input mean / inverse std, target mean / std
+    bool trained = false;
+    // scratch
+    DevBuf<double> nvec, Qx, cs, Qs, acc;
+    int64_t cap = 0;
+
+    int ensure(int64_t B) {
+        if (B <= cap) return CFK_OK;
+        int rc;
+        if ((rc = nvec.alloc(B)) || (rc = Qx.alloc((size_t)NK * B))) return rc;
+        cap = B;
+        return CFK_OK;
+    }
+    int upload_norm() {
+        int rc;
+        if ((rc = mu_x.upload(h_mu_x.data(), NK)) || (rc = is_x.upload(h_is_x.data(), NK)) ||
+            (rc = mu_y.upload(h_mu_y.data(), NK)) || (rc = s_y.upload(h_s_y.data(), NK)))
+            return rc;
+        return CFK_OK;
+    }
+    // exact operator (FP64, symmetrized tables) with the conservation projection: Q = Q(f)
+    int exact(double *Q, const double *f, int64_t B, hipStream_t st) {
+        int rc;
+        if ((rc = fill_f64(Q, 0.0, (int64_t)NK * B, st))) return rc;
+        if ((rc = cfk_plan_execute(plan, CFK_VARIANT_ANGULAR_FIRST, CFK_PREC_FP64_SYM, CFK_LAYOUT_BATCH_FASTEST, Q, f, B,
+                                   st)))
+            return rc;
+        return cfk_conservation_project(Q, B, N_Q, K, CFK_LAYOUT_BATCH_FASTEST, st);
+    }
+    int apply(double *Q, const double *f, int64_t B, bool use_net, hipStream_t st) {
+        int rc;
+        if ((rc = ensure(B))) return rc;
+        if (use_net) {
+            if ((rc = mlp.ensure_batch(B))) return rc;
+            hipLaunchKernelGGL(prep_input_kernel, dim3(nb((int64_t)NK * B)), dim3(ET), 0, st, f, mu_x.p, is_x.p, mlp.X.p,
+                               nvec.p, NK, B);
+            CFK_HIP(hipGetLastError());
+            if ((rc = mlp.forward_bf16(B, false, st))) return rc;
+        } else {
+            hipLaunchKernelGGL(gather_strided_kernel, dim3(nb(B)), dim3(ET), 0, st, f, nvec.p, 1, B, B, (int64_t)0,
+                               (int64_t)1);
+            CFK_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(post_kernel, dim3(nb((int64_t)N_Q * B)), dim3(ET), 0, st, f, nvec.p,
+                           use_net ? mlp.Y.p : nullptr, mu_y.p, s_y.p, JL.p, Q, K, N_Q, B, use_net ? 1 : 0);
+        CFK_HIP(hipGetLastError());
+        return CFK_OK;
+    }
+};
+
+namespace {
+
+// J from the exact operator: column j = (Q(e1 + e_j) - Q(e1 - e_j)) / 2, checked to be block diagonal
+// in (l, m) with one K x K block per l shared by all m.
+int build_linear(cfk_surrogate_s *s) {
+    const int NK = s->NK, K = s->K, N_Q = s->N_Q;
+    const int64_t B = 2 * (int64_t)NK;
+    std::vector<double> f((size_t)NK * B, 0.0);  // batch-fastest [NK][B]
+    for (int j = 0; j < NK; ++j) {
+        f[(size_t)0 * B + j] += 1.0;
+        f[(size_t)0 * B + NK + j] += 1.0;
+        f[(size_t)j * B + j] += 1.0;
+        f[(size_t)j * B + NK + j] -= 1.0;
+    }
+    DevBuf<double> fd, Qd;
+    int rc;
+    if ((rc = fd.upload(f.data(), f.size())) || (rc = Qd.alloc(f.size()))) return rc;
+    if ((rc = fill_f64(Qd.p, 0.0, (int64_t)f.size(), nullptr))) return rc;
+    if ((rc = cfk_plan_execute(s->plan, CFK_VARIANT_ANGULAR_FIRST, CFK_PREC_FP64_SYM, CFK_LAYOUT_BATCH_FASTEST, Qd.p, fd.p,
+                               B, nullptr)))
+        return rc;
+    CFK_HIP(hipDeviceSynchronize());
+    std::vector<double> Q(f.size());
+    CFK_HIP(hipMemcpy(Q.data(), Qd.p, Q.size() * sizeof(double), hipMemcpyDeviceToHost));
+    auto Jat = [&](int i, int j) { return 0.5 * (Q[(size_t)i * B + j] - Q[(size_t)i * B + NK + j]); };
+    double jmax = 0.0;
+    for (int i = 0; i < NK; ++i)
+        for (int j = 0; j < NK; ++j) jmax = std::max(jmax, std::fabs(Jat(i, j)));
+    s->JL_host.assign((size_t)(s->L + 1) * K * K, 0.0);
+    for (int l = 0; l <= s->L; ++l)
+        for (int k1 = 0; k1 < K; ++k1)
+            for (int k = 0; k < K; ++k) s->JL_host[((size_t)l * K + k1) * K + k] = Jat(k1 * N_Q + l * l, k * N_Q + l * l);
+    double dev = 0.0;
+    for (int i = 0; i < NK; ++i)
+        for (int j = 0; j < NK; ++j) {
+            const int qi = i % N_Q, ki = i / N_Q, qj = j % N_Q, kj = j / N_Q;
+            const int l = l_of_q0(qi);
+            const double model = (qi == qj) ? s->JL_host[((size_t)l * K + ki) * K + kj] : 0.0;
+            dev = std::max(dev, std::fabs(Jat(i, j) - model));
+        }
+    if (!(dev <= 1e-9 * jmax))
+        return set_error(CFK_EINVAL, "linearized operator is not block diagonal per l (deviation %.3e of %.3e)", dev, jmax);
+    return s->JL.upload(s->JL_host.data(), s->JL_host.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfk_surrogate_create(cfk_plan plan, int32_t width, int32_t depth, uint64_t seed, cfk_surrogate *out) {
+    if (!out || !plan) return set_error(CFK_EINVAL, "null argument");
+    *out = nullptr;
+    if (plan->K > 16) return set_error(CFK_EUNSUP, "surrogate supports K <= 16 (K=%d)", plan->K);
+    if (!plan->sym_ok) return set_error(CFK_EUNSUP, "plan without symmetrized tables: %s", plan->sym_why.c_str());
+    int rc;
+    if ((rc = check_device())) return rc;
+    std::unique_ptr<cfk_surrogate_s> s(new cfk_surrogate_s());
+    s->plan = plan;
+    s->K = plan->K;
+    s->N_Q = plan->N_Q;
+    s->NK = plan->N_Q * plan->K;
+    s->L = (int)std::lround(std::sqrt((double)plan->N_Q)) - 1;
+    if ((s->L + 1) * (s->L + 1) != s->N_Q) return set_error(CFK_EINVAL, "N_Q=%d is not a square", s->N_Q);
+    if ((rc = s->mlp.create(s->NK, width, depth, s->NK, seed))) return rc;
+    if ((rc = build_linear(s.get()))) return rc;
+    s->h_mu_x.assign(s->NK, 0.0f);
+    s->h_is_x.assign(s->NK, 1.0f);
+    s->h_mu_y.assign(s->NK, 0.0f);
+    s->h_s_y.assign(s->NK, 1.0f);
+    if ((rc = s->upload_norm())) return rc;
+    *out = s.release();
+    return CFK_OK;
+}
+
+int cfk_surrogate_destroy(cfk_surrogate s) {
+    delete s;
+    return CFK_OK;
+}
+
+int cfk_surrogate_mlp(cfk_surrogate s, cfk_mlp *out) {
+    if (!s || !out) return set_error(CFK_EINVAL, "null argument");
+    *out = &s->mlp;
+    return CFK_OK;
+}
+
+int cfk_surrogate_linear_blocks(cfk_surrogate s, double *JL_host) {
+    if (!s || !JL_host) return set_error(CFK_EINVAL, "null argument");
+    std::copy(s->JL_host.begin(), s->JL_host.end(), JL_host);
+    return CFK_OK;
+}
+
+int cfk_surrogate_get_normalization(cfk_surrogate s, float *mu_x, float *inv_s_x, float *mu_y, float *s_y) {
+    if (!s || !mu_x || !inv_s_x || !mu_y || !s_y) return set_error(CFK_EINVAL, "null argument");
+    std::copy(s->h_mu_x.begin(), s->h_mu_x.end(), mu_x);
+    std::copy(s->h_is_x.begin(), s->h_is_x.end(), inv_s_x);
+    std::copy(s->h_mu_y.begin(), s->h_mu_y.end(), mu_y);
+    std::copy(s->h_s_y.begin(), s->h_s_y.end(), s_y);
+    return CFK_OK;
+}
+
+int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev, int64_t n, int32_t epochs,
+                      int32_t batch, float lr, float lr_final, uint64_t seed, double *loss_hist, void *stream) {
+    if (!s || !f_dev || !Q_dev) return set_error(CFK_EINVAL, "null argument");
+    if (n < 2 || n > 0x7fffffff) return set_error(CFK_EINVAL, "n=%lld out of range", (long long)n);
+    if (epochs < 1 || batch < 1 || batch > n) return set_error(CFK_EINVAL, "epochs=%d batch=%d", epochs, batch);
+    if (!(lr > 0.0f) || !(lr_final > 0.0f)) return set_error(CFK_EINVAL, "learning rates must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    const int NK = s->NK;
+    const int64_t tot = (int64_t)NK * n;
+    int rc;
+    // 1. d and r in FP64, per-feature statistics
+    DevBuf<double> d, r, mean, sd;
+    if ((rc = d.alloc(tot)) || (rc = r.alloc(tot)) || (rc = mean.alloc(NK)) || (rc = sd.alloc(NK))) return rc;
+    hipLaunchKernelGGL(target_kernel, dim3(nb((int64_t)s->N_Q * n)), dim3(ET), 0, st, f_dev, Q_dev, s->JL.p, d.p, r.p, s->K,
+                       s->N_Q, n);
+    CFK_HIP(hipGetLastError());
+    std::vector<double> hm(NK), hs(NK);
+    auto stats = [&](const double *X, std::vector<float> &mu_o, std::vector<float> &s_o, bool inverse) -> int {
+        hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)NK), dim3(ET), 0, st, X, mean.p, sd.p, n);
+        CFK_HIP(hipGetLastError());
+        CFK_HIP(hipMemcpyAsync(hm.data(), mean.p, NK * sizeof(double), hipMemcpyDeviceToHost, st));
+        CFK_HIP(hipMemcpyAsync(hs.data(), sd.p, NK * sizeof(double), hipMemcpyDeviceToHost, st));
+        CFK_HIP(hipStreamSynchronize(st));
+        const double smax = *std::max_element(hs.begin(), hs.end());
+        for (int i = 0; i < NK; ++i) {
+            // features that are (near) constant over the set keep unit scale
+            const double sc = (hs[i] > 1e-6 * smax && hs[i] > 0.0) ? hs[i] : 1.0;
+            mu_o[i] = (float)hm[i];
+            s_o[i] = (float)(inverse ? 1.0 / sc : sc);
+        }
+        return CFK_OK;
+    };
+    if ((rc = stats(d.p, s->h_mu_x, s->h_is_x, true)) || (rc = stats(r.p, s->h_mu_y, s->h_s_y, false))) return rc;
+    // target normalization uses 1/s on the device: keep both forms
+    std::vector<float> inv_sy(NK);
+    for (int i = 0; i < NK; ++i) inv_sy[i] = 1.0f / s->h_s_y[i];
+    if ((rc = s->upload_norm())) return rc;
+    DevBuf<float> isy;
+    if ((rc = isy.upload(inv_sy.data(), NK))) return rc;
+    // 2. normalized training arrays
+    DevBuf<__bf16> Xall;
+    DevBuf<float> Tall, Tb;
+    if ((rc = Xall.alloc(tot)) || (rc = Tall.alloc(tot)) || (rc = Tb.alloc((size_t)NK * batch))) return rc;
+    hipLaunchKernelGGL(normalize_kernel<__bf16>, dim3(nb(tot)), dim3(ET), 0, st, d.p, s->mu_x.p, s->is_x.p, Xall.p, NK, n);
+    hipLaunchKernelGGL(normalize_kernel<float>, dim3(nb(tot)), dim3(ET), 0, st, r.p, s->mu_y.p, isy.p, Tall.p, NK, n);
+    CFK_HIP(hipGetLastError());
+    CFK_HIP(hipStreamSynchronize(st));
+    d.release();
+    r.release();
+    // 3. epochs of shuffled minibatches, Adam with a cosine learning-rate schedule
+    if ((rc = s->mlp.ensure_batch(batch))) return rc;
+    std::vector<int32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937_64 rng(seed);
+    DevBuf<int32_t> dperm;
+    if ((rc = dperm.alloc(n))) return rc;
+    const int64_t steps_per_epoch = n / batch;  // the remainder of each shuffle is dropped
+    const int64_t total = steps_per_epoch * epochs;
+    int64_t step = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        CFK_HIP(hipMemcpyAsync(dperm.p, perm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        if ((rc = fill_f64(s->mlp.loss_acc.p, 0.0, 1, st))) return rc;
+        for (int64_t bi = 0; bi < steps_per_epoch; ++bi, ++step) {
+            const int64_t cols = batch;
+            hipLaunchKernelGGL(gather_cols_kernel<__bf16>, dim3(nb((int64_t)NK * cols)), dim3(ET), 0, st, Xall.p, dperm.p,
+                               bi * batch, s->mlp.X.p, NK, cols, n);
+            hipLaunchKernelGGL(gather_cols_kernel<float>, dim3(nb((int64_t)NK * cols)), dim3(ET), 0, st, Tall.p, dperm.p,
+                               bi * batch, Tb.p, NK, cols, n);
+            CFK_HIP(hipGetLastError());
+            if ((rc = s->mlp.forward_bf16(cols, true, st)) || (rc = s->mlp.backward(Tb.p, cols, st))) return rc;
+            const double frac = total > 1 ? (double)step / (double)(total - 1) : 1.0;
+            const float lr_t = (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(M_PI * frac)));
+            if ((rc = s->mlp.adam(lr_t, 0.9f, 0.999f, 1e-8f, st))) return rc;
+        }
+        if (loss_hist) {
+            double h = 0.0;
+            CFK_HIP(hipMemcpyAsync(&h, s->mlp.loss_acc.p, sizeof(double), hipMemcpyDeviceToHost, st));
+            CFK_HIP(hipStreamSynchronize(st));
+            loss_hist[ep] = h / ((double)NK * (double)batch * (double)std::max<int64_t>(steps_per_epoch, 1));
+        }
+    }
+    CFK_HIP(hipStreamSynchronize(st));
+    s->trained = true;
+    return CFK_OK;
+}
+
+int cfk_surrogate_apply(cfk_surrogate s, double *Q_dev, const double *f_dev, int64_t B, int32_t use_network,
+                        void *stream) {
+    if (!s || !Q_dev || !f_dev) return set_error(CFK_EINVAL, "null argument");
+    if (B < 0 || B > 0x7fffffff) return set_error(CFK_EINVAL, "B=%lld out of range", (long long)B);
+    if (B == 0) return CFK_OK;
+    return s->apply(Q_dev, f_dev, B, use_network != 0, (hipStream_t)stream);
+}
+
+int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, int32_t n_steps, int32_t mode, double tol,
+                        int32_t sample_stride, double *residual_hist, cfk_relax_stats *stats, void *stream) {
+    if (!s || !c_dev) return set_error(CFK_EINVAL, "null argument");
+    if (B < 1 || B > 0x7fffffff || n_steps < 0) return set_error(CFK_EINVAL, "B=%lld n_steps=%d", (long long)B, n_steps);
+    if (mode < CFK_RELAX_EXACT || mode > CFK_RELAX_LINEAR) return set_error(CFK_EINVAL, "unknown mode %d", mode);
+    if (mode == CFK_RELAX_SURROGATE && (sample_stride < 1 || !(tol > 0.0)))
+        return set_error(CFK_EINVAL, "sample_stride=%d tol=%g", sample_stride, tol);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = s->ensure(B))) return rc;
+    const int NK = s->NK;
+    const int64_t nsmax = (B + sample_stride - 1) / std::max(sample_stride, 1);
+    if (mode == CFK_RELAX_SURROGATE) {
+        if ((rc = s->cs.ensure((size_t)NK * nsmax)) || (rc = s->Qs.ensure((size_t)NK * nsmax)) || (rc = s->acc.ensure(2)))
+            return rc;
+    }
+    cfk_relax_stats out{};
+    for (int step = 0; step < n_steps; ++step) {
+        double res = -1.0;
+        if (mode == CFK_RELAX_EXACT) {
+            if ((rc = s->exact(s->Qx.p, c_dev, B, st))) return rc;
+            out.exact_evals += B;
+            ++out.steps_exact;
+        } else if (mode == CFK_RELAX_LINEAR) {
+            if ((rc = s->apply(s->Qx.p, c_dev, B, false, st))) return rc;
+            ++out.steps_network;
+        } else {
+            if ((rc = s->apply(s->Qx.p, c_dev, B, true, st))) return rc;
+            // residual check on every sample_stride-th cell, offset rotating with the step
+            const int64_t off = step % sample_stride;
+            const int64_t ns = (B - off + sample_stride - 1) / sample_stride;
+            hipLaunchKernelGGL(gather_strided_kernel, dim3(nb((int64_t)NK * ns)), dim3(ET), 0, st, c_dev, s->cs.p, NK, ns,
+                               B, off, (int64_t)sample_stride);
+            CFK_HIP(hipGetLastError());
+            if ((rc = s->exact(s->Qs.p, s->cs.p, ns, st))) return rc;
+            out.exact_evals += ns;
+            if ((rc = fill_f64(s->acc.p, 0.0, 2, st))) return rc;
+            hipLaunchKernelGGL(residual_kernel, dim3(nb((int64_t)NK * ns)), dim3(ET), 0, st, s->Qx.p, s->Qs.p, s->acc.p, NK,
+                               ns, B, off, (int64_t)sample_stride);
+            CFK_HIP(hipGetLastError());
+            double h[2] = {0.0, 0.0};
+            CFK_HIP(hipMemcpyAsync(h, s->acc.p, sizeof(h), hipMemcpyDeviceToHost, st));
+            CFK_HIP(hipStreamSynchronize(st));
+            res = h[1] > 0.0 ? std::sqrt(h[0] / h[1]) : (h[0] > 0.0 ? INFINITY : 0.0);
+            out.max_residual = std::max(out.max_residual, res);
+            out.last_residual = res;
+            if (!(res <= tol)) {
+                // fallback: the exact operator for every cell of this step
+                if ((rc = s->exact(s->Qx.p, c_dev, B, st))) return rc;
+                out.exact_evals += B;
+                ++out.steps_exact;
+            } else {
+                ++out.steps_network;
+            }
+        }
+        if (residual_hist) residual_hist[step] = res;
+        hipLaunchKernelGGL(axpy_kernel, dim3(nb((int64_t)NK * B)), dim3(ET), 0, st, c_dev, s->Qx.p, dt, (int64_t)NK * B);
+        CFK_HIP(hipGetLastError());
+    }
+    CFK_HIP(hipStreamSynchronize(st));
+    if (stats) *stats = out;
+    return CFK_OK;
+}
+
+}  // extern "C"

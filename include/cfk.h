@@ -159,18 +159,111 @@ int cfk_plan_read_timing(cfk_plan plan, double *total_ms, int64_t *launches);
 typedef struct cfk_multi_s *cfk_multi;
 
 /* Groups plans (K in {6,8,10,12,14}, symmetrized tables available, same device, at most 8). The
- * plans must outlive the multi object; their autotuned tiles are not used (the fused kernel has one
- * tile per K). */
+ * plans must outlive the multi object.  Each plan's autotuned mixed-precision k3 chunk (K or
+ * ceil(K/2)) is used by the kernel of its K. */
 int cfk_multi_create(const cfk_plan *plans, int32_t n_groups, cfk_multi *out);
 int cfk_multi_destroy(cfk_multi multi);
 
 /* Q_dev[g] += Q(f_dev[g]) for B[g] distributions of group g (batch-fastest layout [K][N_Q][B],
- * FP64 in and out, mixed precision inside, invariants projected).  One persistent kernel takes
- * (group, row, batch tile) work items heaviest first from a device counter.  The conversions, the
- * counter reset and the kernel are captured into a hipGraph on the first call for a given set of
- * pointers and sizes; later calls with the same set replay the graph on `stream`. */
+ * FP64 in and out, mixed precision inside, invariants projected).  The work is split into
+ * (group, row, batch tile) items, ordered heaviest first within each radial size K; one kernel per
+ * K runs its items (one workgroup per item, dispatched in that order), the kernels of different K
+ * on parallel graph branches.  The conversions and the kernels are captured into a hipGraph on the
+ * first call for a given set of pointers and sizes; later calls with the same set replay the graph
+ * on `stream`. */
 int cfk_multi_execute(cfk_multi multi, double *const *Q_dev, const double *const *f_dev, const int64_t *B,
                       void *stream);
+
+/* Replay the captured hipGraph (1, the default) or enqueue the same launch sequence directly on
+ * `stream` at every execute (0). */
+int cfk_multi_set_graph(cfk_multi multi, int32_t use_graph);
+
+/* ---------------------------------------------------------------------------------------
+ * Multilayer perceptron of the learned surrogate (no counterpart in the reference): n_in -> width
+ * (GELU) x depth -> n_out, BF16 weights and activations in the products (MFMA), FP32 accumulation,
+ * FP32 master weights and Adam moments.  Device arrays are feature-major, [features][batch].
+ * Parameters are one flat FP32 array, layer by layer: W_l [out][in] row-major, then b_l [out].
+ * ------------------------------------------------------------------------------------- */
+typedef struct cfk_mlp_s *cfk_mlp;
+
+/* Weights seeded He-uniform (hidden) / Glorot-uniform (output), zero biases. */
+int cfk_mlp_create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, uint64_t seed, cfk_mlp *out);
+int cfk_mlp_destroy(cfk_mlp mlp);
+int cfk_mlp_num_params(cfk_mlp mlp, int64_t *n);
+/* Host copies of the FP32 parameters / of the gradients of the last cfk_mlp_loss_grad call.
+ * cfk_mlp_set_params also resets the Adam state. */
+int cfk_mlp_get_params(cfk_mlp mlp, float *host);
+int cfk_mlp_set_params(cfk_mlp mlp, const float *host);
+int cfk_mlp_get_grads(cfk_mlp mlp, float *host);
+/* Y_dev[n_out][B] = network(X_dev[n_in][B]) (FP32 in and out, BF16 inside). */
+int cfk_mlp_forward(cfk_mlp mlp, float *Y_dev, const float *X_dev, int64_t B, void *stream);
+/* Gradients of mean((network(X) - T)^2) over all n_out*B entries; *loss (if not null) receives
+ * the loss (the call then waits for the stream). */
+int cfk_mlp_loss_grad(cfk_mlp mlp, const float *X_dev, const float *T_dev, int64_t B, double *loss, void *stream);
+/* One bias-corrected Adam step with the gradients of the last cfk_mlp_loss_grad call. */
+int cfk_mlp_adam_step(cfk_mlp mlp, float lr, float beta1, float beta2, float eps, void *stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Basis projection of Maxwellian mixtures (tutorial_collisions_spectral_properties.m:100-158):
+ * c_i = sum over the grid of W(v) Psi_i(v) f(v) e^{|v|^2}, with n_rad generalized Gauss-Laguerre
+ * (alpha = 1/2) nodes in |v|^2 (weights halved), n_pol Gauss-Legendre nodes in cos(theta) and n_azi
+ * uniform phi.  params_dev [B][n_comp][7] = (n, ux, uy, uz, Tx, Ty, Tz) per component (n = 0 skips
+ * it); component density n pi^{-3/2} (Tx Ty Tz)^{-1/2} exp(-sum_d (v_d - u_d)^2 / T_d).  Output
+ * c_dev [K][N_Q][B] (batch fastest), FP64.  n_comp <= 3, L <= 12, K <= 16.  Waits for the stream.
+ * ------------------------------------------------------------------------------------- */
+int cfk_project_maxwellians(double *c_dev, const double *params_dev, int64_t B, int32_t n_comp, int32_t L, int32_t K,
+                            int32_t n_rad, int32_t n_pol, int32_t n_azi, void *stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Learned surrogate of the collision operator (no counterpart in the reference).  With
+ * f = n (e1 + d), n = f[k=0,q=0]:  Q~(f) = n^2 (J d + N(d)), J the operator linearized about the
+ * unit Maxwellian e1 (exact, FP64, one K x K block per l), N the MLP trained on the remainder
+ * Q(f)/n^2 - J d; the five invariant coefficients of Q~ are then set to exactly zero.  All device
+ * arrays are batch-fastest [K][N_Q][B] FP64.  Requires n > 0.
+ * ------------------------------------------------------------------------------------- */
+typedef struct cfk_surrogate_s *cfk_surrogate;
+
+/* Builds J from the plan's FP64 operator (2 K N_Q evaluations) and an untrained network
+ * K N_Q -> width x depth -> K N_Q.  The plan must outlive the surrogate. */
+int cfk_surrogate_create(cfk_plan plan, int32_t width, int32_t depth, uint64_t seed, cfk_surrogate *out);
+int cfk_surrogate_destroy(cfk_surrogate s);
+/* The surrogate's network (owned by the surrogate) for parameter I/O. */
+int cfk_surrogate_mlp(cfk_surrogate s, cfk_mlp *out);
+/* Host copy of the linear blocks J_l[k1][k], (L+1) K K values. */
+int cfk_surrogate_linear_blocks(cfk_surrogate s, double *JL_host);
+/* Host copies of the per-feature input mean and inverse std and target mean and std (K N_Q each). */
+int cfk_surrogate_get_normalization(cfk_surrogate s, float *mu_x, float *inv_s_x, float *mu_y, float *s_y);
+/* Trains on n pairs (f_dev, Q_dev = Q(f)): per-feature standardization from the set, `epochs`
+ * passes of shuffled minibatches of `batch` pairs (seeded), Adam with a cosine learning-rate decay
+ * from lr to lr_final.  loss_hist (host, `epochs` values, may be null) receives the mean training
+ * loss of each epoch. */
+int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev, int64_t n, int32_t epochs,
+                      int32_t batch, float lr, float lr_final, uint64_t seed, double *loss_hist, void *stream);
+/* Q_dev = Q~(f_dev) (overwrites).  use_network = 0 evaluates n^2 J d only. */
+int cfk_surrogate_apply(cfk_surrogate s, double *Q_dev, const double *f_dev, int64_t B, int32_t use_network,
+                        void *stream);
+
+#define CFK_RELAX_EXACT 0      /* exact FP64 operator every step */
+#define CFK_RELAX_SURROGATE 1  /* surrogate, sampled residual check, exact fallback */
+#define CFK_RELAX_LINEAR 2     /* n^2 J d only (no network, no check) */
+
+typedef struct cfk_relax_stats {
+    int64_t steps_network;  /* steps advanced with the surrogate (or the linear part) */
+    int64_t steps_exact;    /* steps advanced with the exact operator (mode EXACT, or fallbacks) */
+    int64_t exact_evals;    /* exact distribution evaluations, residual samples included */
+    double max_residual;    /* largest sampled residual (mode SURROGATE) */
+    double last_residual;
+} cfk_relax_stats;
+
+/* n_steps forward-Euler steps c <- c + dt Q(c) of B cells (c_dev in place).  Mode SURROGATE:
+ * every step evaluates Q~ for all cells, the exact operator for every sample_stride-th cell
+ * (offset rotating with the step) and the sampled residual ||Q~ - Q|| / ||Q|| over those cells;
+ * a step whose residual exceeds tol is redone with the exact operator for all cells.  The
+ * conservation projection is applied to every Q used, so mass, momentum and energy coefficients of
+ * c do not change.  residual_hist (host, n_steps values, may be null) receives each step's sampled
+ * residual (-1 when not checked).  Waits for the stream. */
+int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, int32_t n_steps, int32_t mode, double tol,
+                        int32_t sample_stride, double *residual_hist, cfk_relax_stats *stats, void *stream);
 
 /* dense variant on device: Q_dev[b] += C f_b f_b for B distributions of length N (layout natural). */
 int cfk_dense_execute(double *Q_dev, const double *f_dev, const double *C_dev, int64_t N, int64_t B, void *stream);

@@ -1,7 +1,7 @@
 """Batched uneven sizes (cfk_multi_*): distributions with L in {4,6,8,12}, K = L+2, hard spheres,
-mixed precision in one persistent kernel captured in a hipGraph.  Checked per group against the
-plain FP64 GPU operator (itself parity-tested against the oracle), on several graph replays with
-eager work in between.
+mixed precision, one item kernel per K captured in a hipGraph (or enqueued directly).  Checked per
+group against the plain FP64 GPU operator (itself parity-tested against the oracle), on several
+graph replays with eager work in between.
 """
 import numpy as np
 import pytest
@@ -28,7 +28,8 @@ def groups(torch_cuda):
     return out
 
 
-def test_multi_matches_fp64_on_replays(groups, torch_cuda):
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_multi_matches_fp64_on_replays(groups, torch_cuda, use_graph):
     torch = torch_cuda
     Bs = [200, 150, 130, 70]
     fs, Qs, refs = [], [], []
@@ -42,10 +43,11 @@ def test_multi_matches_fp64_on_replays(groups, torch_cuda):
         refs.append(ref)
     torch.cuda.synchronize()
     mp = cfk.MultiPlan([g["plan"] for g in groups])
+    mp.set_graph(use_graph)
     for rep in range(3):
         for Q in Qs:
             Q.zero_()
-        print("multi replay %d" % rep, flush=True)
+        print("multi replay %d graph=%s" % (rep, use_graph), flush=True)
         mp.execute(Qs, fs, Bs)
         torch.cuda.synchronize()
         for g, Q, ref in zip(groups, Qs, refs):
