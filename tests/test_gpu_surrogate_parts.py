@@ -1,0 +1,93 @@
+"""GPU pieces of the learned surrogate against the oracle:
+* Maxwellian-mixture projection kernel (cfk_project_maxwellians) vs oracle/projection.py at 1e-12;
+* the linear blocks J_l of cfk_surrogate_create vs the dense linearization of the oracle
+  angular-first contraction (restated reference MEX) at 1e-12, on the reference's L=4 fixture;
+* the linear-only surrogate n^2 J d vs numpy with those blocks (1e-12) and vs the exact operator
+  for a small perturbation (relative difference of the order of the perturbation).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cfk
+import oracle
+from conftest import load_precalc
+from oracle import geometry as G
+from oracle import projection as pj
+
+pytestmark = pytest.mark.gpu
+
+
+def test_projection_matches_oracle(torch_cuda):
+    L, K = 8, 10
+    params = cfk.maxwellian_mixture_params(6, seed=21)
+    print("projection L=8 K=10, 6 mixtures", flush=True)
+    c = cfk.project_maxwellians(params, L, K).cpu().numpy()
+    co = pj.project(params, L, K)
+    assert np.max(np.abs(c - co)) <= 1e-12 * np.max(np.abs(co))
+    assert np.max(np.abs(c[0] - 1.0)) < 1e-5        # densities scaled so c_000 = 1
+
+
+def _fixture_plan(L=4, K=5):
+    lab, val, ic, R, _ = load_precalc(L, 1.0)
+    ls, vs, ics = G.sort_t_q1(lab, val, ic)
+    return (ls, vs, ics, R), cfk.CollisionPlan(ls, vs, ics, R, (L + 1) ** 2, K)
+
+
+def test_linear_blocks_match_oracle(torch_cuda):
+    L, K = 4, 5
+    nq = (L + 1) ** 2
+    NK = nq * K
+    (ls, vs, ics, R), plan = _fixture_plan(L, K)
+    sur = cfk.Surrogate(plan, width=64, depth=1, seed=0)
+    JL = sur.linear_blocks()
+    # dense J from the oracle: column j = (Q(e1 + e_j) - Q(e1 - e_j)) / 2, MATLAB order q + k*N_Q
+    J = np.zeros((NK, NK))
+    for j in range(NK):
+        fp = np.zeros(NK)
+        fp[0] = 1.0
+        fm = fp.copy()
+        fp[j] += 1.0
+        fm[j] -= 1.0
+        qp = oracle.contract("angular_first", np.zeros(NK), fp, ls, vs, ics, R, nq, K)
+        qm = oracle.contract("angular_first", np.zeros(NK), fm, ls, vs, ics, R, nq, K)
+        J[:, j] = 0.5 * (qp - qm)
+    scale = np.max(np.abs(J))
+    model = np.zeros_like(J)
+    for q in range(nq):
+        l = int(math.isqrt(q))
+        idx = q + nq * np.arange(K)
+        model[np.ix_(idx, idx)] = JL[l]
+    assert np.max(np.abs(J - model)) <= 1e-12 * scale
+
+
+def test_linear_surrogate(torch_cuda):
+    torch = torch_cuda
+    L, K = 4, 5
+    nq = (L + 1) ** 2
+    (ls, vs, ics, R), plan = _fixture_plan(L, K)
+    sur = cfk.Surrogate(plan, width=64, depth=1, seed=0)
+    JL = sur.linear_blocks()
+    rng = np.random.default_rng(5)
+    B = 37
+    f = np.zeros((K * nq, B))
+    f[0] = 1.0 + 0.1 * rng.random(B)
+    f += 1e-4 * rng.standard_normal(f.shape) * f[0]
+    fd = torch.from_numpy(f).cuda()
+    Q = sur.apply(fd, use_network=False).cpu().numpy()
+    n = f[0]
+    d = f / n - np.eye(K * nq)[:, [0]]
+    ref = np.zeros_like(f)
+    for q in range(nq):
+        l = int(math.isqrt(q))
+        idx = q + nq * np.arange(K)
+        ref[idx] = n ** 2 * (JL[l] @ d[idx])
+    ref[[0, 1, 2, 3, nq]] = 0.0
+    assert np.max(np.abs(Q - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # against the exact operator: the remainder is quadratic in the 1e-4 perturbation
+    Qx = torch.zeros_like(fd)
+    plan.execute(Qx, fd, B, precision="fp64", layout="batch_fastest")
+    Qx = Qx.cpu().numpy()
+    Qx[[0, 1, 2, 3, nq]] = 0.0
+    assert np.linalg.norm(Q - Qx) <= 1e-3 * np.linalg.norm(Qx)
