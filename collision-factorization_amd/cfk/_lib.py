@@ -87,7 +87,8 @@ SIGNATURES = {
     "cfk_surrogate_mlp": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_surrogate_linear_blocks": ([_vp, _dp], ctypes.c_int),
     "cfk_surrogate_get_normalization": ([_vp, _fp, _fp, _fp, _fp], ctypes.c_int),
-    "cfk_surrogate_fit": ([_vp, _vp, _vp, _i64, _i32, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _dp, _vp],
+    "cfk_surrogate_fit": ([_vp, _vp, _vp, _i64, _i32, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _i32, _dp,
+                           _vp],
                           ctypes.c_int),
     "cfk_surrogate_apply": ([_vp, _vp, _vp, _i64, _i32, _vp], ctypes.c_int),
     "cfk_surrogate_relax": ([_vp, _vp, _i64, ctypes.c_double, _i32, _i32, ctypes.c_double, _i32, _dp,

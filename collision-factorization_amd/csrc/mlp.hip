@@ -3,22 +3,23 @@
 //
 //   n_in -> width (GELU) -> ... -> width (GELU) -> n_out        (depth hidden layers)
 //
-// Products run on the CDNA4 matrix cores, v_mfma_f32_16x16x32_bf16: BF16 weights and activations,
-// FP32 accumulation.  FP32 master weights, FP32 Adam moments.  Activations are feature-major,
-// [features][batch] with the batch fastest, the layout of the batch-fastest distribution arrays,
-// so no transposes are needed around the network.
+// Every product runs on the CDNA4 matrix cores, v_mfma_f32_16x16x32_bf16: BF16 weights and
+// activations, FP32 accumulation.  FP32 master weights and Adam moments.  Activations are
+// batch-major, [B][features] (feature dimension padded to 32 with zeros), so each product is an NT
+// product whose operands are both contiguous along the reduction: tiles move global -> registers ->
+// LDS as 16-byte vectors, double-buffered.
 //
-// Training step (per layer l, H_0 = X):
-//   Z_l = W_l H_{l-1} + b_l,  H_l = bf16(gelu(Z_l));  Y = W_L H_{L-1} + b_L
-//   loss = mean((Y - T)^2),   dY = bf16(2 (Y - T) / numel)
-//   gW_l = dZ_l H_{l-1}^T,    gb_l = sum_batch dZ_l,   dH_{l-1} = W_l^T dZ_l,
-//   dZ_{l-1} = bf16(dH_{l-1} * gelu'(Z_{l-1}))
+// Training step (per layer l, H_0 = X, rows = batch):
+//   Z_l = H_{l-1} W_l^T + b_l,  H_l = bf16(gelu(Z_l));  Y = H_{L-1} W_L^T + b_L
+//   loss = mean((Y - T)^2),     dY = bf16(2 (Y - T) / numel)
+//   gW_l = dZ_l^T H_{l-1}  (as the NT product of the transposes),  gb_l = column sums of dZ_l,
+//   dH_{l-1} = dZ_l W_l,  dZ_{l-1} = bf16(dH_{l-1} * gelu'(Z_{l-1}))
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <memory>
-#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -29,99 +30,131 @@ namespace cfk {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT = 256;  // threads per GEMM workgroup (4 waves)
-constexpr int BM = 128, BN = 128, BK = 32;
-constexpr int LDK = BK + 8;  // padded LDS row (elements)
+constexpr int GT = 256;  // threads per GEMM workgroup (4 waves, 2 x 2 wave tiles)
+constexpr int BK = 32;   // reduction depth per LDS stage (one MFMA k-step)
+constexpr int LDK = BK + 8;  // padded LDS row (elements): 80 bytes keeps 16-byte rows spread over banks
 
 __device__ inline float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ inline float gelu_grad_f(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// C[M x N] = op(A)[M x K] op(B)[K x N] (+ bias[m]) with row-major storage:
-//   TA = false: A is [M][K] (lda);  TA = true: A is stored [K][M] (lda) and A^T is used.
-//   TB = false: B is [K][N] (ldb);  TB = true: B is stored [N][K] (ldb) and B^T is used.
-// Epilogue: v = acc (+ bias[m] if bias); C (fp32, optional) = v; Cb (bf16, optional) = bf16(gelu(v))
-// when GELU, else bf16(v).  A workgroup computes a 128 x 128 tile, each of its four waves a 64 x 64
-// quarter as 4 x 4 blocks of 16 x 16 MFMA accumulators; K advances 32 per LDS stage.
-//
-// MFMA 16x16x32 operand layout: lane l supplies A[row l%16][8 consecutive k of group l/16] and
-// B[the same 8 k][column l%16]; D[row 4*(l/16)+r][column l%16] comes back in element r.  (Which
-// k values a lane's group covers does not matter for the product as long as A and B agree.)
-template <bool TA, bool TB, bool GELU>
-__global__ __launch_bounds__(GT) void gemm_bf16_kernel(const __bf16 *__restrict__ A, const __bf16 *__restrict__ B,
-                                                       float *__restrict__ C, __bf16 *__restrict__ Cb,
-                                                       const float *__restrict__ bias, int M, int N, int K, int64_t lda,
-                                                       int64_t ldb, int64_t ldc) {
-    __shared__ __align__(16) __bf16 As[BM][LDK];
-    __shared__ __align__(16) __bf16 Bs[BN][LDK];
+// C[M][N] = A[M][K] B[N][K]^T (+ bias[n]).  A BM x BN workgroup tile, four waves of (BM/2) x (BN/2),
+// each a grid of 16 x 16 MFMA accumulators.  MFMA 16x16x32 operands: lane l supplies A[row l%16][8
+// consecutive k of group l/16] and B[column l%16][the same 8 k]; D[row 4*(l/16)+r][column l%16]
+// comes back in element r.
+template <int BM, int BN, bool GELU>
+__global__ __launch_bounds__(GT) void gemm_nt_kernel(const __bf16 *__restrict__ A, const __bf16 *__restrict__ B,
+                                                     float *__restrict__ C, __bf16 *__restrict__ Cb,
+                                                     const float *__restrict__ bias, int M, int N, int K, int64_t lda,
+                                                     int64_t ldb, int64_t ldc, int64_t ldcb) {
+    constexpr int FM = BM / 32, FN = BN / 32;          // MFMA blocks per wave and dimension
+    constexpr int AV = BM * BK / 8 / GT, BV = BN * BK / 8 / GT;  // 16-byte vectors per thread and stage
+    static_assert(AV >= 1 && BV >= 1, "tile too small for the loader");
+    __shared__ __align__(16) __bf16 As[2][BM][LDK];
+    __shared__ __align__(16) __bf16 Bs[2][BN][LDK];
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
-    f32x4 acc[4][4];
+    const int wm = (wave / 2) * (BM / 2), wn = (wave % 2) * (BN / 2);
+    uint4 ra[AV], rb[BV];
+    auto gload = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const __bf16 zero = (__bf16)0.0f;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        // stage op(A)[m0:+128, k0:+32] as As[m][k] and op(B)[k0:+32, n0:+128] as Bs[n][k]; consecutive
-        // threads walk the contiguous dimension of the stored matrix
-        for (int e = threadIdx.x; e < BM * BK; e += GT) {
-            const int m = TA ? e % BM : e / BK;
-            const int k = TA ? e / BM : e % BK;
-            const int gm = m0 + m, gk = k0 + k;
-            __bf16 v = zero;
-            if (gm < M && gk < K) v = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
-            As[m][k] = v;
+        for (int i = 0; i < AV; ++i) {
+            const int v = tid + i * GT, row = v >> 2, kv = (v & 3) * 8;
+            const int gm = m0 + row;
+            ra[i] = gm < M ? *reinterpret_cast<const uint4 *>(A + (int64_t)gm * lda + k0 + kv) : make_uint4(0, 0, 0, 0);
         }
-        for (int e = threadIdx.x; e < BN * BK; e += GT) {
-            const int n = TB ? e / BK : e % BN;
-            const int k = TB ? e % BK : e / BN;
-            const int gn = n0 + n, gk = k0 + k;
-            __bf16 v = zero;
-            if (gn < N && gk < K) v = TB ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
-            Bs[n][k] = v;
+#pragma unroll
+        for (int i = 0; i < BV; ++i) {
+            const int v = tid + i * GT, row = v >> 2, kv = (v & 3) * 8;
+            const int gn = n0 + row;
+            rb[i] = gn < N ? *reinterpret_cast<const uint4 *>(B + (int64_t)gn * ldb + k0 + kv) : make_uint4(0, 0, 0, 0);
         }
-        __syncthreads();
-        const int r = lane % 16, kq = (lane / 16) * 8;
-        bf16x8 af[4], bfr[4];
+    };
+    auto swrite = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(&As[wm + 16 * i + r][kq]);
+        for (int i = 0; i < AV; ++i) {
+            const int v = tid + i * GT;
+            *reinterpret_cast<uint4 *>(&As[buf][v >> 2][(v & 3) * 8]) = ra[i];
+        }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + 16 * j + r][kq]);
+        for (int i = 0; i < BV; ++i) {
+            const int v = tid + i * GT;
+            *reinterpret_cast<uint4 *>(&Bs[buf][v >> 2][(v & 3) * 8]) = rb[i];
+        }
+    };
+    f32x4 acc[FM][FN];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nk = K / BK;
+    gload(0);
+    swrite(0);
+    __syncthreads();
+    const int r = lane % 16, kq = (lane / 16) * 8;
+    for (int s = 0; s < nk; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nk) gload((s + 1) * BK);
+        bf16x8 af[FM], bfr[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(&As[buf][wm + 16 * i + r][kq]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][wn + 16 * j + r][kq]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        // the other buffer was last read in iteration s-1, before the barrier that ended it
+        if (s + 1 < nk) swrite(buf ^ 1);
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < FN; ++j) {
+        const int gn = n0 + wn + 16 * j + lane % 16;
+        if (gn >= N) continue;
+        const float bn = bias ? bias[gn] : 0.0f;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            const int gm = m0 + wm + 16 * i + 4 * (lane / 16) + rr;
-            if (gm >= M) continue;
-            const float bm = bias ? bias[gm] : 0.0f;
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gn = n0 + wn + 16 * j + lane % 16;
-                if (gn < N) {
-                    const float v = acc[i][j][rr] + bm;
+            for (int rr = 0; rr < 4; ++rr) {
+                const int gm = m0 + wm + 16 * i + 4 * (lane / 16) + rr;
+                if (gm < M) {
+                    const float v = acc[i][j][rr] + bn;
                     if (C) C[(int64_t)gm * ldc + gn] = v;
-                    if (Cb) Cb[(int64_t)gm * ldc + gn] = (__bf16)(GELU ? gelu_f(v) : v);
+                    if (Cb) Cb[(int64_t)gm * ldcb + gn] = (__bf16)(GELU ? gelu_f(v) : v);
                 }
             }
-        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Elementwise and reduction kernels of the training step
+// Elementwise, transpose and reduction kernels
 // ---------------------------------------------------------------------------------------------
-__global__ void f32_to_bf16_kernel(const float *__restrict__ x, __bf16 *__restrict__ y, int64_t n) {
+template <typename T>
+__global__ void fill_kernel(T *__restrict__ x, T v, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = (__bf16)x[i];
+    if (i < n) x[i] = v;
+}
+
+// out[c][r] = in[r][c] for r < rows, c < cols; rows <= r < rows_pad are written as zero.
+// 32 x 32 tiles through LDS (coalesced on both sides).
+template <typename TI, typename TO>
+__global__ void transpose_kernel(const TI *__restrict__ in, int64_t ld_in, TO *__restrict__ out, int64_t ld_out,
+                                 int64_t rows, int64_t cols, int64_t rows_pad) {
+    __shared__ float t[32][33];
+    const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 256 threads: 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t r = r0 + k, c = c0 + tx;
+        t[k][tx] = (r < rows && c < cols) ? (float)in[r * ld_in + c] : 0.0f;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t c = c0 + k, r = r0 + tx;
+        if (c < cols && r < rows_pad) out[c * ld_out + r] = (TO)t[tx][k];
+    }
 }
 
 // dY = bf16(2 (Y - T) * scale); *loss += sum (Y - T)^2 (block partial sums, one FP64 atomic each)
@@ -141,7 +174,7 @@ __global__ void mse_grad_kernel(const float *__restrict__ Y, const float *__rest
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && loss) atomicAdd(loss, (double)red[0]);
+    if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
 }
 
 // dZ = bf16(dH * gelu'(Z))
@@ -151,25 +184,20 @@ __global__ void gelu_back_kernel(const float *__restrict__ dH, const float *__re
     if (i < n) dZ[i] = (__bf16)(dH[i] * gelu_grad_f(Z[i]));
 }
 
-// db[r] = sum_c dZ[r][c], one workgroup per row
-__global__ void row_sum_kernel(const __bf16 *__restrict__ dZ, float *__restrict__ db, int64_t cols) {
-    __shared__ float red[256];
-    const int64_t r = blockIdx.x;
+// gb[c] = sum_r dZ[r][c] for c < cols (rows x ld layout), a thread per column
+__global__ void col_sum_kernel(const __bf16 *__restrict__ dZ, int64_t ld, float *__restrict__ gb, int64_t rows,
+                               int cols) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
     float s = 0.f;
-    for (int64_t c = threadIdx.x; c < cols; c += 256) s += (float)dZ[r * cols + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[r] = red[0];
+    for (int64_t r = 0; r < rows; ++r) s += (float)dZ[r * ld + c];
+    gb[c] = s;
 }
 
-// Adam on FP32 master weights (bias-corrected), refreshing the BF16 copy
+// Adam on FP32 master weights (bias-corrected)
 __global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
-                            const float *__restrict__ g, __bf16 *__restrict__ wb, int64_t n, float lr, float b1, float b2,
-                            float eps, float bc1, float bc2) {
+                            const float *__restrict__ g, int64_t n, float lr, float b1, float b2, float eps, float bc1,
+                            float bc2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
@@ -177,57 +205,77 @@ __global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float 
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-    w[i] = wi;
-    wb[i] = (__bf16)wi;
+    w[i] = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
-__global__ void fill_f32_kernel(float *__restrict__ x, float v, int64_t n) {
+// padded BF16 copies of one layer: Wb[r][c] = Wt[c][r] = bf16(w[r*in + c]) for r < out, c < in
+__global__ void refresh_kernel(const float *__restrict__ w, __bf16 *__restrict__ Wb, __bf16 *__restrict__ Wt, int out,
+                               int in, int p_out, int p_in) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
-}
-
-__global__ void fill_f64_kernel(double *__restrict__ x, double v, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
+    if (i >= (int64_t)out * in) return;
+    const int r = (int)(i / in), c = (int)(i % in);
+    const __bf16 b = (__bf16)w[i];
+    Wb[(int64_t)r * p_in + c] = b;
+    Wt[(int64_t)c * p_out + r] = b;
 }
 
 static inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-int gemm_bf16(bool ta, bool tb, bool gelu, const __bf16 *A, const __bf16 *B, float *C, __bf16 *Cb, const float *bias,
-              int M, int N, int K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t st) {
+int gemm_nt(const __bf16 *A, const __bf16 *B, float *C, __bf16 *Cb, const float *bias, bool gelu, int M, int N, int K,
+            int64_t lda, int64_t ldb, int64_t ldc, int64_t ldcb, hipStream_t st) {
     if (M <= 0 || N <= 0) return CFK_OK;
-    if ((M + BM - 1) / BM > 65535) return set_error(CFK_EINVAL, "GEMM too large (M=%d)", M);
-    const dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
-#define CFK_G(TA_, TB_, G_) \
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA_, TB_, G_>), grid, dim3(GT), 0, st, A, B, C, Cb, bias, M, N, K, lda, ldb, ldc)
-    if (gelu) {
-        if (ta || tb) return set_error(CFK_EUNSUP, "GELU epilogue with transposed operands");
-        CFK_G(false, false, true);
-    } else if (!ta && !tb) {
-        CFK_G(false, false, false);
-    } else if (ta && !tb) {
-        CFK_G(true, false, false);
-    } else if (!ta && tb) {
-        CFK_G(false, true, false);
+    if (K % BK || lda % 8 || ldb % 8) return set_error(CFK_EINVAL, "gemm_nt: K=%d lda=%lld ldb=%lld not aligned", K,
+                                                       (long long)lda, (long long)ldb);
+    // 128 x 128 tiles when they fill the GPU twice over, 64 x 64 otherwise
+    const int64_t big = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+    const bool b128 = big >= 512;
+    const int TM = b128 ? 128 : 64, TN = TM;
+    if ((M + TM - 1) / TM > 65535) return set_error(CFK_EINVAL, "gemm_nt: M=%d too large", M);
+    const dim3 grid((unsigned)((N + TN - 1) / TN), (unsigned)((M + TM - 1) / TM));
+#define CFK_G(BM_, G_) \
+    hipLaunchKernelGGL((gemm_nt_kernel<BM_, BM_, G_>), grid, dim3(GT), 0, st, A, B, C, Cb, bias, M, N, K, lda, ldb, ldc, ldcb)
+    if (b128) {
+        if (gelu) CFK_G(128, true); else CFK_G(128, false);
     } else {
-        CFK_G(true, true, false);
+        if (gelu) CFK_G(64, true); else CFK_G(64, false);
     }
 #undef CFK_G
     CFK_HIP(hipGetLastError());
     return CFK_OK;
 }
 
+template <typename TI, typename TO>
+int transpose(const TI *in, int64_t ld_in, TO *out, int64_t ld_out, int64_t rows, int64_t cols, int64_t rows_pad,
+              hipStream_t st) {
+    if (cols <= 0 || rows_pad <= 0) return CFK_OK;
+    const dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((std::max(rows, rows_pad) + 31) / 32));
+    hipLaunchKernelGGL((transpose_kernel<TI, TO>), grid, dim3(256), 0, st, in, ld_in, out, ld_out, rows, cols, rows_pad);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+int transpose_f32(const float *in, int64_t ld_in, float *out, int64_t ld_out, int64_t rows, int64_t cols,
+                  int64_t rows_pad, hipStream_t st) {
+    return transpose(in, ld_in, out, ld_out, rows, cols, rows_pad, st);
+}
+
 int fill_f32(float *x, float v, int64_t n, hipStream_t st) {
     if (n <= 0) return CFK_OK;
-    hipLaunchKernelGGL(fill_f32_kernel, dim3(nblk(n)), dim3(256), 0, st, x, v, n);
+    hipLaunchKernelGGL(fill_kernel<float>, dim3(nblk(n)), dim3(256), 0, st, x, v, n);
     CFK_HIP(hipGetLastError());
     return CFK_OK;
 }
 
 int fill_f64(double *x, double v, int64_t n, hipStream_t st) {
     if (n <= 0) return CFK_OK;
-    hipLaunchKernelGGL(fill_f64_kernel, dim3(nblk(n)), dim3(256), 0, st, x, v, n);
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(nblk(n)), dim3(256), 0, st, x, v, n);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+int fill_bf16(__bf16 *x, int64_t n, hipStream_t st) {
+    if (n <= 0) return CFK_OK;
+    hipLaunchKernelGGL(fill_kernel<__bf16>, dim3(nblk(n)), dim3(256), 0, st, x, (__bf16)0.0f, n);
     CFK_HIP(hipGetLastError());
     return CFK_OK;
 }
@@ -251,69 +299,96 @@ struct SplitMix {
     }
     double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
 };
+
+template <typename T>
+int zalloc(DevBuf<T> &b, size_t n) {  // allocation with zeroed contents (pads must read as zero)
+    int rc = b.alloc(n);
+    if (rc) return rc;
+    if constexpr (std::is_same<T, float>::value) return fill_f32(b.p, 0.0f, (int64_t)n, nullptr);
+    else if constexpr (std::is_same<T, double>::value) return fill_f64(b.p, 0.0, (int64_t)n, nullptr);
+    else return fill_bf16(b.p, (int64_t)n, nullptr);
+}
 }  // namespace
 
 int cfk_mlp_s::ensure_batch(int64_t B) {
     if (B <= cap) return CFK_OK;
     int rc;
-    const size_t w = (size_t)width * B;
-    if ((rc = X.alloc((size_t)n_in * B))) return rc;
+    const int64_t nb = std::max<int64_t>(B, 32);
+    const int64_t nbp = (nb + 31) / 32 * 32;
+    const int pmax = std::max(p_in, std::max(p_w, p_out));
+    CFK_HIP(hipDeviceSynchronize());  // buffers may still be in use by queued work
+    if ((rc = zalloc(X, (size_t)nb * p_in))) return rc;
     Z.clear();
     H.clear();
     for (int l = 0; l < depth; ++l) {
         Z.emplace_back(new DevBuf<float>());
         H.emplace_back(new DevBuf<__bf16>());
-        if ((rc = Z.back()->alloc(w)) || (rc = H.back()->alloc(w))) return rc;
+        if ((rc = zalloc(*Z.back(), (size_t)nb * p_w)) || (rc = zalloc(*H.back(), (size_t)nb * p_w))) return rc;
     }
-    if ((rc = Y.alloc((size_t)n_out * B)) || (rc = dY.alloc((size_t)n_out * B)) || (rc = dH.alloc(w)) ||
-        (rc = dZ.alloc(w)))
+    if ((rc = zalloc(Y, (size_t)nb * p_out)) || (rc = zalloc(T, (size_t)nb * p_out)) ||
+        (rc = zalloc(dY, (size_t)nb * p_out)) || (rc = zalloc(dH, (size_t)nb * p_w)) || (rc = zalloc(dZ, (size_t)nb * p_w)) ||
+        (rc = zalloc(dZt, (size_t)pmax * nbp)) || (rc = zalloc(Ht, (size_t)pmax * nbp)))
         return rc;
-    cap = B;
+    CFK_HIP(hipDeviceSynchronize());
+    cap = nb;
+    bp = nbp;
     return CFK_OK;
 }
 
-// Forward pass on the BF16 input already in X (n_in x B).  train: keep Z for the backward pass.
-int cfk_mlp_s::forward_bf16(int64_t B, bool train, hipStream_t st) {
+int cfk_mlp_s::forward(int64_t B, bool train, hipStream_t st) {
     int rc;
     const __bf16 *in = X.p;
-    int n_prev = n_in;
+    int kin = p_in;
     for (int l = 0; l < depth; ++l) {
-        if ((rc = gemm_bf16(false, false, true, wb.p + w_off[l], in, train ? Z[l]->p : nullptr, H[l]->p, w.p + b_off[l],
-                            width, (int)B, n_prev, n_prev, B, B, st)))
+        if ((rc = gemm_nt(in, Wb[l]->p, train ? Z[l]->p : nullptr, H[l]->p, w.p + b_off[l], true, (int)B, width, kin, kin,
+                          kin, p_w, p_w, st)))
             return rc;
         in = H[l]->p;
-        n_prev = width;
+        kin = p_w;
     }
-    return gemm_bf16(false, false, false, wb.p + w_off[depth], in, Y.p, nullptr, w.p + b_off[depth], n_out, (int)B,
-                     n_prev, n_prev, B, B, st);
+    return gemm_nt(in, Wb[depth]->p, Y.p, nullptr, w.p + b_off[depth], false, (int)B, n_out, kin, kin, kin, p_out, 0, st);
 }
 
-// Gradients of mean((Y - T)^2) for the batch whose forward pass (train = true) left Y and Z.
-int cfk_mlp_s::backward(const float *T, int64_t B, hipStream_t st) {
+int cfk_mlp_s::backward(int64_t B, hipStream_t st) {
     int rc;
-    const int64_t n = (int64_t)n_out * B;
-    hipLaunchKernelGGL(mse_grad_kernel, dim3(nblk(n)), dim3(256), 0, st, Y.p, T, dY.p, loss_acc.p, n, 1.0f / (float)n);
+    const int64_t n = B * p_out;
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(nblk(n)), dim3(256), 0, st, Y.p, T.p, dY.p, loss_acc.p, n,
+                       1.0f / (float)(B * (int64_t)n_out));
     CFK_HIP(hipGetLastError());
-    const __bf16 *dcur = dY.p;  // gradient w.r.t. the pre-activation of layer l
+    const int64_t kb = (B + 31) / 32 * 32;  // reduction length of the weight gradients (zero-padded batch)
+    const __bf16 *dcur = dY.p;
+    int p_o = p_out, n_o = n_out;
     for (int l = depth; l >= 0; --l) {
-        const int n_o = (l == depth) ? n_out : width;
-        const int n_i = (l == 0) ? n_in : width;
+        const int n_i = (l == 0) ? n_in : width, p_i = (l == 0) ? p_in : p_w;
         const __bf16 *hin = (l == 0) ? X.p : H[l - 1]->p;
-        // gW_l = dZ_l H_{l-1}^T   (M = n_o, N = n_i, K = B)
-        if ((rc = gemm_bf16(false, true, false, dcur, hin, g.p + w_off[l], nullptr, nullptr, n_o, n_i, (int)B, B, B, n_i,
-                            st)))
+        // gW_l[o][i] = sum_b dZ[b][o] H[b][i]: NT product of the transposes, written in the flat layout
+        if ((rc = transpose(dcur, p_o, dZt.p, bp, B, n_o, kb, st)) || (rc = transpose(hin, p_i, Ht.p, bp, B, n_i, kb, st)))
             return rc;
-        hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)n_o), dim3(256), 0, st, dcur, g.p + b_off[l], B);
+        if ((rc = gemm_nt(dZt.p, Ht.p, g.p + w_off[l], nullptr, nullptr, false, n_o, n_i, (int)kb, bp, bp, n_i, 0, st)))
+            return rc;
+        hipLaunchKernelGGL(col_sum_kernel, dim3(nblk(n_o)), dim3(256), 0, st, dcur, (int64_t)p_o, g.p + b_off[l], B, n_o);
         CFK_HIP(hipGetLastError());
         if (l == 0) break;
-        // dH_{l-1} = W_l^T dZ_l   (M = n_i, N = B, K = n_o); W_l is stored [n_o][n_i]
-        if ((rc = gemm_bf16(true, false, false, wb.p + w_off[l], dcur, dH.p, nullptr, nullptr, n_i, (int)B, n_o, n_i, B, B,
-                            st)))
+        // dH[b][i] = sum_o dZ[b][o] W[o][i] = NT product with W^T [i][o]
+        if ((rc = gemm_nt(dcur, Wt[l]->p, dH.p, nullptr, nullptr, false, (int)B, n_i, p_o, p_o, p_o, p_w, 0, st)))
             return rc;
-        const int64_t nh = (int64_t)width * B;
+        const int64_t nh = B * p_w;
         hipLaunchKernelGGL(gelu_back_kernel, dim3(nblk(nh)), dim3(256), 0, st, dH.p, Z[l - 1]->p, dZ.p, nh);
         CFK_HIP(hipGetLastError());
         dcur = dZ.p;
+        p_o = p_w;
+        n_o = width;
+    }
+    return CFK_OK;
+}
+
+int cfk_mlp_s::refresh_bf16(hipStream_t st) {
+    for (int l = 0; l <= depth; ++l) {
+        const int fi = (l == 0) ? n_in : width, fo = (l == depth) ? n_out : width;
+        const int pi = (l == 0) ? p_in : p_w, po = (l == depth) ? p_out : p_w;
+        hipLaunchKernelGGL(refresh_kernel, dim3(nblk((int64_t)fi * fo)), dim3(256), 0, st, w.p + w_off[l], Wb[l]->p,
+                           Wt[l]->p, fo, fi, po, pi);
+        CFK_HIP(hipGetLastError());
     }
     return CFK_OK;
 }
@@ -321,42 +396,28 @@ int cfk_mlp_s::backward(const float *T, int64_t B, hipStream_t st) {
 int cfk_mlp_s::adam(float lr, float b1, float b2, float eps, hipStream_t st) {
     ++step;
     const float bc1 = 1.0f - powf(b1, (float)step), bc2 = 1.0f - powf(b2, (float)step);
-    hipLaunchKernelGGL(adam_kernel, dim3(nblk(n_params)), dim3(256), 0, st, w.p, m.p, v.p, g.p, wb.p, n_params, lr, b1, b2,
-                       eps, bc1, bc2);
+    hipLaunchKernelGGL(adam_kernel, dim3(nblk(n_params)), dim3(256), 0, st, w.p, m.p, v.p, g.p, n_params, lr, b1, b2, eps,
+                       bc1, bc2);
     CFK_HIP(hipGetLastError());
-    return CFK_OK;
-}
-
-int cfk_mlp_s::input_f32(const float *Xf, int64_t B, hipStream_t st) {
-    const int64_t n = (int64_t)n_in * B;
-    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(nblk(n)), dim3(256), 0, st, Xf, X.p, n);
-    CFK_HIP(hipGetLastError());
-    return CFK_OK;
+    return refresh_bf16(st);
 }
 
 int cfk_mlp_s::upload_params(const float *host) {
     CFK_HIP(hipMemcpy(w.p, host, (size_t)n_params * sizeof(float), hipMemcpyHostToDevice));
     int rc;
-    if ((rc = fill_f32(m.p, 0.0f, n_params, nullptr)) || (rc = fill_f32(v.p, 0.0f, n_params, nullptr))) return rc;
+    if ((rc = fill_f32(m.p, 0.0f, n_params, nullptr)) || (rc = fill_f32(v.p, 0.0f, n_params, nullptr)) ||
+        (rc = refresh_bf16(nullptr)))
+        return rc;
     step = 0;
-    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(nblk(n_params)), dim3(256), 0, nullptr, w.p, wb.p, n_params);
-    CFK_HIP(hipGetLastError());
     CFK_HIP(hipDeviceSynchronize());
     return CFK_OK;
 }
 
-int cfk_mlp_s::loss_grad(int64_t B, const float *T, hipStream_t st) {
-    int rc;
-    if ((rc = fill_f64(loss_acc.p, 0.0, 1, st))) return rc;
-    if ((rc = forward_bf16(B, true, st))) return rc;
-    return backward(T, B, st);
-}
-
-int cfk_mlp_s::read_loss(int64_t B, hipStream_t st, double *loss) {
+int cfk_mlp_s::read_loss(int64_t count, hipStream_t st, double *loss) {
     double h = 0.0;
     CFK_HIP(hipMemcpyAsync(&h, loss_acc.p, sizeof(double), hipMemcpyDeviceToHost, st));
     CFK_HIP(hipStreamSynchronize(st));
-    *loss = h / (double)((int64_t)n_out * B);
+    *loss = h / (double)count;
     return CFK_OK;
 }
 
@@ -365,6 +426,9 @@ int cfk_mlp_s::create(int32_t n_in_, int32_t width_, int32_t depth_, int32_t n_o
     width = width_;
     depth = depth_;
     n_out = n_out_;
+    p_in = pad32(n_in);
+    p_w = pad32(width);
+    p_out = pad32(n_out);
     int64_t off = 0;
     w_off.clear();
     b_off.clear();
@@ -377,9 +441,17 @@ int cfk_mlp_s::create(int32_t n_in_, int32_t width_, int32_t depth_, int32_t n_o
     }
     n_params = off;
     int rc;
-    if ((rc = w.alloc(off)) || (rc = m.alloc(off)) || (rc = v.alloc(off)) || (rc = g.alloc(off)) || (rc = wb.alloc(off)) ||
+    if ((rc = w.alloc(off)) || (rc = m.alloc(off)) || (rc = v.alloc(off)) || (rc = g.alloc(off)) ||
         (rc = loss_acc.alloc(1)))
         return rc;
+    Wb.clear();
+    Wt.clear();
+    for (int l = 0; l <= depth; ++l) {
+        const int pi = (l == 0) ? p_in : p_w, po = (l == depth) ? p_out : p_w;
+        Wb.emplace_back(new DevBuf<__bf16>());
+        Wt.emplace_back(new DevBuf<__bf16>());
+        if ((rc = zalloc(*Wb.back(), (size_t)po * pi)) || (rc = zalloc(*Wt.back(), (size_t)pi * po))) return rc;
+    }
     // He-uniform hidden layers (GELU), Glorot-uniform output layer, zero biases
     std::vector<float> host((size_t)off, 0.0f);
     SplitMix rng{seed};
@@ -443,10 +515,10 @@ int cfk_mlp_forward(cfk_mlp mlp, float *Y_dev, const float *X_dev, int64_t B, vo
     if (B < 1 || B > 0x7fffffff) return set_error(CFK_EINVAL, "batch %lld out of range", (long long)B);
     hipStream_t st = (hipStream_t)stream;
     int rc;
-    if ((rc = mlp->ensure_batch(B)) || (rc = mlp->input_f32(X_dev, B, st)) || (rc = mlp->forward_bf16(B, false, st)))
+    if ((rc = mlp->ensure_batch(B)) || (rc = transpose(X_dev, B, mlp->X.p, (int64_t)mlp->p_in, (int64_t)mlp->n_in, B, (int64_t)mlp->n_in, st)))
         return rc;
-    CFK_HIP(hipMemcpyAsync(Y_dev, mlp->Y.p, (size_t)mlp->n_out * B * sizeof(float), hipMemcpyDeviceToDevice, st));
-    return CFK_OK;
+    if ((rc = mlp->forward(B, false, st))) return rc;
+    return transpose(mlp->Y.p, (int64_t)mlp->p_out, Y_dev, B, B, (int64_t)mlp->n_out, B, st);
 }
 
 int cfk_mlp_loss_grad(cfk_mlp mlp, const float *X_dev, const float *T_dev, int64_t B, double *loss, void *stream) {
@@ -454,9 +526,12 @@ int cfk_mlp_loss_grad(cfk_mlp mlp, const float *X_dev, const float *T_dev, int64
     if (B < 1 || B > 0x7fffffff) return set_error(CFK_EINVAL, "batch %lld out of range", (long long)B);
     hipStream_t st = (hipStream_t)stream;
     int rc;
-    if ((rc = mlp->ensure_batch(B)) || (rc = mlp->input_f32(X_dev, B, st)) || (rc = mlp->loss_grad(B, T_dev, st)))
+    if ((rc = mlp->ensure_batch(B))) return rc;
+    if ((rc = transpose(X_dev, B, mlp->X.p, (int64_t)mlp->p_in, (int64_t)mlp->n_in, B, (int64_t)mlp->n_in, st)) ||
+        (rc = transpose(T_dev, B, mlp->T.p, (int64_t)mlp->p_out, (int64_t)mlp->n_out, B, (int64_t)mlp->n_out, st)) ||
+        (rc = fill_f64(mlp->loss_acc.p, 0.0, 1, st)) || (rc = mlp->forward(B, true, st)) || (rc = mlp->backward(B, st)))
         return rc;
-    return loss ? mlp->read_loss(B, st, loss) : CFK_OK;
+    return loss ? mlp->read_loss(B * (int64_t)mlp->n_out, st, loss) : CFK_OK;
 }
 
 int cfk_mlp_adam_step(cfk_mlp mlp, float lr, float beta1, float beta2, float eps, void *stream) {

@@ -7,9 +7,12 @@
 //   Q(f) = n^2 ( Q(e1) + J d + Q(d) ),     J d = Q(e1 + d) - Q(e1 - d) over 2 (exact, linear).
 // J is the collision operator linearized about an isotropic Maxwellian: it commutes with rotations,
 // so it is block diagonal, one K x K block per l shared by all m.  The surrogate evaluates the
-// linear part exactly in FP64 and learns only the remainder with the network:
-//   Q~(f) = n^2 ( J d + N(d) ),   N(d) ~ r(d) = Q(f)/n^2 - J d,
-// then applies the FP64 conservation projection (the five invariant coefficients of Q set to 0).
+// linear part exactly in FP64 and learns only the remainder r(d) = Q(f)/n^2 - J d = Q(e1) + Q(d),
+// which is (up to Q(e1) ~ 1e-14) homogeneous of degree 2 in d.  With s = |d| the network sees the
+// direction u = d / s and predicts r / s^2, so cells far from and close to e1 are learned with the
+// same relative accuracy:
+//   Q~(f) = n^2 ( J d + s^2 N(d / s) ),
+// then the FP64 conservation projection sets the five invariant coefficients of Q~ to zero.
 // Network inputs and targets are standardized per feature with statistics of the training set.
 #include <hip/hip_runtime.h>
 
@@ -32,23 +35,66 @@ inline unsigned nb(int64_t n) { return (unsigned)((n + ET - 1) / ET); }
 
 __device__ inline bool invariant_slot(int q, int k) { return (k == 0 && q < 4) || (k == 1 && q == 0); }
 
-// X[i][b] = bf16((d_i - mu[i]) * inv_s[i]), d = f/n - e1, and n_out[b] = n
-__global__ void prep_input_kernel(const double *__restrict__ f, const float *__restrict__ mu,
-                                  const float *__restrict__ inv_s, __bf16 *__restrict__ X, double *__restrict__ n_out,
-                                  int NK, int64_t B) {
-    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
-    if (t >= (int64_t)NK * B) return;
-    const int i = (int)(t / B);
-    const int64_t b = t % B;
-    const double n = f[b];
-    const double d = f[t] / n - (i == 0 ? 1.0 : 0.0);
-    X[t] = (__bf16)((float)(d - (double)mu[i]) * inv_s[i]);
-    if (i == 0) n_out[b] = n;
+// out[b][i] = (TO)((x_i - mu[i]) * inv_s[i]) for i < rows, b < n, from in [rows][n] (FP64), where
+// x = in[i][b] / in[0][b] - e1_i when `relative`, else x = in[i][b]; then x /= sv[b]^power if sv.
+// The network reads rows of features; 32 x 32 tiles through LDS keep both sides coalesced.
+template <typename TO>
+__global__ void norm_transpose_kernel(const double *__restrict__ in, const float *__restrict__ mu,
+                                      const float *__restrict__ inv_s, TO *__restrict__ out, int64_t ld_out, int rows,
+                                      int64_t n, int relative, const double *__restrict__ sv, int power) {
+    __shared__ float t[32][33];
+    const int64_t i0 = (int64_t)blockIdx.y * 32, b0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    const int64_t b = b0 + tx;
+    const double rn = (relative && b < n) ? 1.0 / in[b] : 1.0;
+    double cs = 1.0;
+    if (sv && b < n) cs = power == 2 ? 1.0 / (sv[b] * sv[b]) : 1.0 / sv[b];
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t i = i0 + k;
+        float v = 0.0f;
+        if (i < rows && b < n) {
+            double x = in[i * n + b];
+            if (relative) x = x * rn - (i == 0 ? 1.0 : 0.0);
+            v = (float)(x * cs - (double)mu[i]) * inv_s[i];
+        }
+        t[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int64_t bb = b0 + k, i = i0 + tx;
+        if (bb < n && i < rows) out[bb * ld_out + i] = (TO)t[tx][k];
+    }
 }
 
-// Q[k1][q][b] = n^2 (sum_k J_l[k1][k] d[k][q][b] + Y[k1][q][b] s[..] + mu[..]), invariants zeroed.
+// Per cell: n = f[0], s = max(|f/n - e1|, S_MIN).  A workgroup covers 64 cells with 4 threads per
+// cell striding over the features (coalesced over cells), reduced through LDS.
+constexpr double S_MIN = 1e-6;
+__global__ void dev_norm_kernel(const double *__restrict__ f, int NK, int64_t B, double *__restrict__ nvec,
+                                double *__restrict__ svec) {
+    __shared__ double red[4][64];
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int64_t b = (int64_t)blockIdx.x * 64 + tx;
+    double s2 = 0.0, n = 1.0;
+    if (b < B) {
+        n = f[b];
+        const double in = 1.0 / n;
+        for (int i = 1 + ty; i < NK; i += 4) {
+            const double d = f[(int64_t)i * B + b] * in;
+            s2 = fma(d, d, s2);
+        }
+    }
+    red[ty][tx] = s2;
+    __syncthreads();
+    if (ty == 0 && b < B) {
+        nvec[b] = n;
+        svec[b] = fmax(sqrt(red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx]), S_MIN);
+    }
+}
+
+// Q[k1][q][b] = n^2 (sum_k J_l[k1][k] d[k][q][b] + s_b^2 (Y[k1][q][b] s[..] + mu[..])), invariants zeroed.
 // use_net = 0 keeps the linear part only.
-__global__ void post_kernel(const double *__restrict__ f, const double *__restrict__ nvec, const float *__restrict__ Y,
+__global__ void post_kernel(const double *__restrict__ f, const double *__restrict__ nvec,
+                            const double *__restrict__ svec, const float *__restrict__ Y,
                             const float *__restrict__ mu, const float *__restrict__ s, const double *__restrict__ JL,
                             double *__restrict__ Q, int K, int N_Q, int64_t B, int use_net) {
     const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
@@ -58,6 +104,7 @@ __global__ void post_kernel(const double *__restrict__ f, const double *__restri
     int l = 0;
     while ((l + 1) * (l + 1) <= q) ++l;
     const double n = nvec[b];
+    const double s2 = use_net ? svec[b] * svec[b] : 0.0;
     const int64_t ks = (int64_t)N_Q * B;
     double d[16];
     for (int k = 0; k < K; ++k) d[k] = f[k * ks + t] / n - ((k == 0 && q == 0) ? 1.0 : 0.0);
@@ -67,7 +114,7 @@ __global__ void post_kernel(const double *__restrict__ f, const double *__restri
         for (int k = 0; k < K; ++k) v = fma(J[k1 * K + k], d[k], v);
         if (use_net) {
             const int64_t i = (int64_t)k1 * N_Q + q;
-            v += (double)Y[k1 * ks + t] * (double)s[i] + (double)mu[i];
+            v += s2 * ((double)Y[k1 * ks + t] * (double)s[i] + (double)mu[i]);
         }
         Q[k1 * ks + t] = invariant_slot(q, k1) ? 0.0 : n * n * v;
     }
@@ -97,13 +144,15 @@ __global__ void target_kernel(const double *__restrict__ f, const double *__rest
     }
 }
 
-// Per-row mean and standard deviation of X [rows][n] (one workgroup per row, two passes, FP64)
+// Per-row mean and standard deviation of X[i][b] / sv[b]^power over b (one workgroup per row, two
+// passes, FP64)
 __global__ void row_stats_kernel(const double *__restrict__ X, double *__restrict__ mean, double *__restrict__ sd,
-                                 int64_t n) {
+                                 int64_t n, const double *__restrict__ sv, int power) {
     __shared__ double red[ET];
     const double *x = X + (int64_t)blockIdx.x * n;
+    auto val = [&](int64_t i) { return power == 0 ? x[i] : (power == 1 ? x[i] / sv[i] : x[i] / (sv[i] * sv[i])); };
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += ET) s += x[i];
+    for (int64_t i = threadIdx.x; i < n; i += ET) s += val(i);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int k = ET / 2; k > 0; k >>= 1) {
@@ -113,7 +162,7 @@ __global__ void row_stats_kernel(const double *__restrict__ X, double *__restric
     const double m = red[0] / (double)n;
     __syncthreads();
     s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += ET) s += (x[i] - m) * (x[i] - m);
+    for (int64_t i = threadIdx.x; i < n; i += ET) s += (val(i) - m) * (val(i) - m);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int k = ET / 2; k > 0; k >>= 1) {
@@ -126,24 +175,15 @@ __global__ void row_stats_kernel(const double *__restrict__ X, double *__restric
     }
 }
 
-template <typename TO>
-__global__ void normalize_kernel(const double *__restrict__ X, const float *__restrict__ mu,
-                                 const float *__restrict__ inv_s, TO *__restrict__ out, int rows, int64_t n) {
-    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
-    if (t >= (int64_t)rows * n) return;
-    const int i = (int)(t / n);
-    out[t] = (TO)((float)(X[t] - (double)mu[i]) * inv_s[i]);
-}
-
-// dst[i][j] = src[i][perm[start + j]] for j < cols (rows x cols), src row length n
+// dst[j][:] = src[perm[start + j]][:] for j < rows, rows of `ld` elements
 template <typename T>
-__global__ void gather_cols_kernel(const T *__restrict__ src, const int32_t *__restrict__ perm, int64_t start,
-                                   T *__restrict__ dst, int rows, int64_t cols, int64_t n) {
+__global__ void gather_rows_kernel(const T *__restrict__ src, const int32_t *__restrict__ perm, int64_t start,
+                                   T *__restrict__ dst, int64_t rows, int ld) {
     const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
-    if (t >= (int64_t)rows * cols) return;
-    const int i = (int)(t / cols);
-    const int64_t j = t % cols;
-    dst[t] = src[(int64_t)i * n + perm[start + j]];
+    if (t >= rows * ld) return;
+    const int64_t j = t / ld;
+    const int k = (int)(t % ld);
+    dst[t] = src[(int64_t)perm[start + j] * ld + k];
 }
 
 // dst[i][j] = src[i][off + j*stride] (rows x cols), src row length B
@@ -205,13 +245,16 @@ struct cfk_surrogate_s {
     DevBuf<float> mu_x, is_x, mu_y, s_y;  // input mean / inverse std, target mean / std
     bool trained = false;
     // scratch
-    DevBuf<double> nvec, Qx, cs, Qs, acc;
+    DevBuf<double> nvec, svec, Qx, cs, Qs, acc;
+    DevBuf<float> Yt;  // network output, feature-major [NK][B]
     int64_t cap = 0;
 
     int ensure(int64_t B) {
         if (B <= cap) return CFK_OK;
         int rc;
-        if ((rc = nvec.alloc(B)) || (rc = Qx.alloc((size_t)NK * B))) return rc;
+        if ((rc = nvec.alloc(B)) || (rc = svec.alloc(B)) || (rc = Qx.alloc((size_t)NK * B)) ||
+            (rc = Yt.alloc((size_t)NK * B)))
+            return rc;
         cap = B;
         return CFK_OK;
     }
@@ -234,19 +277,19 @@ struct cfk_surrogate_s {
     int apply(double *Q, const double *f, int64_t B, bool use_net, hipStream_t st) {
         int rc;
         if ((rc = ensure(B))) return rc;
+        hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((B + 63) / 64)), dim3(ET), 0, st, f, NK, B, nvec.p, svec.p);
+        CFK_HIP(hipGetLastError());
         if (use_net) {
             if ((rc = mlp.ensure_batch(B))) return rc;
-            hipLaunchKernelGGL(prep_input_kernel, dim3(nb((int64_t)NK * B)), dim3(ET), 0, st, f, mu_x.p, is_x.p, mlp.X.p,
-                               nvec.p, NK, B);
+            const dim3 grid((unsigned)((B + 31) / 32), (unsigned)((NK + 31) / 32));
+            hipLaunchKernelGGL(norm_transpose_kernel<__bf16>, grid, dim3(ET), 0, st, f, mu_x.p, is_x.p, mlp.X.p,
+                               (int64_t)mlp.p_in, NK, B, 1, svec.p, 1);
             CFK_HIP(hipGetLastError());
-            if ((rc = mlp.forward_bf16(B, false, st))) return rc;
-        } else {
-            hipLaunchKernelGGL(gather_strided_kernel, dim3(nb(B)), dim3(ET), 0, st, f, nvec.p, 1, B, B, (int64_t)0,
-                               (int64_t)1);
-            CFK_HIP(hipGetLastError());
+            if ((rc = mlp.forward(B, false, st))) return rc;
+            if ((rc = transpose_f32(mlp.Y.p, mlp.p_out, Yt.p, B, B, NK, B, st))) return rc;
         }
-        hipLaunchKernelGGL(post_kernel, dim3(nb((int64_t)N_Q * B)), dim3(ET), 0, st, f, nvec.p,
-                           use_net ? mlp.Y.p : nullptr, mu_y.p, s_y.p, JL.p, Q, K, N_Q, B, use_net ? 1 : 0);
+        hipLaunchKernelGGL(post_kernel, dim3(nb((int64_t)N_Q * B)), dim3(ET), 0, st, f, nvec.p, svec.p,
+                           use_net ? Yt.p : nullptr, mu_y.p, s_y.p, JL.p, Q, K, N_Q, B, use_net ? 1 : 0);
         CFK_HIP(hipGetLastError());
         return CFK_OK;
     }
@@ -353,7 +396,8 @@ int cfk_surrogate_get_normalization(cfk_surrogate s, float *mu_x, float *inv_s_x
 }
 
 int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev, int64_t n, int32_t epochs,
-                      int32_t batch, float lr, float lr_final, uint64_t seed, double *loss_hist, void *stream) {
+                      int32_t batch, float lr, float lr_final, uint64_t seed, int32_t flags, double *loss_hist,
+                      void *stream) {
     if (!s || !f_dev || !Q_dev) return set_error(CFK_EINVAL, "null argument");
     if (n < 2 || n > 0x7fffffff) return set_error(CFK_EINVAL, "n=%lld out of range", (long long)n);
     if (epochs < 1 || batch < 1 || batch > n) return set_error(CFK_EINVAL, "epochs=%d batch=%d", epochs, batch);
@@ -362,29 +406,39 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
     const int NK = s->NK;
     const int64_t tot = (int64_t)NK * n;
     int rc;
-    // 1. d and r in FP64, per-feature statistics
-    DevBuf<double> d, r, mean, sd;
-    if ((rc = d.alloc(tot)) || (rc = r.alloc(tot)) || (rc = mean.alloc(NK)) || (rc = sd.alloc(NK))) return rc;
+    // 1. d and r in FP64, per-cell scale s = |d|, per-feature statistics of d / s and r / s^2
+    DevBuf<double> d, r, mean, sd, nv, sv;
+    if ((rc = d.alloc(tot)) || (rc = r.alloc(tot)) || (rc = mean.alloc(NK)) || (rc = sd.alloc(NK)) || (rc = nv.alloc(n)) ||
+        (rc = sv.alloc(n)))
+        return rc;
     hipLaunchKernelGGL(target_kernel, dim3(nb((int64_t)s->N_Q * n)), dim3(ET), 0, st, f_dev, Q_dev, s->JL.p, d.p, r.p, s->K,
                        s->N_Q, n);
+    hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((n + 63) / 64)), dim3(ET), 0, st, f_dev, NK, n, nv.p, sv.p);
     CFK_HIP(hipGetLastError());
     std::vector<double> hm(NK), hs(NK);
-    auto stats = [&](const double *X, std::vector<float> &mu_o, std::vector<float> &s_o, bool inverse) -> int {
-        hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)NK), dim3(ET), 0, st, X, mean.p, sd.p, n);
+    auto stats = [&](const double *X, int power, bool global, std::vector<float> &mu_o, std::vector<float> &s_o,
+                     bool inverse) -> int {
+        hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)NK), dim3(ET), 0, st, X, mean.p, sd.p, n, sv.p, power);
         CFK_HIP(hipGetLastError());
         CFK_HIP(hipMemcpyAsync(hm.data(), mean.p, NK * sizeof(double), hipMemcpyDeviceToHost, st));
         CFK_HIP(hipMemcpyAsync(hs.data(), sd.p, NK * sizeof(double), hipMemcpyDeviceToHost, st));
         CFK_HIP(hipStreamSynchronize(st));
         const double smax = *std::max_element(hs.begin(), hs.end());
+        double rms = 0.0;
+        for (int i = 0; i < NK; ++i) rms += hs[i] * hs[i];
+        rms = std::sqrt(rms / NK);
         for (int i = 0; i < NK; ++i) {
-            // features that are (near) constant over the set keep unit scale
-            const double sc = (hs[i] > 1e-6 * smax && hs[i] > 0.0) ? hs[i] : 1.0;
+            // one scale for all features (global), or per feature; (near) constant features keep unit scale
+            double sc = global ? rms : hs[i];
+            if (!(sc > 1e-6 * smax && sc > 0.0)) sc = 1.0;
             mu_o[i] = (float)hm[i];
             s_o[i] = (float)(inverse ? 1.0 / sc : sc);
         }
         return CFK_OK;
     };
-    if ((rc = stats(d.p, s->h_mu_x, s->h_is_x, true)) || (rc = stats(r.p, s->h_mu_y, s->h_s_y, false))) return rc;
+    if ((rc = stats(d.p, 1, (flags & CFK_FIT_GLOBAL_INPUT_SCALE) != 0, s->h_mu_x, s->h_is_x, true)) ||
+        (rc = stats(r.p, 2, (flags & CFK_FIT_GLOBAL_TARGET_SCALE) != 0, s->h_mu_y, s->h_s_y, false)))
+        return rc;
     // target normalization uses 1/s on the device: keep both forms
     std::vector<float> inv_sy(NK);
     for (int i = 0; i < NK; ++i) inv_sy[i] = 1.0f / s->h_s_y[i];
@@ -392,11 +446,17 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
     DevBuf<float> isy;
     if ((rc = isy.upload(inv_sy.data(), NK))) return rc;
     // 2. normalized training arrays
-    DevBuf<__bf16> Xall;
-    DevBuf<float> Tall, Tb;
-    if ((rc = Xall.alloc(tot)) || (rc = Tall.alloc(tot)) || (rc = Tb.alloc((size_t)NK * batch))) return rc;
-    hipLaunchKernelGGL(normalize_kernel<__bf16>, dim3(nb(tot)), dim3(ET), 0, st, d.p, s->mu_x.p, s->is_x.p, Xall.p, NK, n);
-    hipLaunchKernelGGL(normalize_kernel<float>, dim3(nb(tot)), dim3(ET), 0, st, r.p, s->mu_y.p, isy.p, Tall.p, NK, n);
+    DevBuf<__bf16> Xall;  // sample-major [n][p_in], zero padded
+    DevBuf<float> Tall;   // sample-major [n][p_out]
+    const int p_in = s->mlp.p_in, p_out = s->mlp.p_out;
+    if ((rc = Xall.alloc((size_t)n * p_in)) || (rc = Tall.alloc((size_t)n * p_out)) ||
+        (rc = fill_bf16(Xall.p, n * p_in, st)) || (rc = fill_f32(Tall.p, 0.0f, n * p_out, st)))
+        return rc;
+    const dim3 tgrid((unsigned)((n + 31) / 32), (unsigned)((NK + 31) / 32));
+    hipLaunchKernelGGL(norm_transpose_kernel<__bf16>, tgrid, dim3(ET), 0, st, d.p, s->mu_x.p, s->is_x.p, Xall.p,
+                       (int64_t)p_in, NK, n, 0, sv.p, 1);
+    hipLaunchKernelGGL(norm_transpose_kernel<float>, tgrid, dim3(ET), 0, st, r.p, s->mu_y.p, isy.p, Tall.p, (int64_t)p_out,
+                       NK, n, 0, sv.p, 2);
     CFK_HIP(hipGetLastError());
     CFK_HIP(hipStreamSynchronize(st));
     d.release();
@@ -417,12 +477,12 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
         if ((rc = fill_f64(s->mlp.loss_acc.p, 0.0, 1, st))) return rc;
         for (int64_t bi = 0; bi < steps_per_epoch; ++bi, ++step) {
             const int64_t cols = batch;
-            hipLaunchKernelGGL(gather_cols_kernel<__bf16>, dim3(nb((int64_t)NK * cols)), dim3(ET), 0, st, Xall.p, dperm.p,
-                               bi * batch, s->mlp.X.p, NK, cols, n);
-            hipLaunchKernelGGL(gather_cols_kernel<float>, dim3(nb((int64_t)NK * cols)), dim3(ET), 0, st, Tall.p, dperm.p,
-                               bi * batch, Tb.p, NK, cols, n);
+            hipLaunchKernelGGL(gather_rows_kernel<__bf16>, dim3(nb(cols * p_in)), dim3(ET), 0, st, Xall.p, dperm.p,
+                               bi * batch, s->mlp.X.p, cols, p_in);
+            hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(nb(cols * p_out)), dim3(ET), 0, st, Tall.p, dperm.p,
+                               bi * batch, s->mlp.T.p, cols, p_out);
             CFK_HIP(hipGetLastError());
-            if ((rc = s->mlp.forward_bf16(cols, true, st)) || (rc = s->mlp.backward(Tb.p, cols, st))) return rc;
+            if ((rc = s->mlp.forward(cols, true, st)) || (rc = s->mlp.backward(cols, st))) return rc;
             const double frac = total > 1 ? (double)step / (double)(total - 1) : 1.0;
             const float lr_t = (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(M_PI * frac)));
             if ((rc = s->mlp.adam(lr_t, 0.9f, 0.999f, 1e-8f, st))) return rc;

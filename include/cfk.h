@@ -235,10 +235,15 @@ int cfk_surrogate_linear_blocks(cfk_surrogate s, double *JL_host);
 int cfk_surrogate_get_normalization(cfk_surrogate s, float *mu_x, float *inv_s_x, float *mu_y, float *s_y);
 /* Trains on n pairs (f_dev, Q_dev = Q(f)): per-feature standardization from the set, `epochs`
  * passes of shuffled minibatches of `batch` pairs (seeded), Adam with a cosine learning-rate decay
- * from lr to lr_final.  loss_hist (host, `epochs` values, may be null) receives the mean training
+ * from lr to lr_final.  flags: CFK_FIT_GLOBAL_INPUT_SCALE / CFK_FIT_GLOBAL_TARGET_SCALE standardize
+ * inputs / targets with one scale for all features (the RMS of the per-feature deviations) instead
+ * of one per feature.  loss_hist (host, `epochs` values, may be null) receives the mean training
  * loss of each epoch. */
+#define CFK_FIT_GLOBAL_INPUT_SCALE 1
+#define CFK_FIT_GLOBAL_TARGET_SCALE 2
 int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev, int64_t n, int32_t epochs,
-                      int32_t batch, float lr, float lr_final, uint64_t seed, double *loss_hist, void *stream);
+                      int32_t batch, float lr, float lr_final, uint64_t seed, int32_t flags, double *loss_hist,
+                      void *stream);
 /* Q_dev = Q~(f_dev) (overwrites).  use_network = 0 evaluates n^2 J d only. */
 int cfk_surrogate_apply(cfk_surrogate s, double *Q_dev, const double *f_dev, int64_t B, int32_t use_network,
                         void *stream);
