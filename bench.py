@@ -58,9 +58,10 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor"],
+    ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate"],
                     help="hs-l8-k10: the metric; uneven: 4096 distributions with L in {4,6,8,12}, K=L+2 "
-                         "(batched item kernels in a hipGraph); rtensor: GPU tensor generation, L=12, K=12, 48^7 grids")
+                         "(batched item kernels in a hipGraph); rtensor: GPU tensor generation, L=12, K=12, 48^7 "
+                         "grids; surrogate: training on 2e5 mixture pairs and 16384-cell relaxation")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
@@ -75,6 +76,8 @@ def main():
         return bench_uneven(args, torch, cfk, perturbed_maxwellians)
     if args.workload == "rtensor":
         return bench_rtensor(args, cfk)
+    if args.workload == "surrogate":
+        return bench_surrogate(args, torch, cfk)
 
     L, K_max, gamma, B = 8, 9, 1.0, args.batch
     K = K_max + 1
@@ -255,6 +258,88 @@ def bench_rtensor(args, cfk):
            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "compute_rtensor_sumfac, L=12, K=12, 48^7", "gain_points": gain_points},
            "gain_points_per_s": gain_points / res["gamma1_s"], **res}
+    print(json.dumps(out))
+
+
+def bench_surrogate(args, torch, cfk):
+    """Learned surrogate, hard spheres L=8, K=10: training (2e5 mixture pairs, 40 epochs) and
+    forward-Euler relaxation of 16384 cells, 200 steps of 0.005 / mu_stress, exact stepping against
+    the surrogate with the 1%-sampled residual check at 5%."""
+    L, K, B, steps, tol = 8, 10, 16384, 200, 5e-2
+    nq = (L + 1) ** 2
+    lab, val, ic = cfk.real_gaunt(L)
+    ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
+    R = cfk.build_rtensor(K - 1, L, 1.0, (48,) * 7, geometry=(lab, val, ic))
+    plan = cfk.CollisionPlan(ls, vs, ics, R, nq, K)
+
+    def pairs(n, seed):
+        f = cfk.project_maxwellians(cfk.maxwellian_mixture_params(n, seed=seed), L, K)
+        Q = torch.zeros_like(f)
+        for s0 in range(0, n, 16384):
+            e = min(n, s0 + 16384)
+            fs = f[:, s0:e].contiguous()
+            Qs = torch.zeros_like(fs)
+            plan.execute(Qs, fs, e - s0, precision="fp64_sym", layout="batch_fastest")
+            Q[:, s0:e] = Qs
+        cfk.conservation_project(Q, n, nq, K, layout="batch_fastest")
+        return f, Q
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f, Q = pairs(200000, 1)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    fv, Qv = pairs(4096, 2)
+    sur = cfk.Surrogate(plan, width=1024, depth=3, seed=0)
+    t0 = time.perf_counter()
+    hist = sur.fit(f, Q, epochs=40, batch=1024, lr=1e-3, lr_final=1e-5, seed=3)
+    t_fit = time.perf_counter() - t0
+    del f, Q
+    rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    err_net, err_lin = rel(sur.apply(fv), Qv), rel(sur.apply(fv, use_network=False), Qv)
+    c0 = cfk.project_maxwellians(cfk.maxwellian_mixture_params(B, seed=5), L, K)
+    dt = 0.005 / abs(sur.linear_blocks()[2][0][0])
+    out_modes = {}
+    finals = {}
+    for mode in ("exact", "surrogate", "linear"):
+        c = c0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, _ = sur.relax(c, dt, steps, mode=mode, tol=tol, sample_stride=100)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        finals[mode] = c
+        out_modes[mode] = {"s": el, "ms_per_step": el / steps * 1e3, **st}
+    chg = float(torch.linalg.norm(finals["exact"] - c0))
+    for mode in ("surrogate", "linear"):
+        out_modes[mode]["final_err_rel_change"] = float(torch.linalg.norm(finals[mode] - finals["exact"])) / chg
+    # one evaluation of all cells: surrogate vs exact FP64 vs mixed
+    Qb = torch.zeros_like(c0)
+
+    def timed(fn, n=20):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n * 1e3
+
+    t_sur = timed(lambda: sur.apply(c0, out=Qb))
+    t_f64 = timed(lambda: plan.execute(Qb, c0, B, precision="fp64_sym", layout="batch_fastest"))
+    t_mix = timed(lambda: plan.execute(Qb, c0, B, precision="mixed", layout="batch_fastest"))
+    gemm_flop = 2.0 * B * (810 * 1024 + 2 * 1024 * 1024 + 1024 * 810)
+    out = {"metric": "surrogate relaxation, 16384 cells x 200 forward-Euler steps, HS L=8 K=10",
+           "value": B * steps / out_modes["surrogate"]["s"], "unit": "cell-steps/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "bf16", "data": "synthetic",
+           "config": {"workload": "surrogate 810 -> 3 x 1024 GELU -> 810, 2e5 training pairs, 40 epochs, "
+                                  "tolerance %g on 1%% sampled cells" % tol},
+           "exact_cell_steps_per_s": B * steps / out_modes["exact"]["s"],
+           "train_data_s": t_data, "fit_s": t_fit, "loss_first_last": [float(hist[0]), float(hist[-1])],
+           "heldout_rel_err": {"network": err_net, "linear_part_only": err_lin},
+           "relax": out_modes,
+           "eval_ms_16384": {"surrogate": t_sur, "exact_fp64": t_f64, "mixed": t_mix},
+           "network_tflops": gemm_flop / (t_sur * 1e-3) / 1e12}
     print(json.dumps(out))
 
 

@@ -157,15 +157,14 @@ __global__ void transpose_kernel(const TI *__restrict__ in, int64_t ld_in, TO *_
     }
 }
 
-// dY = bf16(2 (Y - T) * scale); *loss += sum (Y - T)^2 (block partial sums, one FP64 atomic each)
+// dY = bf16(2 (Y - T) * scale); *loss += sum (Y - T)^2 (grid-stride, one FP64 atomic per workgroup)
 __global__ void mse_grad_kernel(const float *__restrict__ Y, const float *__restrict__ T, __bf16 *__restrict__ dY,
                                 double *__restrict__ loss, int64_t n, float scale) {
     __shared__ float red[256];
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float l = 0.f;
-    if (i < n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const float d = Y[i] - T[i];
-        l = d * d;
+        l += d * d;
         dY[i] = (__bf16)(2.f * d * scale);
     }
     red[threadIdx.x] = l;
@@ -184,13 +183,30 @@ __global__ void gelu_back_kernel(const float *__restrict__ dH, const float *__re
     if (i < n) dZ[i] = (__bf16)(dH[i] * gelu_grad_f(Z[i]));
 }
 
-// gb[c] = sum_r dZ[r][c] for c < cols (rows x ld layout), a thread per column
-__global__ void col_sum_kernel(const __bf16 *__restrict__ dZ, int64_t ld, float *__restrict__ gb, int64_t rows,
-                               int cols) {
+// Column sums gb[c] = sum_r dZ[r][c] (rows x ld layout) in two deterministic passes: workgroup
+// (column block of 64, row chunk) sums its chunk with 4 threads per column into part[chunk][c],
+// then a thread per column adds the chunks in order.
+constexpr int CS_CHUNKS = 32;
+__global__ void col_sum_part_kernel(const __bf16 *__restrict__ dZ, int64_t ld, float *__restrict__ part, int64_t rows,
+                                    int cols) {
+    __shared__ float red[4][64];
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int c = blockIdx.x * 64 + tx;
+    const int64_t per = (rows + CS_CHUNKS - 1) / CS_CHUNKS;
+    const int64_t r0 = (int64_t)blockIdx.y * per, r1 = std::min<int64_t>(rows, r0 + per);
+    float s = 0.f;
+    if (c < cols)
+        for (int64_t r = r0 + ty; r < r1; r += 4) s += (float)dZ[r * ld + c];
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && c < cols) part[(int64_t)blockIdx.y * cols + c] = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
+}
+
+__global__ void col_sum_final_kernel(const float *__restrict__ part, float *__restrict__ gb, int cols) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cols) return;
     float s = 0.f;
-    for (int64_t r = 0; r < rows; ++r) s += (float)dZ[r * ld + c];
+    for (int k = 0; k < CS_CHUNKS; ++k) s += part[(int64_t)k * cols + c];
     gb[c] = s;
 }
 
@@ -327,7 +343,8 @@ int cfk_mlp_s::ensure_batch(int64_t B) {
     }
     if ((rc = zalloc(Y, (size_t)nb * p_out)) || (rc = zalloc(T, (size_t)nb * p_out)) ||
         (rc = zalloc(dY, (size_t)nb * p_out)) || (rc = zalloc(dH, (size_t)nb * p_w)) || (rc = zalloc(dZ, (size_t)nb * p_w)) ||
-        (rc = zalloc(dZt, (size_t)pmax * nbp)) || (rc = zalloc(Ht, (size_t)pmax * nbp)))
+        (rc = zalloc(dZt, (size_t)pmax * nbp)) || (rc = zalloc(Ht, (size_t)pmax * nbp)) ||
+        (rc = zalloc(colpart, (size_t)CS_CHUNKS * pmax)))
         return rc;
     CFK_HIP(hipDeviceSynchronize());
     cap = nb;
@@ -352,8 +369,8 @@ int cfk_mlp_s::forward(int64_t B, bool train, hipStream_t st) {
 int cfk_mlp_s::backward(int64_t B, hipStream_t st) {
     int rc;
     const int64_t n = B * p_out;
-    hipLaunchKernelGGL(mse_grad_kernel, dim3(nblk(n)), dim3(256), 0, st, Y.p, T.p, dY.p, loss_acc.p, n,
-                       1.0f / (float)(B * (int64_t)n_out));
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(std::min<unsigned>(nblk(n), 1024u)), dim3(256), 0, st, Y.p, T.p, dY.p,
+                       loss_acc.p, n, 1.0f / (float)(B * (int64_t)n_out));
     CFK_HIP(hipGetLastError());
     const int64_t kb = (B + 31) / 32 * 32;  // reduction length of the weight gradients (zero-padded batch)
     const __bf16 *dcur = dY.p;
@@ -366,7 +383,9 @@ int cfk_mlp_s::backward(int64_t B, hipStream_t st) {
             return rc;
         if ((rc = gemm_nt(dZt.p, Ht.p, g.p + w_off[l], nullptr, nullptr, false, n_o, n_i, (int)kb, bp, bp, n_i, 0, st)))
             return rc;
-        hipLaunchKernelGGL(col_sum_kernel, dim3(nblk(n_o)), dim3(256), 0, st, dcur, (int64_t)p_o, g.p + b_off[l], B, n_o);
+        hipLaunchKernelGGL(col_sum_part_kernel, dim3((unsigned)((n_o + 63) / 64), CS_CHUNKS), dim3(256), 0, st, dcur,
+                           (int64_t)p_o, colpart.p, B, n_o);
+        hipLaunchKernelGGL(col_sum_final_kernel, dim3(nblk(n_o)), dim3(256), 0, st, colpart.p, g.p + b_off[l], n_o);
         CFK_HIP(hipGetLastError());
         if (l == 0) break;
         // dH[b][i] = sum_o dZ[b][o] W[o][i] = NT product with W^T [i][o]

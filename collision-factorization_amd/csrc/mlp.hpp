@@ -47,6 +47,7 @@ struct cfk_mlp_s {
     cfk::DevBuf<float> dH;              // [cap][p_w]
     cfk::DevBuf<__bf16> dZ;             // [cap][p_w]
     cfk::DevBuf<__bf16> dZt, Ht;        // transposes for the weight gradients [p_max][bp]
+    cfk::DevBuf<float> colpart;         // partial column sums of the bias gradients
     cfk::DevBuf<double> loss_acc;       // sum of squared errors accumulated by backward()
 
     int create(int32_t n_in, int32_t width, int32_t depth, int32_t n_out, uint64_t seed);

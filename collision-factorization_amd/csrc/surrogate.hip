@@ -103,11 +103,11 @@ __global__ void post_kernel(const double *__restrict__ f, const double *__restri
     const int64_t b = t % B;
     int l = 0;
     while ((l + 1) * (l + 1) <= q) ++l;
-    const double n = nvec[b];
+    const double n = nvec[b], rn = 1.0 / n;
     const double s2 = use_net ? svec[b] * svec[b] : 0.0;
     const int64_t ks = (int64_t)N_Q * B;
     double d[16];
-    for (int k = 0; k < K; ++k) d[k] = f[k * ks + t] / n - ((k == 0 && q == 0) ? 1.0 : 0.0);
+    for (int k = 0; k < K; ++k) d[k] = f[k * ks + t] * rn - ((k == 0 && q == 0) ? 1.0 : 0.0);
     const double *J = JL + (int64_t)l * K * K;
     for (int k1 = 0; k1 < K; ++k1) {
         double v = 0.0;
@@ -524,12 +524,14 @@ int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, in
             return rc;
     }
     cfk_relax_stats out{};
+    int skip = 0, backoff = 1;  // after a failed check the surrogate rests for `backoff` steps (doubling, <= 64)
     for (int step = 0; step < n_steps; ++step) {
         double res = -1.0;
-        if (mode == CFK_RELAX_EXACT) {
+        if (mode == CFK_RELAX_EXACT || (mode == CFK_RELAX_SURROGATE && skip > 0)) {
             if ((rc = s->exact(s->Qx.p, c_dev, B, st))) return rc;
             out.exact_evals += B;
             ++out.steps_exact;
+            if (skip > 0) --skip;
         } else if (mode == CFK_RELAX_LINEAR) {
             if ((rc = s->apply(s->Qx.p, c_dev, B, false, st))) return rc;
             ++out.steps_network;
@@ -558,8 +560,11 @@ int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, in
                 if ((rc = s->exact(s->Qx.p, c_dev, B, st))) return rc;
                 out.exact_evals += B;
                 ++out.steps_exact;
+                skip = backoff;
+                backoff = std::min(2 * backoff, 64);
             } else {
                 ++out.steps_network;
+                backoff = 1;
             }
         }
         if (residual_hist) residual_hist[step] = res;

@@ -261,12 +261,14 @@ typedef struct cfk_relax_stats {
 } cfk_relax_stats;
 
 /* n_steps forward-Euler steps c <- c + dt Q(c) of B cells (c_dev in place).  Mode SURROGATE:
- * every step evaluates Q~ for all cells, the exact operator for every sample_stride-th cell
- * (offset rotating with the step) and the sampled residual ||Q~ - Q|| / ||Q|| over those cells;
- * a step whose residual exceeds tol is redone with the exact operator for all cells.  The
- * conservation projection is applied to every Q used, so mass, momentum and energy coefficients of
- * c do not change.  residual_hist (host, n_steps values, may be null) receives each step's sampled
- * residual (-1 when not checked).  Waits for the stream. */
+ * a step evaluates Q~ for all cells, the exact operator for every sample_stride-th cell (offset
+ * rotating with the step) and the sampled residual ||Q~ - Q|| / ||Q|| over those cells; a step
+ * whose residual exceeds tol is redone with the exact operator for all cells, and the surrogate
+ * then rests for `backoff` steps (exact steps, unchecked), backoff doubling with each consecutive
+ * failed check up to 64 and returning to 1 after a passed one.  The conservation projection is
+ * applied to every Q used, so mass, momentum and energy coefficients of c do not change.
+ * residual_hist (host, n_steps values, may be null) receives each step's sampled residual (-1 when
+ * not checked).  Waits for the stream. */
 int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, int32_t n_steps, int32_t mode, double tol,
                         int32_t sample_stride, double *residual_hist, cfk_relax_stats *stats, void *stream);
 
