@@ -23,7 +23,7 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
 # HBM bytes per launch of the mixed kernel at this workload, from two rocprofv3 PMC passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration (factors 2.0 and 1.0); the commands and
 # raw figures are in profiles/af_kernel_mixed_hs_l8_k10_b4096.txt.  Re-measured when the kernel changes.
-MEASURED_TRAFFIC_BYTES = 77868 * 1024
+MEASURED_TRAFFIC_BYTES = 78136 * 1024
 
 
 def work_model(lab, ic, K):
@@ -58,10 +58,11 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate"],
+    ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate", "copy"],
                     help="hs-l8-k10: the metric; uneven: 4096 distributions with L in {4,6,8,12}, K=L+2 "
                          "(batched item kernels in a hipGraph); rtensor: GPU tensor generation, L=12, K=12, 48^7 "
-                         "grids; surrogate: training on 2e5 mixture pairs and 16384-cell relaxation")
+                         "grids; surrogate: training on 2e5 mixture pairs and 16384-cell relaxation; copy: "
+                         "a plain 512 MiB device copy, three times (calibration of HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
@@ -78,6 +79,8 @@ def main():
         return bench_rtensor(args, cfk)
     if args.workload == "surrogate":
         return bench_surrogate(args, torch, cfk)
+    if args.workload == "copy":
+        return bench_copy(torch)
 
     L, K_max, gamma, B = 8, 9, 1.0, args.batch
     K = K_max + 1
@@ -259,6 +262,22 @@ def bench_rtensor(args, cfk):
            "config": {"workload": "compute_rtensor_sumfac, L=12, K=12, 48^7", "gain_points": gain_points},
            "gain_points_per_s": gain_points / res["gamma1_s"], **res}
     print(json.dumps(out))
+
+
+def bench_copy(torch):
+    """Three plain copies of 512 MiB on the device: the reference reading for FETCH_SIZE and
+    WRITE_SIZE counters (bytes read and written are known exactly)."""
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    print(json.dumps({"metric": "device copy bandwidth", "value": 2 * n * 8 / dt / 1e9, "unit": "GB/s",
+                      "n_gpus": 1, "higher_is_better": True, "config": {"workload": "copy 512 MiB x 3"}}))
 
 
 def bench_surrogate(args, torch, cfk):
