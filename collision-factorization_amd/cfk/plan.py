@@ -52,6 +52,10 @@ class CollisionPlan:
                                      _ptr(f), int(B), _stream_ptr(stream)))
         return Q
 
+    def set_tile(self, precision, kc=0, nb=0, pf=0):
+        """Fix one precision's tile (0 = default); for "mixed", pf=1 runs the radial stage on MFMA."""
+        check(lib().cfk_plan_set_tile(self._h, PRECISION[precision], int(kc), int(nb), int(pf)))
+
     def autotune(self, B, stream=None):
         """Pick the fastest tile per precision for batches of B; returns {precision: (KC, NB, PF)}."""
         t = (ctypes.c_int32 * 9)()

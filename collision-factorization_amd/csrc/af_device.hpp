@@ -1,0 +1,67 @@
+// Device-side helpers shared by the contraction kernels (contract.hip, contract_mf.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cfk {
+
+constexpr int WAVE = 64;
+// routing-table entries fetched per scalar round trip; the device tables carry ZB padding entries
+constexpr int ZB = 8;
+
+// The routing tables and R are read-only for the whole launch and read at wave-uniform addresses.
+// They are read through the constant address space, which lets the compiler fetch them with scalar
+// loads even where their pointers were themselves loaded from memory (the batched kernel's group
+// descriptors) and it could not otherwise rule out aliasing with the output stores.
+// Per-lane data (f in, Q out) is read and written through the global address space, so the
+// compiler emits global rather than flat memory instructions for it even where the pointers were
+// loaded from memory: flat loads also count against the scalar-load counter, and every wait for a
+// batch of table or R values would then wait for the outstanding f loads too.
+// (The host pass of this file only needs the types to exist.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+#else
+template <typename T>
+using cptr = const T *;
+template <typename T>
+using gptr = T *;
+#endif
+
+// Per-lane element u + b of a batch-fastest array, with u wave-uniform and lo = b * sizeof(T) the
+// lane's 32-bit byte offset.
+template <typename T>
+__device__ __forceinline__ gptr<T> lane_ptr(gptr<T> base, int64_t u, uint32_t lo) {
+    return (gptr<T>)((gptr<char>)(base + u) + lo);
+}
+
+// Reads of f go through a buffer resource: the address is resource base + lane offset (a VGPR set
+// once) + uniform byte offset (an SGPR), so a load costs no per-lane address arithmetic.  (Plain
+// pointer arithmetic gets the lane part hoisted out of the loops and then pays one 64-bit vector add
+// per load, which at L=8, K=10 was 7% of the kernel's vector instructions.)  The host guarantees
+// that the whole array is below 4 GiB.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __amdgpu_buffer_rsrc_t frsrc_t;
+__device__ __forceinline__ frsrc_t make_frsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T fload(frsrc_t rs, uint32_t lo, int64_t u) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, lo, (int)(u * 4), 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, lo, (int)(u * 8), 0));
+}
+#else
+typedef const void *frsrc_t;
+__device__ inline frsrc_t make_frsrc(const void *p, uint32_t) { return p; }
+template <typename T>
+__device__ inline T fload(frsrc_t rs, uint32_t lo, int64_t u) {
+    return *(const T *)((const char *)rs + lo + u * (int64_t)sizeof(T));
+}
+#endif
+
+}  // namespace cfk

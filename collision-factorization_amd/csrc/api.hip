@@ -47,6 +47,8 @@ int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStr
 int default_kc(int K);
 int table_padding();
 int half_kc(int K);
+bool mf_supports_k(int K);
+int launch_af_mf(cfk_plan_s *p, const float *fT, double *QT, int64_t B, int accumulate, hipStream_t st);
 int launch_af_sym_f64(const cfk_plan_s *p, const Tile &tl, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st);
 int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, double *QT, int64_t B, int accumulate,
@@ -282,6 +284,9 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     int dev = -1;
     CFK_HIP(hipGetDevice(&dev));
     if (dev != p->device) return set_error(CFK_EINVAL, "plan lives on device %d, current device is %d", p->device, dev);
+    if (B > max_batch(p->N_Q, p->K))
+        return set_error(CFK_EINVAL, "B=%lld exceeds %lld distributions per call (f must stay below 4 GiB)", (long long)B,
+                         (long long)max_batch(p->N_Q, p->K));
     const int64_t NK = (int64_t)p->N_Q * p->K;
     const bool natural = layout == CFK_LAYOUT_NATURAL;
     const Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
@@ -294,18 +299,21 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
         if (!p->sym_ok) return set_error(CFK_EUNSUP, "symmetrized tables unavailable: %s", p->sym_why.c_str());
     }
     if (precision == CFK_PREC_MIXED) {
+        const bool mf = tl.pf != 0 && mf_supports_k(p->K);  // matrix-core radial stage
         if ((rc = p->fT32.ensure((size_t)(NK * B)))) return rc;
         if (natural) {
             if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
             if ((rc = launch_transpose_f32(f, p->fT32.p, B, NK, st))) return rc;
             KernelTimer tm(p, st);
-            if ((rc = launch_af_sym_f32(p, tl, p->fT32.p, p->QT.p, B, 0, st))) return rc;
+            if ((rc = mf ? launch_af_mf(p, p->fT32.p, p->QT.p, B, 0, st)
+                         : launch_af_sym_f32(p, tl, p->fT32.p, p->QT.p, B, 0, st)))
+                return rc;
             tm.stop();
             return launch_transpose(p->QT.p, Q, NK, B, true, st);
         }
         if ((rc = launch_to_f32(f, p->fT32.p, NK * B, st))) return rc;
         KernelTimer tm(p, st);
-        if ((rc = launch_af_sym_f32(p, tl, p->fT32.p, Q, B, 1, st))) return rc;
+        if ((rc = mf ? launch_af_mf(p, p->fT32.p, Q, B, 1, st) : launch_af_sym_f32(p, tl, p->fT32.p, Q, B, 1, st))) return rc;
         tm.stop();
         return CFK_OK;
     }
@@ -465,7 +473,10 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
         for (int c = 0; c < 2; ++c) {
             if (c == 1 && kcs[1] == kcs[0]) continue;
             for (int nb = 1; nb <= 2; ++nb)
-                for (int pf = 0; pf <= (prec == CFK_PREC_FP64 ? 1 : 0); ++pf) {
+                for (int pf = 0; pf <= ((prec == CFK_PREC_FP64 || (prec == CFK_PREC_MIXED && mf_supports_k(p->K))) ? 1 : 0);
+                     ++pf) {
+                    // mixed: pf = 1 selects the matrix-core radial stage (fixed chunk, one distribution per lane)
+                    if (prec == CFK_PREC_MIXED && pf == 1 && (nb != 1 || c != 1)) continue;
                     p->kc[prec] = kcs[c];
                     p->nb[prec] = nb;
                     p->pf[prec] = pf;
@@ -502,6 +513,19 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
             kc_out[3 * i + 1] = p->nb[i] > 0 ? p->nb[i] : 1;
             kc_out[3 * i + 2] = p->pf[i];
         }
+    return CFK_OK;
+}
+
+int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf) {
+    if (!plan) return set_error(CFK_EINVAL, "null plan");
+    if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
+    if (kc < 0 || kc > plan->K || nb < 0 || nb > 2 || pf < 0 || pf > 1)
+        return set_error(CFK_EINVAL, "tile (%d, %d, %d) out of range", kc, nb, pf);
+    if (precision == CFK_PREC_MIXED && pf == 1 && !mf_supports_k(plan->K))
+        return set_error(CFK_EUNSUP, "matrix-core radial stage: K=%d not in {6,8,10,12,14}", plan->K);
+    plan->kc[precision] = kc;
+    plan->nb[precision] = nb;
+    plan->pf[precision] = pf;
     return CFK_OK;
 }
 

@@ -76,6 +76,9 @@ struct MultiGroupDev {
     int N_Q, K;
 };
 constexpr int MULTI_MAX_GROUPS = 8;
+// Largest batch per call: the contraction kernels read f through a buffer resource, which spans
+// at most 4 GiB (8-byte coefficients counted).
+inline int64_t max_batch(int N_Q, int K) { return (((int64_t)1 << 32) - 1) / (8 * (int64_t)N_Q * K); }
 
 // One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.  In the device copy
 // of the symmetrized tables the last field instead flags the precise slices (partner block l = 0)
@@ -114,6 +117,7 @@ struct cfk_plan_s {
     cfk::DevBuf<int2> sym_zq;
     cfk::DevBuf<double> sym_zg, sym_R;
     cfk::DevBuf<float> sym_zg32, sym_R32;
+    cfk::DevBuf<float> sym_Rmf;  // R' in the matrix-core operand order (built on first use)
     std::vector<double> sym_row_work;  // cost model per output row (symmetrized tables)
     // scratch for the natural layout (batch-fastest staging)
     cfk::DevBuf<double> fT, QT;
@@ -121,7 +125,7 @@ struct cfk_plan_s {
     // tile choice per precision (index = CFK_PREC_*), set by the autotuner; 0 = default
     int kc[3] = {0, 0, 0};
     int nb[3] = {0, 0, 0};   // distributions per lane
-    int pf[3] = {0, 0, 0};   // prefetch next nonzero (plain tables)
+    int pf[3] = {0, 0, 0};   // FP64: prefetch next nonzero (plain tables); mixed: matrix-core radial stage
     // kernel timing (cfk_plan_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -12,12 +12,10 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "af_device.hpp"
 
 namespace cfk {
 
-constexpr int WAVE = 64;
-// routing-table entries fetched per scalar round trip; the device tables carry ZB padding entries
-constexpr int ZB = 8;
 int table_padding() { return ZB; }
 
 #define CFK_K_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
@@ -55,26 +53,6 @@ struct SparseTables {
 // ---------------------------------------------------------------------------------------------
 // One work item of the angular-first contraction: output row `row` for batch tile `tile` (64*NB
 // distributions), lane = threadIdx.x % 64 of a one-wave workgroup.
-// The routing tables and R are read-only for the whole launch and read at wave-uniform addresses.
-// They are read through the constant address space, which lets the compiler fetch them with scalar
-// loads even where their pointers were themselves loaded from memory (the batched kernel's group
-// descriptors) and it could not otherwise rule out aliasing with the output stores.
-// Per-lane data (f in, Q out) is read and written through the global address space, so the
-// compiler emits global rather than flat memory instructions for it even where the pointers were
-// loaded from memory: flat loads also count against the scalar-load counter, and every wait for a
-// batch of table or R values would then wait for the outstanding f loads too.
-// (The host pass of this file only needs the types to exist.)
-#if defined(__HIP_DEVICE_COMPILE__)
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-template <typename T>
-using gptr = __attribute__((address_space(1))) T *;
-#else
-template <typename T>
-using cptr = const T *;
-template <typename T>
-using gptr = T *;
-#endif
 
 template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
 __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__restrict__ QT_in,
@@ -93,16 +71,18 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
     const cptr<float> R32_c = (cptr<float>)R32;
     // Each lane owns NB distributions b_i = (tile*NB + i)*64 + lane, so every wave-uniform value
     // (table entry, Gaunt value, R row) fetched through the scalar path feeds NB FMAs.
-    int64_t bi[NB];
+    uint32_t lo[NB], lq[NB];  // byte offsets of the lane's distribution in an f row / a Q row (B * 8 < 2^32)
     bool act[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         const int64_t b = (tile * NB + i) * WAVE + (threadIdx.x % WAVE);
         act[i] = b < B;
-        bi[i] = act[i] ? b : 0;
+        lo[i] = act[i] ? (uint32_t)b * (uint32_t)sizeof(T) : 0u;
+        lq[i] = act[i] ? (uint32_t)b * 8u : 0u;
     }
     const int s0 = row_ptr_c[row], s1 = row_ptr_c[row + 1];
     const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
+    const frsrc_t frs = make_frsrc(fT_in, (uint32_t)(K * kstride * (int64_t)sizeof(T)));
     double q[NB][K];
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -125,14 +105,15 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
             // Table entries are fetched ZB at a time (one scalar round trip per batch; the device arrays
             // carry ZB padding entries so the batch read never leaves them).
             if (GROUPED) {
+                // A q2 group accumulates h = sum g f3 and is closed (phi += f2 (x) h) by the next
+                // group's first entry or the end of the slice; the slice's first entry opens a group
+                // without closing one.
                 T f2[NB][K], h[NB][KC];
 #pragma unroll
-                for (int i = 0; i < NB; ++i) {
+                for (int i = 0; i < NB; ++i)
 #pragma unroll
                     for (int j = 0; j < KC; ++j) h[i][j] = T(0);
-#pragma unroll
-                    for (int k = 0; k < K; ++k) f2[i][k] = T(0);
-                }
+                bool have = false;  // wave-uniform: a group is open
                 for (int zb = sl.z0; zb < sl.z1; zb += ZB) {
                     int2 qv[ZB];
                     T gv[ZB];
@@ -146,33 +127,35 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                         if (zb + e < sl.z1) {
                             const int2 qq = qv[e];
                             const T g = gv[e];
-                            if (qq.x < 0) {  // new q2 group: flush the previous one, load f[q2,:]
+                            if (qq.x < 0) {  // new q2 group: close the open one, load f[q2,:]
+                                if (have) {
+#pragma unroll
+                                    for (int i = 0; i < NB; ++i)
+#pragma unroll
+                                        for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                                            for (int k2 = 0; k2 < K; ++k2)
+                                                phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
+                                            h[i][j] = T(0);
+                                        }
+                                }
+                                have = true;
+                                const int64_t u2 = (int64_t)(~qq.x) * B;
 #pragma unroll
                                 for (int i = 0; i < NB; ++i)
 #pragma unroll
-                                    for (int j = 0; j < KC; ++j) {
-#pragma unroll
-                                        for (int k2 = 0; k2 < K; ++k2)
-                                            phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
-                                        h[i][j] = T(0);
-                                    }
-#pragma unroll
-                                for (int i = 0; i < NB; ++i) {
-                                    const gptr<const T> f2p = fT + (int64_t)(~qq.x) * B + bi[i];
-#pragma unroll
-                                    for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = f2p[k2 * kstride];
-                                }
+                                    for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = fload<T>(frs, lo[i], u2 + k2 * kstride);
                             }
+                            const int64_t u3 = (int64_t)qq.y * B;
 #pragma unroll
-                            for (int i = 0; i < NB; ++i) {
-                                const gptr<const T> f3p = fT + (int64_t)qq.y * B + bi[i];
+                            for (int i = 0; i < NB; ++i)
 #pragma unroll
                                 for (int j = 0; j < KC; ++j)
-                                    if (c0 + j < K) h[i][j] = fma(g, f3p[(c0 + j) * kstride], h[i][j]);
-                            }
+                                    if (c0 + j < K) h[i][j] = fma(g, fload<T>(frs, lo[i], u3 + (c0 + j) * kstride), h[i][j]);
                         }
                     }
                 }
+                // every slice has at least one entry, so a group is open here
 #pragma unroll
                 for (int i = 0; i < NB; ++i)
 #pragma unroll
@@ -186,12 +169,12 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                 T f2n[NB][K], f3n[NB][KC];
 #pragma unroll
                 for (int i = 0; i < NB; ++i) {
-                    const gptr<const T> f2p = fT + (int64_t)qn.x * B + bi[i];
-                    const gptr<const T> f3p = fT + (int64_t)qn.y * B + bi[i];
+                    const int64_t u2 = (int64_t)qn.x * B, u3 = (int64_t)qn.y * B;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) f2n[i][k] = f2p[k * kstride];
+                    for (int k = 0; k < K; ++k) f2n[i][k] = fload<T>(frs, lo[i], u2 + k * kstride);
 #pragma unroll
-                    for (int j = 0; j < KC; ++j) f3n[i][j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                    for (int j = 0; j < KC; ++j)
+                        f3n[i][j] = (c0 + j < K) ? fload<T>(frs, lo[i], u3 + (c0 + j) * kstride) : T(0);
                 }
                 for (int z = sl.z0; z < sl.z1; ++z) {
                     const T g = gn;
@@ -208,12 +191,12 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                         gn = zg_c[z + 1];
 #pragma unroll
                         for (int i = 0; i < NB; ++i) {
-                            const gptr<const T> f2p = fT + (int64_t)qn.x * B + bi[i];
-                            const gptr<const T> f3p = fT + (int64_t)qn.y * B + bi[i];
+                            const int64_t u2 = (int64_t)qn.x * B, u3 = (int64_t)qn.y * B;
 #pragma unroll
-                            for (int k = 0; k < K; ++k) f2n[i][k] = f2p[k * kstride];
+                            for (int k = 0; k < K; ++k) f2n[i][k] = fload<T>(frs, lo[i], u2 + k * kstride);
 #pragma unroll
-                            for (int j = 0; j < KC; ++j) f3n[i][j] = (c0 + j < K) ? f3p[(c0 + j) * kstride] : T(0);
+                            for (int j = 0; j < KC; ++j)
+                                f3n[i][j] = (c0 + j < K) ? fload<T>(frs, lo[i], u3 + (c0 + j) * kstride) : T(0);
                         }
                     }
 #pragma unroll
@@ -237,15 +220,14 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                         if (zb + e < sl.z1) {
 #pragma unroll
                             for (int i = 0; i < NB; ++i) {
-                                const gptr<const T> f2p = fT + (int64_t)qv[e].x * B + bi[i];
-                                const gptr<const T> f3p = fT + (int64_t)qv[e].y * B + bi[i];
+                                const int64_t u2 = (int64_t)qv[e].x * B, u3 = (int64_t)qv[e].y * B;
                                 T gf2[K];
 #pragma unroll
-                                for (int k2 = 0; k2 < K; ++k2) gf2[k2] = gv[e] * f2p[k2 * kstride];
+                                for (int k2 = 0; k2 < K; ++k2) gf2[k2] = gv[e] * fload<T>(frs, lo[i], u2 + k2 * kstride);
 #pragma unroll
                                 for (int j = 0; j < KC; ++j) {
                                     if (c0 + j < K) {
-                                        const T f3 = f3p[(c0 + j) * kstride];
+                                        const T f3 = fload<T>(frs, lo[i], u3 + (c0 + j) * kstride);
 #pragma unroll
                                         for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[k2], f3, phi[i][k2][j]);
                                     }
@@ -323,7 +305,7 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
         for (int k1 = 0; k1 < K; ++k1) {
             double v = q[i][k1];
             if (PROJ && ((row < 4 && k1 == 0) || (row == 0 && k1 == 1))) v = 0.0;
-            const gptr<double> o = QT + ((int64_t)k1 * N_Q + row) * B + bi[i];
+            const gptr<double> o = lane_ptr(QT, ((int64_t)k1 * N_Q + row) * B, lq[i]);
             *o = accumulate ? *o + v : v;
         }
     }

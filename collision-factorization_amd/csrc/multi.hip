@@ -211,7 +211,8 @@ int cfk_multi_execute(cfk_multi m, double *const *Q_dev, const double *const *f_
     if (!m || !Q_dev || !f_dev || !B) return set_error(CFK_EINVAL, "null argument");
     const int G = (int)m->plans.size();
     for (int g = 0; g < G; ++g) {
-        if (B[g] < 0) return set_error(CFK_EINVAL, "B[%d] < 0", g);
+        if (B[g] < 0 || B[g] > max_batch(m->plans[g]->N_Q, m->plans[g]->K))
+            return set_error(CFK_EINVAL, "B[%d] = %lld out of range", g, (long long)B[g]);
         if (B[g] > 0 && (!Q_dev[g] || !f_dev[g])) return set_error(CFK_EINVAL, "null buffer for group %d", g);
     }
     int dev = -1;

@@ -137,13 +137,18 @@ int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t 
                      const double *f_dev, int64_t B, void *stream);
 
 /* Autotuner: times every tile variant of the angular-first kernels -- k3 chunk width KC in
- * {K, ceil(K/2)} x distributions per lane NB in {1,2} x (FP64 only) prefetch of the next nonzero's
- * rows on/off -- for each precision on a synthetic batch of B distributions (batch-fastest layout)
- * on `stream` and keeps the fastest per precision in the plan.  When tiles_out is not NULL it
- * receives 9 ints: (KC, NB, PF) for FP64, MIXED, FP64_SYM in that order.  Allocates temporary
- * device buffers; synchronizes `stream`.  The full variant set exists for K in {6,8,10,12,14}; other
- * K run the default tile. */
+ * {K, ceil(K/2)} x distributions per lane NB in {1,2} x a third switch PF: for FP64 the prefetch of
+ * the next nonzero's rows, for MIXED the radial stage on the matrix cores (v_mfma_f32_16x16x4_f32,
+ * KC = ceil(K/2), NB = 1) -- for each precision on a synthetic batch of B distributions
+ * (batch-fastest layout) on `stream` and keeps the fastest per precision in the plan.  When
+ * tiles_out is not NULL it receives 9 ints: (KC, NB, PF) for FP64, MIXED, FP64_SYM in that order.
+ * Allocates temporary device buffers; synchronizes `stream`.  The full variant set exists for K in
+ * {6,8,10,12,14}; other K run the default tile. */
 int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *tiles_out);
+
+/* Sets one precision's tile (KC, NB, PF as above; KC = 0 and NB = 0 select the defaults) without
+ * timing anything. */
+int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf);
 
 /* Kernel timing: when enabled, every execute brackets its main contraction kernel (not the
  * layout/precision conversions around it) with a pair of HIP events on the execution stream.

@@ -77,3 +77,23 @@ def test_multi_rejects_unsupported_groups(torch_cuda):
     p = cfk.CollisionPlan(ls, vs, ics, R, 9, 5)    # K=5 has no fused-kernel path
     with pytest.raises(cfk.CfkError):
         cfk.MultiPlan([p])
+
+
+def test_matrix_core_radial_stage_per_k(groups, torch_cuda):
+    """The mixed path with the radial stage on the matrix cores, for every supported K, against the
+    FP64 GPU operator (ragged batches)."""
+    torch = torch_cuda
+    for i, g in enumerate(groups):
+        B = 130 + 7 * i
+        f = perturbed_maxwellians(g["L"], g["K"], B, seed=200 + i)
+        fT = torch.from_numpy(f).cuda().reshape(B, -1).t().contiguous()
+        ref = torch.zeros_like(fT)
+        g["plan"].execute(ref, fT, B, precision="fp64", layout="batch_fastest")
+        g["plan"].set_tile("mixed", 0, 1, 1)
+        print("matrix-core radial stage L=%d K=%d" % (g["L"], g["K"]), flush=True)
+        Q = torch.zeros_like(fT)
+        g["plan"].execute(Q, fT, B, precision="mixed", layout="batch_fastest")
+        torch.cuda.synchronize()
+        g["plan"].set_tile("mixed")
+        err = float(((Q - ref).abs().amax(dim=0) / ref.abs().amax(dim=0)).max())
+        assert err <= TOL_MIXED, (g["L"], err)

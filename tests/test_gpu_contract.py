@@ -175,3 +175,30 @@ def test_autotuned_tiles_keep_parity(config1, torch_cuda):
         c["plan"].execute(Qd, fd, B, precision=prec)
         torch.cuda.synchronize()
         assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= tol, prec
+
+
+def test_every_mixed_tile_keeps_parity(config1, torch_cuda):
+    """Each mixed-precision tile variant (k3 chunk, distributions per lane, radial stage on the VALU
+    or on the matrix cores) against the oracle, on a ragged batch, natural and batch-fastest."""
+    torch = torch_cuda
+    c = config1
+    nq, K, B = c["nq"], c["K"], 200
+    f = perturbed_maxwellians(c["L"], K, B, seed=98)
+    Qref = oracle.angular_first_batch(np.zeros(B * nq * K), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
+                                      nthreads=8)
+    fd = torch.from_numpy(f).cuda()
+    fT = fd.reshape(B, -1).t().contiguous()
+    for kc, nb, pf in ((K, 1, 0), ((K + 1) // 2, 1, 0), (K, 2, 0), ((K + 1) // 2, 1, 1)):
+        c["plan"].set_tile("mixed", kc, nb, pf)
+        print("mixed tile", kc, nb, pf, flush=True)
+        Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
+        c["plan"].execute(Qd, fd, B, precision="mixed")
+        QT = torch.zeros_like(fT)
+        c["plan"].execute(QT, fT, B, precision="mixed", layout="batch_fastest")
+        torch.cuda.synchronize()
+        assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= TOL_MIXED, (kc, nb, pf)
+        Qn = QT.t().contiguous().cpu().numpy().ravel()
+        assert rel_err(Qn, Qref, nq, K) <= TOL_MIXED, (kc, nb, pf)
+        Qn = Qn.reshape(B, K, nq)
+        assert float(np.abs(Qn[:, 0, :4]).max()) == 0.0 and float(np.abs(Qn[:, 1, 0]).max()) == 0.0
+    c["plan"].set_tile("mixed")
