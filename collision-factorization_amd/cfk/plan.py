@@ -52,6 +52,16 @@ class CollisionPlan:
                                      _ptr(f), int(B), _stream_ptr(stream)))
         return Q
 
+    def linearize(self, f0=None, stream=None):
+        """J = dQ/dc at f0 (default: the absolute Maxwellian), [N, N] with N = N_Q*K, index k*N_Q + q."""
+        n = self.N_Q * self.K
+        J = np.empty((n, n), order="F")
+        f0a = f64(f0) if f0 is not None else None   # kept alive across the call
+        f0p = dptr(f0a) if f0a is not None else None
+        check(lib().cfk_plan_linearize(self._h, f0p, J.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       _stream_ptr(stream)))
+        return J
+
     def set_tile(self, precision, kc=0, nb=0, pf=0):
         """Fix one precision's tile (0 = default); for "mixed", pf=1 runs the radial stage on MFMA."""
         check(lib().cfk_plan_set_tile(self._h, PRECISION[precision], int(kc), int(nb), int(pf)))

@@ -516,6 +516,36 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
     return CFK_OK;
 }
 
+int cfk_plan_linearize(cfk_plan plan, const double *f0, double *J, void *stream) {
+    if (!plan || !J) return set_error(CFK_EINVAL, "null argument");
+    cfk_plan_s *p = plan;
+    const int64_t NK = (int64_t)p->N_Q * p->K, B = 2 * NK;
+    if (NK > 8192) return set_error(CFK_EINVAL, "linearization of %lld coefficients is too large", (long long)NK);
+    // column j from Q(f0 + e_j) - Q(f0 - e_j) = 2 J e_j (Q is quadratic): 2 NK distributions, natural layout
+    std::vector<double> f((size_t)(B * NK), 0.0);
+    for (int64_t j = 0; j < NK; ++j)
+        for (int64_t i = 0; i < NK; ++i) {
+            const double base = f0 ? f0[i] : (i == 0 ? 1.0 : 0.0);
+            f[(size_t)(j * NK + i)] = base;
+            f[(size_t)((NK + j) * NK + i)] = base;
+        }
+    for (int64_t j = 0; j < NK; ++j) {
+        f[(size_t)(j * NK + j)] += 1.0;
+        f[(size_t)((NK + j) * NK + j)] -= 1.0;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf<double> fd, Qd;
+    int rc;
+    if ((rc = fd.upload(f.data(), f.size())) || (rc = Qd.alloc(f.size()))) return rc;
+    CFK_HIP(hipMemsetAsync(Qd.p, 0, f.size() * sizeof(double), st));
+    if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, CFK_PREC_FP64, CFK_LAYOUT_NATURAL, Qd.p, fd.p, B, st))) return rc;
+    CFK_HIP(hipStreamSynchronize(st));
+    CFK_HIP(hipMemcpy(f.data(), Qd.p, f.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < NK; ++j)
+        for (int64_t i = 0; i < NK; ++i) J[i + j * NK] = 0.5 * (f[(size_t)(j * NK + i)] - f[(size_t)((NK + j) * NK + i)]);
+    return CFK_OK;
+}
+
 int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf) {
     if (!plan) return set_error(CFK_EINVAL, "null plan");
     if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);

@@ -146,6 +146,13 @@ int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t 
  * {6,8,10,12,14}; other K run the default tile. */
 int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *tiles_out);
 
+/* Linearized operator J = dQ/dc at f0 (host, N_Q x K MATLAB order; NULL = the absolute Maxwellian
+ * c_000 = 1), written to J (host, (N_Q K) x (N_Q K) column-major, index k*N_Q + q): the reference's
+ * J = C(:,:,1) + C(:,1,:) for f0 = e_000 (tests/TestWCUEigenvalues.m:37,
+ * tutorial_stress_relaxation.m:51).  Computed exactly with the plan's FP64 operator on the GPU as
+ * J e_j = (Q(f0 + e_j) - Q(f0 - e_j)) / 2.  Synchronizes `stream`. */
+int cfk_plan_linearize(cfk_plan plan, const double *f0, double *J, void *stream);
+
 /* Sets one precision's tile (KC, NB, PF as above; KC = 0 and NB = 0 select the defaults) without
  * timing anything. */
 int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf);
