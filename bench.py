@@ -262,21 +262,57 @@ def bench_uneven(args, torch, cfk, perturbed_maxwellians):
     print(json.dumps(out))
 
 
+def sumfac_flops(L, NK, sizes, S):
+    """Algorithmic FP64 flops of the matrix-core quadrature: per inner-t step and side the degree
+    blocks M (2 NK N_Q per gain point) and the slot contraction (2 S NK); per unit the fold
+    (two products per R element: 4 NK^3 N_L)."""
+    Nx, Nu1, Nt1, Ny2, Nt2, Nchi, Neps = sizes
+    NQ = (L + 1) ** 2
+    n_l = {4: 42, 6: 106, 8: 215, 10: 381, 12: 616}.get(L)
+    n_tau = Nx * (Nu1 * Nt1 + Ny2 * Nt2)
+    n_units = Nx * (Nu1 + Ny2 * Nt2)
+    f_units = n_tau * 2 * (2 * NK * NQ * Nchi * Neps + 2 * S * NK)
+    f_fold = n_units * 4 * NK ** 3 * n_l
+    return f_units, f_fold
+
+
 def bench_rtensor(args, cfk):
-    """GPU tensor generation at L=12, K=12 radial modes, 48 nodes per axis, gamma = 0 and 1."""
+    """GPU tensor generation at L=12, K=12 radial modes, 48 nodes per axis, gamma = 0 and 1, on the
+    matrix-core quadrature; the per-channel kernel (reference summation order) timed beside it."""
     L, K_max = 12, 11
+    NK = K_max + 1
+    sizes = (48,) * 7
     geo = cfk.real_gaunt(L)
+    S = len({(int(t), int(q)) for t, q in zip(geo[2], geo[0][:, 0])})
     res = {}
-    for gamma in (0.0, 1.0):
-        t0 = time.perf_counter()
-        R = cfk.build_rtensor(K_max, L, gamma, (48,) * 7, geometry=geo)
-        res["gamma%d_s" % int(gamma)] = time.perf_counter() - t0
-        res["gamma%d_max_abs_R" % int(gamma)] = float(np.max(np.abs(R)))
+    timing = {}
+    for kind, gammas in (("matrix_core", (0.0, 1.0)), ("per_channel", (1.0,))):
+        cfk.set_sumfac_kernel(kind)
+        for gamma in gammas:
+            t0 = time.perf_counter()
+            R = cfk.build_rtensor(K_max, L, gamma, sizes, geometry=geo)
+            key = "%s_gamma%d" % (kind, int(gamma))
+            res[key + "_s"] = time.perf_counter() - t0
+            res[key + "_max_abs_R"] = float(np.max(np.abs(R)))
+            timing[key] = cfk.sumfac_last_timing()
+            print("%s: %.3f s, kernels %s" % (key, res[key + "_s"], timing[key]), file=sys.stderr, flush=True)
+    cfk.set_sumfac_kernel("matrix_core")
+    f_units, f_fold = sumfac_flops(L, NK, sizes, S)
+    tm = timing["matrix_core_gamma1"]
+    achieved = f_units / (tm["units_ms"] * 1e-3) / 1e12
     gain_points = 48 * (48 * 48 + 48 * 48) * 48 * 48
-    out = {"metric": "tensor generation seconds, L=12, K=12, 48 nodes per axis", "value": res["gamma1_s"],
+    out = {"metric": "tensor generation seconds, L=12, K=12, 48 nodes per axis", "value": res["matrix_core_gamma1_s"],
            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "compute_rtensor_sumfac, L=12, K=12, 48^7", "gain_points": gain_points},
-           "gain_points_per_s": gain_points / res["gamma1_s"], **res}
+           "config": {"workload": "compute_rtensor_sumfac, L=12, K=12, 48^7, matrix-core quadrature",
+                      "gain_points": gain_points, "slots": S},
+           "gain_points_per_s": gain_points / res["matrix_core_gamma1_s"],
+           "kernel_ms": timing,
+           "speedup_vs_per_channel": res["per_channel_gamma1_s"] / res["matrix_core_gamma1_s"],
+           "roofline": {"bound": "mfma", "kernel": "sumfac_units_mf_kernel", "achieved": achieved, "peak": 78.6,
+                        "unit": "TFLOP/s", "frac": achieved / 78.6, "flop_units": f_units, "flop_fold": f_fold,
+                        "fold_tflops": f_fold / (tm["fold_ms"] * 1e-3) / 1e12,
+                        "traffic": None},
+           **res}
     print(json.dumps(out))
 
 

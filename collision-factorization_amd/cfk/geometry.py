@@ -5,7 +5,8 @@
     gauss_legendre      src/Gauss.m:38-105
     gauss_gen_laguerre  src/Gauss.m:128-173
     sumfac_grid_sizes   src/GeneralCollisionTensor.m:55-64
-    build_rtensor       src/GeneralCollisionTensor.m:43-192 (prep + sumfac + conservation zeroing, on the GPU)
+    build_rtensor       src/GeneralCollisionTensor.m:43-192 (host prep, GPU sumfac, conservation zeroing)
+    set_sumfac_kernel   choice of the GPU quadrature kernel ("matrix_core" or "per_channel")
 """
 import ctypes
 
@@ -51,6 +52,26 @@ def sumfac_grid_sizes(K_max, L, radial_pad=40, angular_pad=0):
     s = (_i32 * 7)()
     check(lib().cfk_sumfac_grid_sizes(int(K_max), int(L), int(radial_pad), int(angular_pad), s))
     return tuple(int(v) for v in s)
+
+
+SUMFAC_KERNELS = {"matrix_core": 0, "per_channel": 1}
+
+
+def set_sumfac_kernel(kind):
+    """Select the GPU quadrature kernel for later sumfac calls (process-wide).
+
+    "matrix_core" (default): gain-point sums once per SH degree on FP64 matrix cores, then the
+    P_gain slot contraction; "per_channel": one lane per t_chan in the reference's summation order
+    (compute_rtensor_sumfac_mex.cpp:219-234).  Both FP64; they agree to rounding."""
+    check(lib().cfk_sumfac_set_kernel(SUMFAC_KERNELS[kind]))
+
+
+def sumfac_last_timing():
+    """Device time of the last sumfac call: {"kernel", "units", "units_ms", "fold_ms"}."""
+    k, n, a, b = _i32(0), _i64(0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+    check(lib().cfk_sumfac_last_timing(ctypes.byref(k), ctypes.byref(n), ctypes.byref(a), ctypes.byref(b)))
+    names = {v: kk for kk, v in SUMFAC_KERNELS.items()}
+    return {"kernel": names[k.value], "units": n.value, "units_ms": a.value, "fold_ms": b.value}
 
 
 def build_rtensor(K_max, L, alpha, grid_sizes, geometry=None):

@@ -397,6 +397,605 @@ __global__ __launch_bounds__(256) void sumfac_fold_kernel(int NK, int NL, int L,
     R[o] = acc;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Matrix-core variant (the default).
+//
+// The valid q_i of a t_chan are the q1 labels of its Gaunt entries, all of degree l1 = trip[t][0]
+// (src/GeneralCollisionTensor.m:130-160), so the reference's per-t_chan gain sum over the points
+// p of one inner-t step (compute_rtensor_sumfac_mex.cpp:219-234)
+//     Phi_gain[k][tc] += sum_p w(p) Rad(p, l1, k) sum_{s in slots(tc)} Y(p, q_s) P(s)
+// reassociates into
+//     M(l, k, q) = sum_p w(p) Rad(p, l, k) Y(p, q)          for every degree l and l(q) = l
+//     Phi_gain[k][tc] += sum_{s in slots(tc)} P(s) M(l1, k, q_s).
+// The p-sum is done once per degree instead of once per t_chan (about N_L / (L+1) = 47 times fewer
+// operations at L = 12), and it is a dense product: for each side and degree l, the NK x (2l+1)
+// block M_l = A_l B_l with A_l[k][p] = w(p) Rad(p, l, k), B_l[p][m] = Y(p, l^2 + m), inner length
+// the N_chi N_eps gain points.  Each block is tiled 16 x 16 onto v_mfma_f64_16x16x4_f64 (rows k,
+// columns m), FP64 operands and accumulation.  The slot contraction, the loss term (whose radial
+// factor is fixed per unit) and the unit Jacobian follow per unit.  Sums run in a different order
+// than the reference's; the results agree to rounding.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int MF_THREADS = 512;         // 8 waves; two workgroups per CU give 4 waves per SIMD
+constexpr int MF_WAVES = MF_THREADS / 64;
+constexpr int MF_NPB = 12;              // gain points per batch (a multiple of 4, the MFMA inner length)
+constexpr int MF_ITEMS = 2 * MF_NPB;    // (point, side) items per batch; item = side * MF_NPB + point
+constexpr int MF_PAD = 32;              // operand reads run up to 31 doubles past the last item's row
+                                        // (they land in unused rows/columns of the products)
+constexpr int MF_SH_LANES = 192;        // evaluation lanes: SH tasks in [0, 192), radial tasks in
+constexpr int MF_GEO_LANE = 384;        // [192, 384), the next batch's geometry from lane 384 on
+
+// Operand row stride in doubles: n rounded up to odd.  The evaluation tasks store with one lane
+// per item, so an odd stride puts 32 consecutive items on distinct LDS bank pairs.
+__host__ __device__ inline int mf_stride(int n) { return n | 1; }
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// 1/i for the three-term recursions (exactly rounded constants; the products differ from the
+// reference's divisions by at most one rounding per step)
+__constant__ double c_rinv[32] = {0.0,        1.0,        1.0 / 2,  1.0 / 3,  1.0 / 4,  1.0 / 5,  1.0 / 6,
+                                  1.0 / 7,    1.0 / 8,    1.0 / 9,  1.0 / 10, 1.0 / 11, 1.0 / 12, 1.0 / 13,
+                                  1.0 / 14,   1.0 / 15,   1.0 / 16, 1.0 / 17, 1.0 / 18, 1.0 / 19, 1.0 / 20,
+                                  1.0 / 21,   1.0 / 22,   1.0 / 23, 1.0 / 24, 1.0 / 25, 1.0 / 26, 1.0 / 27,
+                                  1.0 / 28,   1.0 / 29,   1.0 / 30, 1.0 / 31};
+
+// LDS layout of the matrix-core unit kernel, in doubles.  After a unit's point loop the M blocks
+// [2][N_Q][NK] reuse [0, geo).
+struct MfLayout {
+    int y, a, geo, loss, trig, wts, norm, total;
+};
+
+// 16 x 16 tiles per wave for degrees up to L (2 sides, ceil((2l+1)/16) column blocks per degree,
+// 8 waves): 3 up to L = 9, 5 up to L = 12.
+inline int mf_tiles_per_wave(int L) {
+    int n_side = 0;
+    for (int l = 0; l <= L; ++l) n_side += (2 * l + 16) / 16;
+    const int per_wave = (2 * n_side + MF_WAVES - 1) / MF_WAVES;
+    return per_wave <= 3 ? 3 : 5;
+}
+
+__host__ __device__ inline MfLayout mf_layout(int L, int NK, int Nt1, int Nchi, int Neps) {
+    const int NQ = (L + 1) * (L + 1);
+    MfLayout m;
+    m.y = 0;
+    m.a = MF_ITEMS * mf_stride(NQ) + MF_PAD;
+    m.geo = m.a + MF_ITEMS * mf_stride((L + 1) * NK) + MF_PAD;   // two batches of item geometry
+    m.loss = m.geo + 2 * MF_ITEMS * 5;                // loss_weight per inner t
+    m.trig = m.loss + Nt1;                            // cos/sin eps, mu/sin chi
+    m.wts = m.trig + 2 * Neps + 2 * Nchi;             // W_chi, W_eps
+    m.norm = m.wts + Nchi + Neps;                     // SH_Norm [N_Q], RadialNorm [(L+1) NK], 1/i [32],
+    m.total = m.norm + NQ + (L + 1) * NK + 32 + 32;  // (2i-1)!! [32]
+    return m;
+}
+
+// Unnormalized real SH of order m >= 0 for degrees m..L into Y: P_l^m(x) cos(m phi) at
+// l^2+l+m and P_l^m(x) sin(m phi) at l^2+l-m (eval_SH / legendre_P,
+// compute_rtensor_sumfac_mex.cpp:36-75, with SH_Norm applied later to the summed blocks and the
+// recursion divisions taken as products with 1/(l-m) from rinv).
+__device__ inline void sh_order_m_raw(int L, int m, double x, double cphi, double sphi, const double *rinv,
+                                      double *Y) {
+    double pmm = 1.0;
+    if (m > 0) {
+        const double s = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
+        double fact = 1.0;
+        for (int i = 1; i <= m; ++i) {
+            pmm *= -fact * s;
+            fact += 2.0;
+        }
+    }
+    double cm = 1.0, sm = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const double c = cm * cphi - sm * sphi;
+        sm = sm * cphi + cm * sphi;
+        cm = c;
+    }
+    int qc = m * m + m;           // l^2 + l at l = m
+    double p0 = pmm;              // P_{l-1}^m
+    double p1 = x * (2 * m + 1) * pmm;
+    Y[qc + m] = pmm * cm;
+    if (m > 0) Y[qc - m] = pmm * sm;
+    if (m == L) return;
+    qc += 2 * m + 2;              // l = m + 1
+    Y[qc + m] = p1 * cm;
+    if (m > 0) Y[qc - m] = p1 * sm;
+    double a = 2 * m + 3, bcoef = 2 * m + 1;   // 2l-1 and l+m-1 at l = m + 2
+    for (int l = m + 2; l <= L; ++l) {
+        const double P = (x * a * p1 - bcoef * p0) * rinv[l - m];
+        p0 = p1;
+        p1 = P;
+        qc += 2 * l;              // l^2 + l from (l-1)^2 + (l-1)
+        Y[qc + m] = P * cm;
+        if (m > 0) Y[qc - m] = P * sm;
+        a += 2.0;
+        bcoef += 1.0;
+    }
+}
+
+// w v^l L_k^{(l+1/2)}(v^2) for all k at one (l, v) into out[k] (eval_radial,
+// compute_rtensor_sumfac_mex.cpp:14-34, with RadialNorm applied later to the summed blocks and the
+// division by k taken as a product with the constant 1/k).
+template <int NK>
+__device__ inline void radial_raw(int l, double v, double w, double *out) {
+    double vl = w;
+    for (int i = 0; i < l; ++i) vl *= v;
+    const double v2 = v * v, al = (double)l + 0.5;
+    double L0 = 1.0, L1 = 1.0 + al - v2;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        double Lk;
+        if (k == 0)
+            Lk = 1.0;
+        else if (k == 1)
+            Lk = L1;
+        else {
+            Lk = ((2.0 * (k - 1) + 1.0 + al - v2) * L1 - ((k - 1) + al) * L0) * (1.0 / (double)k);
+            L0 = L1;
+            L1 = Lk;
+        }
+        out[k] = Lk * vl;
+    }
+}
+
+// (c + i s)^m by binary powering: cos(m phi), sin(m phi) from cos(phi), sin(phi).
+__device__ inline void cis_pow(int m, double c, double s, double &cm, double &sm) {
+    double rc = 1.0, rs = 0.0, bc = c, bs = s;
+    while (m) {
+        if (m & 1) {
+            const double t = rc * bc - rs * bs;
+            rs = rc * bs + rs * bc;
+            rc = t;
+        }
+        m >>= 1;
+        if (m) {
+            const double t = bc * bc - bs * bs;
+            bs = 2.0 * bc * bs;
+            bc = t;
+        }
+    }
+    cm = rc;
+    sm = rs;
+}
+
+// x^m by binary powering.
+__device__ inline double ipow(double x, int m) {
+    double r = 1.0;
+    while (m) {
+        if (m & 1) r *= x;
+        m >>= 1;
+        if (m) x *= x;
+    }
+    return r;
+}
+
+// Unnormalized real SH of the two orders m0 <= m1 = L - m0, degrees m..L, in one loop (two
+// independent recursions per step).  P_m^m = (-1)^m (2m-1)!! s^m with s = sqrt(1 - x^2) and
+// (2m-1)!! from dfact; other conventions as sh_order_m_raw.
+__device__ inline void sh_pair_raw(int L, int m0, double x, double cphi, double sphi, const double *rinv,
+                                   const double *dfact, double *Y) {
+    const int m1 = L - m0;
+    const bool two = m1 != m0;
+    const double s = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
+    double c0, s0, c1, s1;
+    cis_pow(m0, cphi, sphi, c0, s0);
+    cis_pow(m1, cphi, sphi, c1, s1);
+    const double pa = ((m0 & 1) ? -dfact[m0] : dfact[m0]) * ipow(s, m0);
+    const double pb = ((m1 & 1) ? -dfact[m1] : dfact[m1]) * ipow(s, m1);
+    // degree m and m + 1
+    int qa = m0 * m0 + m0, qb = m1 * m1 + m1;
+    double a0 = pa, a1 = x * (2 * m0 + 1) * pa;
+    double b0 = pb, b1 = x * (2 * m1 + 1) * pb;
+    Y[qa + m0] = a0 * c0;
+    if (m0 > 0) Y[qa - m0] = a0 * s0;
+    if (two) {
+        Y[qb + m1] = b0 * c1;
+        Y[qb - m1] = b0 * s1;
+    }
+    if (m0 == L) return;
+    qa += 2 * m0 + 2;
+    Y[qa + m0] = a1 * c0;
+    if (m0 > 0) Y[qa - m0] = a1 * s0;
+    if (two && m1 < L) {
+        qb += 2 * m1 + 2;
+        Y[qb + m1] = b1 * c1;
+        Y[qb - m1] = b1 * s1;
+    }
+    double aa = 2 * m0 + 3, ab = 2 * m0 + 1;   // 2l-1, l+m-1 of chain a at l = m0 + 2
+    double ba = 2 * m1 + 3, bb = 2 * m1 + 1;
+    for (int i = 2; i <= L - m0; ++i) {
+        const int la = m0 + i;
+        const double P = (x * aa * a1 - ab * a0) * rinv[i];
+        a0 = a1;
+        a1 = P;
+        qa += 2 * la;
+        Y[qa + m0] = P * c0;
+        if (m0 > 0) Y[qa - m0] = P * s0;
+        aa += 2.0;
+        ab += 1.0;
+        if (two && i <= L - m1) {
+            const int lb = m1 + i;
+            const double Q = (x * ba * b1 - bb * b0) * rinv[i];
+            b0 = b1;
+            b1 = Q;
+            qb += 2 * lb;
+            Y[qb + m1] = Q * c1;
+            Y[qb - m1] = Q * s1;
+            ba += 2.0;
+            bb += 1.0;
+        }
+    }
+}
+
+// w v^l L_k^{(l+1/2)}(v^2) for the two degrees l0 and l1 = L - l0 (if different), the two
+// Laguerre recursions interleaved; outputs at A[l * NK + k] (conventions as radial_raw).
+template <int NK>
+__device__ inline void radial_pair_raw(int L, int l0, double v, double w, double *A) {
+    const int l1 = L - l0;
+    const bool two = l1 != l0;
+    const double v2 = v * v;
+    const double w0 = w * ipow(v, l0), w1 = w * ipow(v, l1);
+    const double al0 = (double)l0 + 0.5, al1 = (double)l1 + 0.5;
+    double P0 = 1.0, P1 = 1.0 + al0 - v2, Q0 = 1.0, Q1 = 1.0 + al1 - v2;
+    double *o0 = A + l0 * NK, *o1 = A + l1 * NK;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        double Pk, Qk;
+        if (k == 0) {
+            Pk = 1.0;
+            Qk = 1.0;
+        } else if (k == 1) {
+            Pk = P1;
+            Qk = Q1;
+        } else {
+            Pk = ((2.0 * (k - 1) + 1.0 + al0 - v2) * P1 - ((k - 1) + al0) * P0) * (1.0 / (double)k);
+            Qk = ((2.0 * (k - 1) + 1.0 + al1 - v2) * Q1 - ((k - 1) + al1) * Q0) * (1.0 / (double)k);
+            P0 = P1;
+            P1 = Pk;
+            Q0 = Q1;
+            Q1 = Qk;
+        }
+        o0[k] = Pk * w0;
+        if (two) o1[k] = Qk * w1;
+    }
+}
+
+// Tile t of a unit's M blocks: side, degree l and column block c (columns m = 16c .. 16c+15).
+__device__ inline void mf_tile(int t, int L, int &side, int &l, int &c) {
+    int n_side = 0;
+    for (int ll = 0; ll <= L; ++ll) n_side += (2 * ll + 16) / 16;
+    side = t >= n_side;
+    int r = t - side * n_side;
+    l = 0;
+    while (r >= (2 * l + 16) / 16) {
+        r -= (2 * l + 16) / 16;
+        ++l;
+    }
+    c = r;
+}
+
+// Stage A, matrix-core form.  One workgroup per unit; block b maps to the patch-1 units of the
+// chunk first (they carry N_t1 inner steps each, so they start early), then the patch-2 units.
+// Storage index of a unit: (x - x0) * Ux + r, the reference's unit order.
+// Outputs as the per-channel kernel: net[(unit*NL + tc)*2*NK + side*NK + k1], rad[unit][side][l][k].
+template <int NK, int MT>
+__global__ __launch_bounds__(MF_THREADS, 4) void sumfac_units_mf_kernel(SumfacDev d, int x0, int nx,
+                                                                        double *__restrict__ net,
+                                                                        double *__restrict__ rad) {
+    extern __shared__ double lds[];
+    __shared__ double sRadU[2][(MAXL + 1) * MAXK];
+    __shared__ TGeom sG;
+
+    const int L = d.L, NQ = d.NQ, NL = d.NL;
+    const int AST = mf_stride((L + 1) * NK), YST = mf_stride(NQ);
+    const MfLayout lay = mf_layout(L, NK, d.Nt1, d.Nchi, d.Neps);
+    double *sY = lds + lay.y;
+    double *sA = lds + lay.a;
+    double *sGeo = lds + lay.geo;
+    double *sLw = lds + lay.loss;
+    double *sCE = lds + lay.trig, *sSE = sCE + d.Neps, *sCC = sSE + d.Neps, *sSC = sCC + d.Nchi;
+    double *sWchi = lds + lay.wts, *sWeps = sWchi + d.Nchi;
+    double *sShN = lds + lay.norm, *sRadN = sShN + NQ, *sRinv = sRadN + (L + 1) * NK, *sDfact = sRinv + 32;
+    double *sM = lds;
+
+    const int Ux = d.Nu1 + d.Ny2 * d.Nt2;
+    const int b = blockIdx.x;
+    int xl, r;
+    if (b < nx * d.Nu1) {
+        xl = b / d.Nu1;
+        r = b % d.Nu1;
+    } else {
+        const int bb = b - nx * d.Nu1;
+        xl = bb / (d.Ny2 * d.Nt2);
+        r = d.Nu1 + bb % (d.Ny2 * d.Nt2);
+    }
+    const int64_t unit = (int64_t)xl * Ux + r;
+    const int xi = x0 + xl;
+    const bool patch1 = r < d.Nu1;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ns_all = d.qi_off[NL];
+    const double x_i = d.x[xi];
+
+    for (int i = tid; i < d.Neps; i += MF_THREADS) {
+        sCE[i] = cos(d.eps[i]);
+        sSE[i] = sin(d.eps[i]);
+        sWeps[i] = d.W_eps[i];
+    }
+    for (int i = tid; i < d.Nchi; i += MF_THREADS) {
+        const double c = d.mu_chi[i];
+        sCC[i] = c;
+        sSC[i] = sqrt(fmax(1.0 - c * c, 0.0));
+        sWchi[i] = d.W_chi[i];
+    }
+    for (int i = tid; i < NQ; i += MF_THREADS) sShN[i] = d.shnorm[i];
+    for (int i = tid; i < (L + 1) * NK; i += MF_THREADS) sRadN[i] = d.radnorm[i];
+    for (int i = tid; i < 32; i += MF_THREADS) sRinv[i] = c_rinv[i];
+    if (tid == 0) {
+        double f = 1.0;
+        sDfact[0] = 1.0;
+        for (int i = 1; i < 32; ++i) {
+            f *= (double)(2 * i - 1);
+            sDfact[i] = f;
+        }
+    }
+    // operand pads: read by products that land in unused rows/columns, kept finite
+    for (int i = tid; i < lay.geo; i += MF_THREADS) lds[i] = 0.0;
+
+    double v_large, v_small, Jac_unit;
+    int u_idx = 0, y_idx = 0, t2_idx = 0;
+    if (patch1) {
+        u_idx = r;
+        const double u = d.u1[u_idx];
+        v_large = sqrt(x_i / 2.0) * (1.0 + u);
+        v_small = sqrt(x_i / 2.0) * (1.0 - u);
+        Jac_unit = 0.5 * d.Wx[xi] * d.Wu1[u_idx] * u * exp(-x_i * u * u) * (v_large * v_large * v_small * v_small);
+    } else {
+        y_idx = (r - d.Nu1) / d.Nt2;
+        t2_idx = (r - d.Nu1) % d.Nt2;
+        const double y = d.y2[y_idx];
+        Jac_unit = 0.5 * d.Wx[xi] * d.Wy2[y_idx] * y * (4.0 * y);
+        const double u = y * d.t2[t2_idx];
+        v_large = sqrt(x_i / 2.0) * (1.0 + u);
+        v_small = sqrt(x_i / 2.0) * (1.0 - u);
+    }
+    if (tid < 2 * (L + 1)) {
+        const int side = tid / (L + 1), l = tid % (L + 1);
+        double tmp[NK];
+        radial_all_k<NK>(l, side ? v_small : v_large, d.radnorm, tmp);
+#pragma unroll
+        for (int k = 0; k < NK; ++k) sRadU[side][l * MAXK + k] = tmp[k];
+    }
+
+    // this wave's tiles
+    int n_side = 0;
+    for (int ll = 0; ll <= L; ++ll) n_side += (2 * ll + 16) / 16;
+    const int n_tiles = 2 * n_side;
+    // tiles t = wave + 4 i; past n_tiles a slot repeats tile 0 (its result unused), so the product
+    // loop has no branches
+    int t_aoff[MT], t_boff[MT], t_l[MT], t_c[MT], t_side[MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+        const int t = wave + MF_WAVES * i;
+        int side = 0, l = 0, c = 0;
+        mf_tile(t < n_tiles ? t : 0, L, side, l, c);
+        t_side[i] = side;
+        t_l[i] = l;
+        t_c[i] = c;
+        t_aoff[i] = side * MF_NPB * AST + l * NK;
+        t_boff[i] = side * MF_NPB * YST + l * l + 16 * c;
+    }
+    const int i16 = lane & 15, pq = lane >> 4;
+
+    const int n_inner_t = patch1 ? d.Nt1 : 1;
+    const int n_gain = d.Nchi * d.Neps;
+    const int n_out = NL * 2 * NK;
+    double *netu = net + unit * n_out;
+    // evaluation tasks per batch: (item, j) covers SH orders (or radial degrees) j and L - j, so
+    // every task runs L + 2 recursion steps; SH tasks on lanes [0, 128), radial tasks on
+    // [128, MF_GEO_LANE), the next batch's geometry on the last MF_ITEMS lanes
+    const int npair = (L + 2) / 2;
+    const int n_ev = MF_ITEMS * npair;
+    // geometry of the items of the batch at p0 into sGeo buffer gb (one thread per item)
+    auto geometry = [&](int item, int p0, int gb) {
+        const int side = item / MF_NPB, j = item % MF_NPB;
+        const int p = p0 + j;
+        const int pc = p < n_gain ? p : n_gain - 1;
+        const int n = pc / d.Neps, e = pc % d.Neps;
+        double vp_mag, ct, cp, sp;
+        post_collision(sG, sCC[n], sSC[n], sCE[e], sSE[e], side, vp_mag, ct, cp, sp);
+        double *g = sGeo + (gb * MF_ITEMS + item) * 5;
+        g[0] = vp_mag;
+        g[1] = ct;
+        g[2] = cp;
+        g[3] = sp;
+        g[4] = p < n_gain ? (sWchi[n] * sG.B_val * sG.Jac_t) * sWeps[e] : 0.0;
+    };
+    for (int it = 0; it < n_inner_t; ++it) {
+        __syncthreads();  // sG, sM of the previous step
+        if (tid == 0) {
+            if (patch1) {
+                const double u = d.u1[u_idx], t = d.t1[it];
+                const double y = u * t;
+                const double Jac_t = d.Wt1[it] * (4.0 * u * t);
+                const double c_beta = 1.0 - 2.0 * y * y;
+                const double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, EPS_SAFE));
+                const double u_mag = sqrt(2.0 * x_i) * u * sqrt(1.0 + (1.0 - u * u) * t * t);
+                make_tgeom(sG, x_i, v_large, v_small, c_beta, s_beta, u_mag, Jac_t, d.alpha);
+            } else {
+                const double y = d.y2[y_idx], t = d.t2[t2_idx];
+                const double u = y * t;
+                const double c_beta = 1.0 - 2.0 * y * y;
+                const double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, EPS_SAFE));
+                const double Jac_t = d.Wt2[t2_idx] * exp(-x_i * u * u) * (v_large * v_large * v_small * v_small);
+                const double u_mag = sqrt(2.0 * x_i) * y * sqrt(t * t + 1.0 - y * y * t * t);
+                make_tgeom(sG, x_i, v_large, v_small, c_beta, s_beta, u_mag, Jac_t, d.alpha);
+            }
+        }
+        __syncthreads();
+        if (tid < MF_ITEMS) geometry(tid, 0, 0);
+        v4d acc[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) acc[i] = v4d{0.0, 0.0, 0.0, 0.0};
+
+        int gb = 0;
+        for (int p0 = 0; p0 < n_gain; p0 += MF_NPB, gb ^= 1) {
+            __syncthreads();  // this batch's geometry written; the previous batch's operands consumed
+            const double *geo = sGeo + gb * MF_ITEMS * 5;
+            if (tid < MF_SH_LANES) {
+                for (int t = tid; t < n_ev; t += MF_SH_LANES) {
+                    const int item = t % MF_ITEMS, j = t / MF_ITEMS;
+                    const double *g = geo + 5 * item;
+                    sh_pair_raw(L, j, g[1], g[2], g[3], sRinv, sDfact, sY + item * YST);
+                }
+            } else if (tid < MF_GEO_LANE) {
+                for (int t = tid - MF_SH_LANES; t < n_ev; t += MF_GEO_LANE - MF_SH_LANES) {
+                    const int item = t % MF_ITEMS, j = t / MF_ITEMS;
+                    const double *g = geo + 5 * item;
+                    radial_pair_raw<NK>(L, j, g[0], g[4], sA + item * AST);
+                }
+            } else if (tid < MF_GEO_LANE + MF_ITEMS && p0 + MF_NPB < n_gain) {
+                geometry(tid - MF_GEO_LANE, p0 + MF_NPB, gb ^ 1);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int g4 = 0; g4 < MF_NPB / 4; ++g4) {
+                const int pt = 4 * g4 + pq;
+                double a[MT], bb[MT];
+#pragma unroll
+                for (int i = 0; i < MT; ++i) {
+                    a[i] = sA[t_aoff[i] + pt * AST + i16];
+                    bb[i] = sY[t_boff[i] + pt * YST + i16];
+                }
+#pragma unroll
+                for (int i = 0; i < MT; ++i) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[i], acc[i], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // operand reads done: [0, geo) becomes the M blocks
+        // D layout of v_mfma_f64_16x16x4_f64: register r of a lane holds row 4 r + lane / 16, column
+        // lane % 16 (the f32 16x16x4 form holds rows 4 (lane / 16) + r instead).  The blocks take
+        // their SH_Norm and RadialNorm factors here.
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+            if (wave + MF_WAVES * i < n_tiles) {
+                const int l = t_l[i], m = 16 * t_c[i] + i16;
+                const int q = l * l + m;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int k = 4 * rr + pq;
+                    if (k < NK && m < 2 * l + 1)
+                        sM[(t_side[i] * NQ + q) * NK + k] = acc[i][rr] * (sShN[q] * sRadN[k + NK * l]);
+                }
+            }
+        }
+        __syncthreads();
+        // slot contraction: one (t_chan, side) per thread, all k
+        const double *Pv = patch1 ? d.Pg1 + ((int64_t)u_idx * d.Nt1 + it) * ns_all : d.Pg2 + (int64_t)y_idx * ns_all;
+        for (int pr = tid; pr < 2 * NL; pr += MF_THREADS) {
+            const int tc = pr >> 1, side = pr & 1;
+            const int s0 = d.qi_off[tc], s1 = d.qi_off[tc + 1];
+            double gk[NK];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) gk[k] = 0.0;
+            for (int s = s0; s < s1; ++s) {
+                const double P = Pv[s];
+                const double *Mq = sM + (side * NQ + d.qi[s]) * NK;
+#pragma unroll
+                for (int k = 0; k < NK; ++k) gk[k] += P * Mq[k];
+            }
+            double *o = netu + pr * NK;
+            if (it == 0) {
+#pragma unroll
+                for (int k = 0; k < NK; ++k) o[k] = gk[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < NK; ++k) o[k] += gk[k];
+            }
+        }
+        if (tid == 0) sLw[it] = sG.loss_weight;
+    }
+    __syncthreads();
+    // loss: the radial factor is fixed per unit, so only loss_weight * P_loss is summed over t
+    for (int pr = tid; pr < 2 * NL; pr += MF_THREADS) {
+        const int tc = pr >> 1, side = pr & 1;
+        const int l1 = d.trip[3 * tc];
+        double ls = 0.0;
+        if (patch1) {
+            for (int it = 0; it < d.Nt1; ++it) ls += sLw[it] * d.Pl1[((int64_t)u_idx * d.Nt1 + it) * NL + tc];
+        } else {
+            ls = sLw[0] * d.Pl2[(int64_t)y_idx * NL + tc];
+        }
+        double *o = netu + pr * NK;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) o[k] = Jac_unit * (o[k] - sRadU[side][l1 * MAXK + k] * ls);
+    }
+    double *ro = rad + unit * 2 * (L + 1) * NK;
+    for (int i = tid; i < 2 * (L + 1) * NK; i += MF_THREADS) {
+        const int side = i / ((L + 1) * NK), rem = i % ((L + 1) * NK);
+        ro[i] = sRadU[side][(rem / NK) * MAXK + rem % NK];
+    }
+}
+
+// Stage B, matrix-core form: for one t_chan tc and one block of 16 (k2, k3) pairs,
+//     R[k1, (k2,k3), tc] += sum_u net1[u][tc][k1] r2l[u][k2] r3s[u][k3] + net2[u][tc][k1] r2s[u][k2] r3l[u][k3]
+// as two products over the unit dimension on v_mfma_f64_16x16x4_f64 (rows k1, columns the pairs,
+// inner 4 units per instruction).  One wave per (tc, pair block); two accumulators per term over
+// alternating unit quads keep consecutive instructions independent.
+template <int NK>
+__global__ __launch_bounds__(64) void sumfac_fold_mf_kernel(int NL, int L, const int *__restrict__ trip,
+                                                            int64_t n_units, const double *__restrict__ net,
+                                                            const double *__restrict__ rad, double *__restrict__ R) {
+    constexpr int NPAIR = NK * NK, NBLK = (NPAIR + 15) / 16;
+    const int tc = blockIdx.x / NBLK, blk = blockIdx.x % NBLK;
+    const int lane = threadIdx.x, i16 = lane & 15, uq = lane >> 4;
+    const int l2 = trip[3 * tc + 1], l3 = trip[3 * tc + 2];
+    const int pair = 16 * blk + i16;
+    const bool pair_ok = pair < NPAIR;
+    const int k2 = pair_ok ? pair / NK : 0, k3 = pair_ok ? pair % NK : 0;
+    const int k1 = i16 < NK ? i16 : NK - 1;
+    const int rstride = 2 * (L + 1) * NK;
+    const int o2l = l2 * NK + k2, o2s = (L + 1) * NK + l2 * NK + k2;
+    const int o3l = l3 * NK + k3, o3s = (L + 1) * NK + l3 * NK + k3;
+    const double arow = i16 < NK ? 1.0 : 0.0;
+    const double bcol = pair_ok ? 1.0 : 0.0;
+    v4d a1[2] = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
+    v4d a2[2] = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
+    const int64_t nstride = (int64_t)NL * 2 * NK;
+    const double *nbase = net + (int64_t)tc * 2 * NK + k1;
+    int64_t u0 = 0;
+    for (; u0 + 8 <= n_units; u0 += 8) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t u = u0 + 4 * h + uq;
+            const double *nu = nbase + u * nstride;
+            const double *ru = rad + u * rstride;
+            const double n1 = nu[0] * arow, n2 = nu[NK] * arow;
+            const double b1 = ru[o2l] * ru[o3s] * bcol, b2 = ru[o2s] * ru[o3l] * bcol;
+            a1[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(n1, b1, a1[h], 0, 0, 0);
+            a2[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(n2, b2, a2[h], 0, 0, 0);
+        }
+    }
+    for (; u0 < n_units; u0 += 4) {
+        const int64_t u = u0 + uq;
+        const bool ok = u < n_units;
+        const int64_t uc = ok ? u : n_units - 1;
+        const double *nu = nbase + uc * nstride;
+        const double *ru = rad + uc * rstride;
+        const double z = ok ? 1.0 : 0.0;
+        const double n1 = nu[0] * arow * z, n2 = nu[NK] * arow * z;
+        const double b1 = ru[o2l] * ru[o3s] * bcol, b2 = ru[o2s] * ru[o3l] * bcol;
+        a1[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(n1, b1, a1[0], 0, 0, 0);
+        a2[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(n2, b2, a2[0], 0, 0, 0);
+    }
+    if (!pair_ok) return;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        const int kk = 4 * rr + uq;
+        if (kk < NK) {
+            const int64_t o = kk + (int64_t)NK * (k2 + (int64_t)NK * (k3 + (int64_t)NK * tc));
+            R[o] += (a1[0][rr] + a1[1][rr]) + (a2[0][rr] + a2[1][rr]);
+        }
+    }
+}
+
 __global__ void conservation_zero_kernel(int NK, int NL, const int *__restrict__ trip, double *__restrict__ R) {
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t total = (int64_t)NK * NK * NK * NL;
@@ -406,6 +1005,33 @@ __global__ void conservation_zero_kernel(int NK, int NL, const int *__restrict__
     const int l1 = trip[3 * tc];
     if ((l1 == 0 && k1 <= 1) || (l1 == 1 && k1 == 0)) R[o] = 0.0;
 }
+
+template <int NK, int MT>
+int launch_units_mf_t(size_t dyn, const SumfacDev &d, int xc, int nx, int64_t n_units, double *net, double *rad) {
+    CFK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&sumfac_units_mf_kernel<NK, MT>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    hipLaunchKernelGGL((sumfac_units_mf_kernel<NK, MT>), dim3((unsigned)n_units), dim3(MF_THREADS), dyn, nullptr, d,
+                       xc, nx, net, rad);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+template <int NK>
+int launch_units_mf(int mt, size_t dyn, const SumfacDev &d, int xc, int nx, int64_t n_units, double *net,
+                    double *rad) {
+    if (mt == 3) return launch_units_mf_t<NK, 3>(dyn, d, xc, nx, n_units, net, rad);
+    return launch_units_mf_t<NK, 5>(dyn, d, xc, nx, n_units, net, rad);
+}
+
+int g_sumfac_kernel = CFK_SUMFAC_MATRIX_CORE;
+
+// Device time of the last quadrature call (HIP events around each launch on its stream).
+struct SumfacTiming {
+    int kind = 0;
+    int64_t units = 0;
+    double units_ms = 0.0, fold_ms = 0.0;
+};
+SumfacTiming g_last_timing;
 
 struct SumfacBuffers {
     DevBuf<double> grid[14], radnorm, shnorm, Pg1, Pl1, Pg2, Pl2, net, rad, R;
@@ -490,20 +1116,85 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     x_begin = std::max(0, x_begin);
     x_end = std::min(d.Nx, x_end);
     const int64_t Ux = (int64_t)d.Nu1 + (int64_t)d.Ny2 * d.Nt2;
-    // chunk of x nodes so the per-unit buffers stay below ~1 GiB
+    // The matrix-core form takes each slot's radial factor at the degree of its q_i; it equals the
+    // reference's l1 when every valid q_i of a t_chan has degree l1 (true of the tables built from
+    // the Gaunt COO).  Other tables run on the per-channel kernel.
+    bool slots_on_l1 = true;
+    for (int t = 0; t < NL && slots_on_l1; ++t)
+        for (int s = T.qi_off[t]; s < T.qi_off[t + 1]; ++s)
+            if (l_of_q0(T.qi[s]) != T.trip[3 * t]) slots_on_l1 = false;
+    const bool mf = g_sumfac_kernel == CFK_SUMFAC_MATRIX_CORE && slots_on_l1;
+    g_last_timing = SumfacTiming{};
+    g_last_timing.kind = mf ? CFK_SUMFAC_MATRIX_CORE : CFK_SUMFAC_PER_CHANNEL;
+    hipEvent_t ev[3];
+    for (auto &e : ev) CFK_HIP(hipEventCreate(&e));
+    struct EvGuard {
+        hipEvent_t *e;
+        ~EvGuard() {
+            for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } ev_guard{ev};
+    auto lap = [&](double &acc_ms, hipEvent_t a, hipEvent_t b2) -> int {
+        CFK_HIP(hipEventSynchronize(b2));
+        float ms = 0.f;
+        CFK_HIP(hipEventElapsedTime(&ms, a, b2));
+        acc_ms += ms;
+        return CFK_OK;
+    };
+    // chunk of x nodes so the per-unit buffers stay below 1 GiB (per-channel kernel) or 4 GiB
+    // (matrix-core kernel: more patch-1 units per launch balance the workgroups)
     const int64_t per_unit = (int64_t)NL * 2 * NK + 2 * (L + 1) * NK;
-    int64_t xs_per_chunk = std::max<int64_t>(1, (int64_t)(1 << 27) / (per_unit * Ux));
+    const int64_t cap = mf ? ((int64_t)1 << 29) : ((int64_t)1 << 27);
+    int64_t xs_per_chunk = std::max<int64_t>(1, cap / (per_unit * Ux));
     if (x_end > x_begin) {
         const int64_t max_units = std::min<int64_t>(xs_per_chunk, x_end - x_begin) * Ux;
         if ((rc = b.net.alloc((size_t)(max_units * NL * 2 * NK)))) return rc;
         if ((rc = b.rad.alloc((size_t)(max_units * 2 * (L + 1) * NK)))) return rc;
     }
+    size_t mf_dyn = 0;
+    if (mf) {
+        const MfLayout lay = mf_layout(L, NK, d.Nt1, d.Nchi, d.Neps);
+        if (2 * NQ * NK > lay.geo) return set_error(CFK_EUNSUP, "sumfac: M blocks exceed the operand area");
+        if (MF_ITEMS * ((L + 2) / 2) > std::min(MF_SH_LANES, MF_GEO_LANE - MF_SH_LANES))
+            return set_error(CFK_EUNSUP, "sumfac: L=%d needs more evaluation lanes", L);
+        mf_dyn = (size_t)lay.total * sizeof(double);
+        if (mf_dyn + 8 * 1024 > 160 * 1024)
+            return set_error(CFK_EUNSUP, "sumfac: %zu bytes of LDS per workgroup exceed the budget", mf_dyn);
+    }
+    if (x_end > x_begin) {  // equal chunks
+        const int64_t n_chunks = (x_end - x_begin + xs_per_chunk - 1) / xs_per_chunk;
+        xs_per_chunk = (x_end - x_begin + n_chunks - 1) / n_chunks;
+    }
     for (int xc = x_begin; xc < x_end; xc += (int)xs_per_chunk) {
         const int nx = (int)std::min<int64_t>(xs_per_chunk, x_end - xc);
         const int64_t n_units = (int64_t)nx * Ux;
         if (n_units > 0x7fffffffLL) return set_error(CFK_EINVAL, "sumfac: too many units per chunk");
+        if (mf) {
+            switch (NK) {
+#define X(KK)                                                                                                    \
+    case KK: {                                                                                                   \
+        CFK_HIP(hipEventRecord(ev[0], nullptr));                                                                 \
+        if ((rc = launch_units_mf<KK>(mf_tiles_per_wave(L), mf_dyn, d, xc, nx, n_units, b.net.p, b.rad.p)))      \
+            return rc;                                                                                           \
+        CFK_HIP(hipEventRecord(ev[1], nullptr));                                                                 \
+        constexpr int nblk = (KK * KK + 15) / 16;                                                                \
+        hipLaunchKernelGGL(sumfac_fold_mf_kernel<KK>, dim3((unsigned)(NL * nblk)), dim3(64), 0, nullptr, NL, L,  \
+                           b.trip.p, n_units, b.net.p, b.rad.p, b.R.p);                                          \
+        CFK_HIP(hipGetLastError());                                                                              \
+        CFK_HIP(hipEventRecord(ev[2], nullptr));                                                                 \
+        break;                                                                                                   \
+    }
+                X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+#undef X
+            }
+            if ((rc = lap(g_last_timing.units_ms, ev[0], ev[1]))) return rc;
+            if ((rc = lap(g_last_timing.fold_ms, ev[1], ev[2]))) return rc;
+            g_last_timing.units += n_units;
+            continue;
+        }
         dim3 grid((unsigned)n_units, (unsigned)((NL + SF_THREADS - 1) / SF_THREADS));
         const size_t dyn = (size_t)d.max_block_slots * (sizeof(double) + sizeof(int));
+        CFK_HIP(hipEventRecord(ev[0], nullptr));
         switch (NK) {
 #define X(KK)                                                                                                   \
     case KK:                                                                                                    \
@@ -514,9 +1205,14 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
 #undef X
         }
         CFK_HIP(hipGetLastError());
+        CFK_HIP(hipEventRecord(ev[1], nullptr));
         hipLaunchKernelGGL(sumfac_fold_kernel, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, nullptr, NK, NL, L,
                            b.trip.p, n_units, b.net.p, b.rad.p, b.R.p);
         CFK_HIP(hipGetLastError());
+        CFK_HIP(hipEventRecord(ev[2], nullptr));
+        if ((rc = lap(g_last_timing.units_ms, ev[0], ev[1]))) return rc;
+        if ((rc = lap(g_last_timing.fold_ms, ev[1], ev[2]))) return rc;
+        g_last_timing.units += n_units;
     }
     if (zero_conservation) {
         hipLaunchKernelGGL(conservation_zero_kernel, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, nullptr, NK, NL,
@@ -582,6 +1278,21 @@ extern "C" int cfk_compute_rtensor_sumfac(
     G.eps.assign(eps_vec, eps_vec + N_eps);
     G.W_eps.assign(W_eps, W_eps + N_eps);
     return run_sumfac(T, G, NK, alpha, x_begin, x_end, false, R_out);
+}
+
+extern "C" int cfk_sumfac_set_kernel(int32_t kind) {
+    if (kind != CFK_SUMFAC_MATRIX_CORE && kind != CFK_SUMFAC_PER_CHANNEL)
+        return set_error(CFK_EINVAL, "unknown sumfac kernel %d", kind);
+    g_sumfac_kernel = kind;
+    return CFK_OK;
+}
+
+extern "C" int cfk_sumfac_last_timing(int32_t *kind, int64_t *n_units, double *units_ms, double *fold_ms) {
+    if (kind) *kind = g_last_timing.kind;
+    if (n_units) *n_units = g_last_timing.units;
+    if (units_ms) *units_ms = g_last_timing.units_ms;
+    if (fold_ms) *fold_ms = g_last_timing.fold_ms;
+    return CFK_OK;
 }
 
 extern "C" int cfk_build_rtensor(double *R_out, int32_t K_max, int32_t L, double alpha, const int32_t grid_sizes[7],

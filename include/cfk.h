@@ -82,7 +82,8 @@ int cfk_dense_tensor_kernel(double *Q, const double *f, const double *C, int64_t
  * The 26 MEX inputs in MEX order, each array followed by its element count where the MEX reads
  * one with mxGetNumberOfElements.  R_out (N_K^3*N_L doubles, N_K = K_max+1) is OVERWRITTEN with
  * the contribution of the x nodes [x_begin, x_end) (0, N_x for the full tensor), as the MEX
- * returns a fresh array.  No conservation zeroing (the MATLAB host applies it afterwards). */
+ * returns a fresh array.  No conservation zeroing (the MATLAB host applies it afterwards).
+ * The quadrature runs on the kernel chosen by cfk_sumfac_set_kernel. */
 int cfk_compute_rtensor_sumfac(double *R_out, int32_t K_max, int32_t N_L, int32_t N_Q, double alpha,
                                const double *x_nodes, const double *W_x, int32_t N_x,
                                const double *u1_nodes, const double *W_u1, int32_t N_u1,
@@ -94,6 +95,21 @@ int cfk_compute_rtensor_sumfac(double *R_out, int32_t K_max, int32_t N_L, int32_
                                const double *RadialNorm, const double *SH_Norm, const double *L_triplets,
                                const double *qi_valid_mat, const double *P_loss_p1, const double *P_gain_p1,
                                const double *P_loss_p2, const double *P_gain_p2, int32_t x_begin, int32_t x_end);
+
+/* Kernel for the sumfac quadrature (process-wide; set it before the calls it should govern).
+ * CFK_SUMFAC_MATRIX_CORE (default): per inner-t step, the gain-point sum is taken once per SH
+ *   degree on FP64 matrix cores (M(l,k,q) = sum_p w Rad(p,l,k) Y(p,q)) and then contracted with
+ *   each t_chan's P_gain slots; the unit fold is a matrix-core product over units.
+ * CFK_SUMFAC_PER_CHANNEL: one lane per t_chan sums over the gain points in the reference's order
+ *   (compute_rtensor_sumfac_mex.cpp:219-234), on the vector ALUs.
+ * Both are FP64 and agree to rounding.  No counterpart in the reference (one code path there). */
+#define CFK_SUMFAC_MATRIX_CORE 0
+#define CFK_SUMFAC_PER_CHANNEL 1
+int cfk_sumfac_set_kernel(int32_t kind);
+/* Device time of the last sumfac call: the kernel that ran (tables whose valid q_i are not all of
+ * degree l1 run per channel), the units processed, and the summed HIP-event durations of the
+ * unit (stage A) and fold (stage B) launches.  Any pointer may be NULL.  No reference counterpart. */
+int cfk_sumfac_last_timing(int32_t *kind, int64_t *n_units, double *units_ms, double *fold_ms);
 
 /* ---------------------------------------------------------------------------------------
  * Host-side preparation (the MATLAB host code of the reference, native here).
@@ -113,8 +129,8 @@ int cfk_gauss_gen_laguerre(int32_t n, double alpha, double *x, double *w);
  * (src/GeneralCollisionTensor.m:55-64). */
 int cfk_sumfac_grid_sizes(int32_t K_max, int32_t L, int32_t radial_pad, int32_t angular_pad, int32_t sizes[7]);
 
-/* Build R [N_K,N_K,N_K,N_L] on the GPU end to end: P tables (src/GeneralCollisionTensor.m:106-174,
- * on the device and compacted to the valid q_i), the sumfac quadrature, and the conservation
+/* Build R [N_K,N_K,N_K,N_L]: P tables (src/GeneralCollisionTensor.m:106-174, in C++ on the host,
+ * compacted to the valid q_i), the sumfac quadrature on the GPU, and the conservation
  * zeroing (src/GeneralCollisionTensor.m:183-191).  grid_sizes as from cfk_sumfac_grid_sizes (or
  * chosen explicitly); labels/vals/ic_map as returned by cfk_real_gaunt. */
 int cfk_build_rtensor(double *R_out, int32_t K_max, int32_t L, double alpha, const int32_t grid_sizes[7],
