@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="rtensor: skip the per-channel kernel run kept beside the matrix-core one")
     ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate", "copy"],
                     help="hs-l8-k10: the metric; uneven: 4096 distributions with L in {4,6,8,12}, K=L+2 "
                          "(batched item kernels in a hipGraph); rtensor: GPU tensor generation, L=12, K=12, 48^7 "
@@ -286,7 +288,8 @@ def bench_rtensor(args, cfk):
     S = len({(int(t), int(q)) for t, q in zip(geo[2], geo[0][:, 0])})
     res = {}
     timing = {}
-    for kind, gammas in (("matrix_core", (0.0, 1.0)), ("per_channel", (1.0,))):
+    runs = [("matrix_core", (0.0, 1.0))] + ([] if args.no_compare else [("per_channel", (1.0,))])
+    for kind, gammas in runs:
         cfk.set_sumfac_kernel(kind)
         for gamma in gammas:
             t0 = time.perf_counter()
@@ -307,7 +310,8 @@ def bench_rtensor(args, cfk):
                       "gain_points": gain_points, "slots": S},
            "gain_points_per_s": gain_points / res["matrix_core_gamma1_s"],
            "kernel_ms": timing,
-           "speedup_vs_per_channel": res["per_channel_gamma1_s"] / res["matrix_core_gamma1_s"],
+           "speedup_vs_per_channel": (res["per_channel_gamma1_s"] / res["matrix_core_gamma1_s"]
+                                      if "per_channel_gamma1_s" in res else None),
            "roofline": {"bound": "mfma", "kernel": "sumfac_units_mf_kernel", "achieved": achieved, "peak": 78.6,
                         "unit": "TFLOP/s", "frac": achieved / 78.6, "flop_units": f_units, "flop_fold": f_fold,
                         "fold_tflops": f_fold / (tm["fold_ms"] * 1e-3) / 1e12,

@@ -938,13 +938,19 @@ __global__ __launch_bounds__(MF_THREADS, 4) void sumfac_units_mf_kernel(SumfacDe
 //     R[k1, (k2,k3), tc] += sum_u net1[u][tc][k1] r2l[u][k2] r3s[u][k3] + net2[u][tc][k1] r2s[u][k2] r3l[u][k3]
 // as two products over the unit dimension on v_mfma_f64_16x16x4_f64 (rows k1, columns the pairs,
 // inner 4 units per instruction).  One wave per (tc, pair block); two accumulators per term over
-// alternating unit quads keep consecutive instructions independent.
+// alternating unit quads keep consecutive instructions independent.  The pair blocks of one tc
+// read the same unit rows, so they are mapped to one XCD (workgroups go round-robin over the 8
+// XCDs by block index) and share its L2: block b runs on XCD b % 8 and takes tc = 8 g + b % 8,
+// pair block (b / 8) % NBLK with g = b / (8 NBLK).
+constexpr int N_XCD = 8;
 template <int NK>
 __global__ __launch_bounds__(64) void sumfac_fold_mf_kernel(int NL, int L, const int *__restrict__ trip,
                                                             int64_t n_units, const double *__restrict__ net,
                                                             const double *__restrict__ rad, double *__restrict__ R) {
     constexpr int NPAIR = NK * NK, NBLK = (NPAIR + 15) / 16;
-    const int tc = blockIdx.x / NBLK, blk = blockIdx.x % NBLK;
+    const int xcd = blockIdx.x % N_XCD, rblk = blockIdx.x / N_XCD;
+    const int blk = rblk % NBLK, tc = (rblk / NBLK) * N_XCD + xcd;
+    if (tc >= NL) return;
     const int lane = threadIdx.x, i16 = lane & 15, uq = lane >> 4;
     const int l2 = trip[3 * tc + 1], l3 = trip[3 * tc + 2];
     const int pair = 16 * blk + i16;
@@ -1178,7 +1184,9 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
             return rc;                                                                                           \
         CFK_HIP(hipEventRecord(ev[1], nullptr));                                                                 \
         constexpr int nblk = (KK * KK + 15) / 16;                                                                \
-        hipLaunchKernelGGL(sumfac_fold_mf_kernel<KK>, dim3((unsigned)(NL * nblk)), dim3(64), 0, nullptr, NL, L,  \
+        const int ntg = (NL + N_XCD - 1) / N_XCD;                                                               \
+        hipLaunchKernelGGL(sumfac_fold_mf_kernel<KK>, dim3((unsigned)(ntg * N_XCD * nblk)), dim3(64), 0, nullptr, \
+                           NL, L,                                                                                 \
                            b.trip.p, n_units, b.net.p, b.rad.p, b.R.p);                                          \
         CFK_HIP(hipGetLastError());                                                                              \
         CFK_HIP(hipEventRecord(ev[2], nullptr));                                                                 \
