@@ -300,6 +300,21 @@ def bench_rtensor(args, cfk):
             timing[key] = cfk.sumfac_last_timing()
             print("%s: %.3f s, kernels %s" % (key, res[key + "_s"], timing[key]), file=sys.stderr, flush=True)
     cfk.set_sumfac_kernel("matrix_core")
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's sumfac (plain-C restatement of compute_rtensor_sumfac_mex, OpenMP over the
+        # quadrature units like the reference) on one of the 48 x nodes, all host cores
+        sys.path.insert(0, ROOT)
+        import oracle
+        from oracle import geometry as G
+        prep = G.sumfac_prep(K_max, L, 1.0, *[np.asarray(a) for a in geo], G.sumfac_grids(1.0, sizes))
+        cores = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        oracle.sumfac(prep, 20, 21, nthreads=cores)
+        dt = time.perf_counter() - t0
+        cpu = {"value": dt * sizes[0], "unit": "s", "cores": cores, "kind": "port",
+               "sample": "x node 20 of 48 (gamma 1) in %.1f s, scaled by 48; oracle sumfac (restated reference "
+                         "MEX, OpenMP over quadrature units as the reference)" % dt}
     f_units, f_fold = sumfac_flops(L, NK, sizes, S)
     tm = timing["matrix_core_gamma1"]
     achieved = f_units / (tm["units_ms"] * 1e-3) / 1e12
@@ -312,6 +327,7 @@ def bench_rtensor(args, cfk):
            "kernel_ms": timing,
            "speedup_vs_per_channel": (res["per_channel_gamma1_s"] / res["matrix_core_gamma1_s"]
                                       if "per_channel_gamma1_s" in res else None),
+           "cpu_baseline": cpu,
            "roofline": {"bound": "mfma", "kernel": "sumfac_units_mf_kernel", "achieved": achieved, "peak": 78.6,
                         "unit": "TFLOP/s", "frac": achieved / 78.6, "flop_units": f_units, "flop_fold": f_fold,
                         "fold_tflops": f_fold / (tm["fold_ms"] * 1e-3) / 1e12,
