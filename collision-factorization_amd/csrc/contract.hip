@@ -61,7 +61,6 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                                         const double *__restrict__ R, const float *__restrict__ R32, int64_t B,
                                         int N_Q, int accumulate, int row, int64_t tile) {
     constexpr int K2 = K * K, K3 = K2 * K;
-    const gptr<const T> fT = (gptr<const T>)fT_in;
     const gptr<double> QT = (gptr<double>)QT_in;
     const cptr<Slice> slices_c = (cptr<Slice>)slices;
     const cptr<int32_t> row_ptr_c = (cptr<int32_t>)row_ptr;

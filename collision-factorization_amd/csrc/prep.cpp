@@ -336,117 +336,66 @@ SumfacGrids sumfac_grids(double alpha, const int s[7]) {
     return g;
 }
 
-namespace {
-// Y_q(theta = acos(mu), phi = 0) for all q, as SphericalHarmonics.evaluate (src/SphericalHarmonics.m:15-50)
-void sh_phi0(int L, double mu, const std::vector<double> &norm, double *Y) {
-    double x = std::cos(std::acos(mu));
-    for (int l = 0; l <= L; ++l)
-        for (int m = -l; m <= l; ++m) {
-            int q = l * l + l + m;
-            if (m < 0)
-                Y[q] = 0.0;
-            else
-                Y[q] = norm[q] * legendre_plm(l, m, x);
-        }
-}
-}  // namespace
-
-SumfacTables sumfac_tables(const RealGaunt &g, const SumfacGrids &gr) {
+SumfacTablePlan sumfac_table_plan(const RealGaunt &g) {
     const int L = g.L, nq = (L + 1) * (L + 1), n_l = g.n_l;
-    const int Nu = (int)gr.u1.size(), Nt = (int)gr.t1.size(), Ny = (int)gr.y2.size();
-    std::vector<double> norm(nq);
-    sh_norm(L, norm.data());
-    // SH at phi=0 on the patch-1 (u,t) grid and the patch-2 y grid
-    std::vector<double> Y1((size_t)Nu * Nt * nq), Y2((size_t)Ny * nq);
-    for (int u = 0; u < Nu; ++u)
-        for (int t = 0; t < Nt; ++t) {
-            double y = gr.u1[u] * gr.t1[t];
-            double mu = std::max(std::min(1.0 - 2.0 * y * y, 1.0), -1.0);
-            sh_phi0(L, mu, norm, &Y1[((size_t)u * Nt + t) * nq]);
-        }
-    for (int i = 0; i < Ny; ++i) {
-        double y = gr.y2[i];
-        double mu = std::max(std::min(1.0 - 2.0 * y * y, 1.0), -1.0);
-        sh_phi0(L, mu, norm, &Y2[(size_t)i * nq]);
-    }
-    SumfacTables T;
+    SumfacTablePlan P;
+    SumfacTables &T = P.T;
     T.n_l = n_l;
     T.N_Q = nq;
     T.L = L;
     T.trip.assign((size_t)n_l * 3, 0);
     T.qi_off.assign(n_l + 1, 0);
+    P.scale.assign(n_l, 0.0);
+    P.loss_off.assign(n_l + 1, 0);
+    P.ent_off.push_back(0);
     std::vector<std::vector<size_t>> members(n_l);
     for (size_t z = 0; z < g.val.size(); ++z) members[g.t[z] - 1].push_back(z);
-    // per t: slot list and its (u,t) / y profiles, collected first, then laid out slot-fastest
-    std::vector<std::vector<double>> g1s, g2s;   // per slot
-    std::vector<std::vector<double>> l1s(n_l), l2s(n_l);
     for (int t = 0; t < n_l; ++t) {
         const auto &mz = members[t];
-        size_t z0 = mz.front();
-        int li = l_of_q0(g.q1[z0] - 1), lj = l_of_q0(g.q2[z0] - 1), lk = l_of_q0(g.q3[z0] - 1);
+        const size_t z0 = mz.front();
+        const int li = l_of_q0(g.q1[z0] - 1), lj = l_of_q0(g.q2[z0] - 1), lk = l_of_q0(g.q3[z0] - 1);
         T.trip[3 * t] = li;
         T.trip[3 * t + 1] = lj;
         T.trip[3 * t + 2] = lk;
-        int qi0 = li * li + li, qj0 = lj * lj + lj;  // 0-based m=0 indices
-        double Yi0 = std::sqrt((2.0 * li + 1) / (4.0 * M_PI)), Yj0 = std::sqrt((2.0 * lj + 1) / (4.0 * M_PI));
+        const int qi0 = li * li + li, qj0 = lj * lj + lj;  // 0-based m=0 indices
+        const double Yi0 = std::sqrt((2.0 * li + 1) / (4.0 * M_PI)), Yj0 = std::sqrt((2.0 * lj + 1) / (4.0 * M_PI));
         double D = 0.0;
         for (size_t z : mz) D += g.val[z] * g.val[z];
-        double SCALE = (8.0 * M_PI * M_PI * Yj0) / D;
+        P.scale[t] = (8.0 * M_PI * M_PI * Yj0) / D;
+        // slots in order of first appearance of q_i among the entries with q_j = q_j0; each slot's
+        // entries in member order (the order the reference accumulates them)
         std::vector<int> qlist;
-        std::vector<std::vector<double>> g1, g2;
-        std::vector<double> lp1((size_t)Nu * Nt, 0.0), lp2((size_t)Ny, 0.0);
+        std::vector<std::vector<size_t>> slot_z;
         for (size_t z : mz) {
-            int qi = g.q1[z] - 1, qj = g.q2[z] - 1, qk = g.q3[z] - 1;
+            const int qi = g.q1[z] - 1, qj = g.q2[z] - 1;
             if (qj != qj0) continue;
-            double G = g.val[z];
             int slot = -1;
             for (size_t s = 0; s < qlist.size(); ++s)
                 if (qlist[s] == qi) slot = (int)s;
             if (slot < 0) {
                 slot = (int)qlist.size();
                 qlist.push_back(qi);
-                g1.emplace_back((size_t)Nu * Nt, 0.0);
-                g2.emplace_back((size_t)Ny, 0.0);
+                slot_z.emplace_back();
             }
-            for (int u = 0; u < Nu; ++u)
-                for (int tt = 0; tt < Nt; ++tt) {
-                    double yk = Y1[((size_t)u * Nt + tt) * nq + qk];
-                    g1[slot][(size_t)u * Nt + tt] += G * yk;
-                    if (qi == qi0) lp1[(size_t)u * Nt + tt] += G * Yi0 * yk;
-                }
-            for (int i = 0; i < Ny; ++i) {
-                double yk = Y2[(size_t)i * nq + qk];
-                g2[slot][i] += G * yk;
-                if (qi == qi0) lp2[i] += G * Yi0 * yk;
+            slot_z[slot].push_back(z);
+            if (qi == qi0) {
+                P.loss_qk.push_back(g.q3[z] - 1);
+                P.loss_g.push_back(g.val[z] * Yi0);
             }
         }
-        for (double &v : lp1) v *= SCALE;
-        for (double &v : lp2) v *= SCALE;
-        l1s[t] = std::move(lp1);
-        l2s[t] = std::move(lp2);
+        P.loss_off[t + 1] = (int)P.loss_qk.size();
         for (size_t s = 0; s < qlist.size(); ++s) {
             T.qi.push_back(qlist[s]);
-            for (double &v : g1[s]) v *= SCALE;
-            for (double &v : g2[s]) v *= SCALE;
-            g1s.push_back(std::move(g1[s]));
-            g2s.push_back(std::move(g2[s]));
+            P.slot_t.push_back(t);
+            for (size_t z : slot_z[s]) {
+                P.ent_qk.push_back(g.q3[z] - 1);
+                P.ent_g.push_back(g.val[z]);
+            }
+            P.ent_off.push_back((int)P.ent_qk.size());
         }
         T.qi_off[t + 1] = (int)T.qi.size();
     }
-    const size_t ns = T.qi.size();
-    T.P_gain_p1.resize((size_t)Nu * Nt * ns);
-    T.P_gain_p2.resize((size_t)Ny * ns);
-    T.P_loss_p1.resize((size_t)Nu * Nt * n_l);
-    T.P_loss_p2.resize((size_t)Ny * n_l);
-    for (size_t s = 0; s < ns; ++s) {
-        for (size_t i = 0; i < (size_t)Nu * Nt; ++i) T.P_gain_p1[i * ns + s] = g1s[s][i];
-        for (size_t i = 0; i < (size_t)Ny; ++i) T.P_gain_p2[i * ns + s] = g2s[s][i];
-    }
-    for (int t = 0; t < n_l; ++t) {
-        for (size_t i = 0; i < (size_t)Nu * Nt; ++i) T.P_loss_p1[i * n_l + t] = l1s[t][i];
-        for (size_t i = 0; i < (size_t)Ny; ++i) T.P_loss_p2[i * n_l + t] = l2s[t][i];
-    }
-    return T;
+    return P;
 }
 
 SumfacTables compact_mex_tables(int N_Q, int n_l, const double *L_triplets, const double *qi_valid_mat, int Nu,

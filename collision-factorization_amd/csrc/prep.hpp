@@ -61,8 +61,23 @@ struct SumfacTables {
     std::vector<double> P_gain_p1, P_loss_p1, P_gain_p2, P_loss_p2;
 };
 
-// src/GeneralCollisionTensor.m:106-174, from the find-order COO.
-SumfacTables sumfac_tables(const RealGaunt &g, const SumfacGrids &gr);
+// Recipe for building the compact P tables on the device (src/GeneralCollisionTensor.m:106-174):
+// the slot structure of SumfacTables plus, per slot, its Gaunt entries (q_k, G) in member order and,
+// per t_chan, the loss entries (q_k, G Y_i0) and SCALE.  With Y = real SH at phi = 0 on a grid node,
+//   P_gain[node][slot] = SCALE_t * sum_{e in slot} G_e Y[q_k(e)],
+//   P_loss[node][t]    = SCALE_t * sum_{e in loss(t)} (G Y_i0)_e Y[q_k(e)].
+struct SumfacTablePlan {
+    SumfacTables T;                   // trip, qi_off, qi (P vectors left empty)
+    std::vector<int32_t> slot_t;      // [ns] t_chan of each slot
+    std::vector<int32_t> ent_off;     // [ns+1]
+    std::vector<int32_t> ent_qk;      // 0-based q_k
+    std::vector<double> ent_g;
+    std::vector<int32_t> loss_off;    // [n_l+1]
+    std::vector<int32_t> loss_qk;
+    std::vector<double> loss_g;
+    std::vector<double> scale;        // [n_l]
+};
+SumfacTablePlan sumfac_table_plan(const RealGaunt &g);
 
 // Compact tables from the reference's dense MEX inputs (the cfk_compute_rtensor_sumfac path).
 SumfacTables compact_mex_tables(int N_Q, int n_l, const double *L_triplets, const double *qi_valid_mat,

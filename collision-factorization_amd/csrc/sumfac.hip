@@ -1002,6 +1002,64 @@ __global__ __launch_bounds__(64) void sumfac_fold_mf_kernel(int NL, int L, const
     }
 }
 
+// Compact P tables on the device (src/GeneralCollisionTensor.m:106-174), from a SumfacTablePlan.
+// One workgroup per quadrature node: the patch-1 (u,t) nodes first ([N_u1][N_t1] order), then the
+// patch-2 y nodes.  The real SH at phi = 0 of the node's polar angle (mu = 1 - 2 y^2, clamped) go to
+// LDS (sin terms are 0 there), then one thread per slot sums its Gaunt entries in member order, as
+// the MATLAB loop accumulates them, and applies the t_chan's SCALE; likewise the loss rows.
+__global__ __launch_bounds__(256) void sumfac_ptable_kernel(
+    int L, int n_l, int ns, int Nu, int Nt, int Ny, const double *__restrict__ u1, const double *__restrict__ t1,
+    const double *__restrict__ y2, const double *__restrict__ shnorm, const int *__restrict__ slot_t,
+    const int *__restrict__ ent_off, const int *__restrict__ ent_qk, const double *__restrict__ ent_g,
+    const int *__restrict__ loss_off, const int *__restrict__ loss_qk, const double *__restrict__ loss_g,
+    const double *__restrict__ scale, double *__restrict__ Pg1, double *__restrict__ Pl1, double *__restrict__ Pg2,
+    double *__restrict__ Pl2) {
+    __shared__ double sY[(MAXL + 1) * (MAXL + 1)];
+    const int node = blockIdx.x;
+    const bool p1 = node < Nu * Nt;
+    const double y = p1 ? u1[node / Nt] * t1[node % Nt] : y2[node - Nu * Nt];
+    const double mu = fmax(fmin(1.0 - 2.0 * y * y, 1.0), -1.0);
+    const double x = cos(acos(mu));
+    for (int m = threadIdx.x; m <= L; m += blockDim.x) {
+        double pmm = 1.0;
+        if (m > 0) {
+            const double sq = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
+            double fact = 1.0;
+            for (int i = 1; i <= m; ++i) {
+                pmm *= -fact * sq;
+                fact += 2.0;
+            }
+        }
+        double p0 = pmm, pl = pmm;
+        double p1v = x * (2 * m + 1) * pmm;
+        for (int l = m; l <= L; ++l) {
+            if (l == m + 1) {
+                pl = p1v;
+            } else if (l > m + 1) {
+                pl = (x * (2 * l - 1) * p1v - (l + m - 1) * p0) / (l - m);
+                p0 = p1v;
+                p1v = pl;
+            }
+            const int q = l * l + l;
+            sY[q + m] = shnorm[q + m] * pl;
+            if (m > 0) sY[q - m] = 0.0;
+        }
+    }
+    __syncthreads();
+    double *Pg = p1 ? Pg1 + (size_t)node * ns : Pg2 + (size_t)(node - Nu * Nt) * ns;
+    for (int sl = threadIdx.x; sl < ns; sl += blockDim.x) {
+        double acc = 0.0;
+        for (int e = ent_off[sl]; e < ent_off[sl + 1]; ++e) acc += ent_g[e] * sY[ent_qk[e]];
+        Pg[sl] = acc * scale[slot_t[sl]];
+    }
+    double *Pl = p1 ? Pl1 + (size_t)node * n_l : Pl2 + (size_t)(node - Nu * Nt) * n_l;
+    for (int t = threadIdx.x; t < n_l; t += blockDim.x) {
+        double acc = 0.0;
+        for (int e = loss_off[t]; e < loss_off[t + 1]; ++e) acc += loss_g[e] * sY[loss_qk[e]];
+        Pl[t] = acc * scale[t];
+    }
+}
+
 __global__ void conservation_zero_kernel(int NK, int NL, const int *__restrict__ trip, double *__restrict__ R) {
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t total = (int64_t)NK * NK * NK * NL;
@@ -1044,8 +1102,9 @@ struct SumfacBuffers {
     DevBuf<int> trip, qi_off, qi;
 };
 
+// T's P vectors are uploaded, or, with plan non-null, the P tables are built on the device.
 int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha, int x_begin, int x_end,
-               bool zero_conservation, double *R_host) {
+               bool zero_conservation, double *R_host, const SumfacTablePlan *plan = nullptr) {
     int rc;
     if ((rc = check_device())) return rc;
     const int L = T.L, NL = T.n_l, NQ = T.N_Q;
@@ -1069,10 +1128,34 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     if ((rc = b.trip.upload(T.trip.data(), T.trip.size()))) return rc;
     if ((rc = b.qi_off.upload(T.qi_off.data(), T.qi_off.size()))) return rc;
     if ((rc = b.qi.upload(T.qi.data(), T.qi.size()))) return rc;
-    if ((rc = b.Pg1.upload(T.P_gain_p1.data(), T.P_gain_p1.size()))) return rc;
-    if ((rc = b.Pl1.upload(T.P_loss_p1.data(), T.P_loss_p1.size()))) return rc;
-    if ((rc = b.Pg2.upload(T.P_gain_p2.data(), T.P_gain_p2.size()))) return rc;
-    if ((rc = b.Pl2.upload(T.P_loss_p2.data(), T.P_loss_p2.size()))) return rc;
+    if (plan) {
+        const int Nu = (int)G.u1.size(), Nt = (int)G.t1.size(), Ny = (int)G.y2.size();
+        const int ns = T.qi_off[NL];
+        DevBuf<int> slot_t, ent_off, ent_qk, loss_off, loss_qk;
+        DevBuf<double> ent_g, loss_g, scale;
+        if ((rc = slot_t.upload(plan->slot_t.data(), plan->slot_t.size())) ||
+            (rc = ent_off.upload(plan->ent_off.data(), plan->ent_off.size())) ||
+            (rc = ent_qk.upload(plan->ent_qk.data(), plan->ent_qk.size())) ||
+            (rc = ent_g.upload(plan->ent_g.data(), plan->ent_g.size())) ||
+            (rc = loss_off.upload(plan->loss_off.data(), plan->loss_off.size())) ||
+            (rc = loss_qk.upload(plan->loss_qk.data(), plan->loss_qk.size())) ||
+            (rc = loss_g.upload(plan->loss_g.data(), plan->loss_g.size())) ||
+            (rc = scale.upload(plan->scale.data(), plan->scale.size())))
+            return rc;
+        if ((rc = b.Pg1.alloc((size_t)Nu * Nt * ns)) || (rc = b.Pl1.alloc((size_t)Nu * Nt * NL)) ||
+            (rc = b.Pg2.alloc((size_t)Ny * ns)) || (rc = b.Pl2.alloc((size_t)Ny * NL)))
+            return rc;
+        hipLaunchKernelGGL(sumfac_ptable_kernel, dim3((unsigned)(Nu * Nt + Ny)), dim3(256), 0, nullptr, L, NL, ns, Nu,
+                           Nt, Ny, b.grid[2].p, b.grid[4].p, b.grid[6].p, b.shnorm.p, slot_t.p, ent_off.p, ent_qk.p,
+                           ent_g.p, loss_off.p, loss_qk.p, loss_g.p, scale.p, b.Pg1.p, b.Pl1.p, b.Pg2.p, b.Pl2.p);
+        CFK_HIP(hipGetLastError());
+        CFK_HIP(hipDeviceSynchronize());  // the plan's buffers are freed on return from this block
+    } else {
+        if ((rc = b.Pg1.upload(T.P_gain_p1.data(), T.P_gain_p1.size()))) return rc;
+        if ((rc = b.Pl1.upload(T.P_loss_p1.data(), T.P_loss_p1.size()))) return rc;
+        if ((rc = b.Pg2.upload(T.P_gain_p2.data(), T.P_gain_p2.size()))) return rc;
+        if ((rc = b.Pl2.upload(T.P_loss_p2.data(), T.P_loss_p2.size()))) return rc;
+    }
     SumfacDev d;
     d.NK = NK;
     d.NL = NL;
@@ -1337,6 +1420,6 @@ extern "C" int cfk_build_rtensor(double *R_out, int32_t K_max, int32_t L, double
     } catch (const std::exception &e) {
         return set_error(CFK_EINVAL, "%s", e.what());
     }
-    SumfacTables T = sumfac_tables(g, G);
-    return run_sumfac(T, G, K_max + 1, alpha, 0, grid_sizes[0], true, R_out);
+    const SumfacTablePlan P = sumfac_table_plan(g);
+    return run_sumfac(P.T, G, K_max + 1, alpha, 0, grid_sizes[0], true, R_out, &P);
 }

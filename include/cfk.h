@@ -129,8 +129,9 @@ int cfk_gauss_gen_laguerre(int32_t n, double alpha, double *x, double *w);
  * (src/GeneralCollisionTensor.m:55-64). */
 int cfk_sumfac_grid_sizes(int32_t K_max, int32_t L, int32_t radial_pad, int32_t angular_pad, int32_t sizes[7]);
 
-/* Build R [N_K,N_K,N_K,N_L]: P tables (src/GeneralCollisionTensor.m:106-174, in C++ on the host,
- * compacted to the valid q_i), the sumfac quadrature on the GPU, and the conservation
+/* Build R [N_K,N_K,N_K,N_L] on the GPU end to end: P tables (src/GeneralCollisionTensor.m:106-174,
+ * compacted to the valid q_i; the slot structure on the host, the tables on the device), the
+ * sumfac quadrature, and the conservation
  * zeroing (src/GeneralCollisionTensor.m:183-191).  grid_sizes as from cfk_sumfac_grid_sizes (or
  * chosen explicitly); labels/vals/ic_map as returned by cfk_real_gaunt. */
 int cfk_build_rtensor(double *R_out, int32_t K_max, int32_t L, double alpha, const int32_t grid_sizes[7],
