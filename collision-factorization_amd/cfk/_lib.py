@@ -58,6 +58,8 @@ SIGNATURES = {
     "cfk_gauss_legendre": ([_i32, ctypes.c_double, ctypes.c_double, _dp, _dp], ctypes.c_int),
     "cfk_gauss_gen_laguerre": ([_i32, ctypes.c_double, _dp, _dp], ctypes.c_int),
     "cfk_sumfac_set_kernel": ([_i32], ctypes.c_int),
+    "cfk_h5_read_f64": ([ctypes.c_char_p, ctypes.c_char_p, _dp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i32),
+                         ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_sumfac_last_timing": ([ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cfk_sumfac_grid_sizes": ([_i32, _i32, _i32, _i32, ctypes.POINTER(_i32)], ctypes.c_int),

@@ -137,6 +137,17 @@ int cfk_sumfac_grid_sizes(int32_t K_max, int32_t L, int32_t radial_pad, int32_t 
 int cfk_build_rtensor(double *R_out, int32_t K_max, int32_t L, double alpha, const int32_t grid_sizes[7],
                       const double *labels, const double *vals, const double *ic_map, int64_t N_G);
 
+/* Read one float64 dataset of an HDF5 file — a MATLAB v7.3 .mat file is one, such as the
+ * reference's precalc tensors (the .mat files in src/precalc, written by precompute_collision_operator.m:61;
+ * the TensorObj properties are datasets under "/#refs#").  `dataset` is a '/'-separated path.
+ * dims[0..rank-1] are the HDF5 dimensions (C order), i.e. a MATLAB array's dimensions reversed, so
+ * `out` holds the array in MATLAB's column-major order.  Two-call pattern: out = NULL returns
+ * numel, rank and dims only.  Supported: the HDF5 subset MATLAB writes (superblock 0/1, symbol-table
+ * groups, v1 object headers, compact/contiguous/chunked layouts, deflate and shuffle filters);
+ * anything else returns CFK_EUNSUP.  Host-only; no reference counterpart (MATLAB's load). */
+int cfk_h5_read_f64(const char *path, const char *dataset, double *out, int64_t capacity, int64_t *numel,
+                    int32_t *rank, int64_t dims[8]);
+
 /* ---------------------------------------------------------------------------------------
  * Device-resident plans: tables uploaded once, many batched executions.
  * ------------------------------------------------------------------------------------- */

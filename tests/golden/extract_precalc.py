@@ -19,10 +19,15 @@ executed.  Output: one ``precalc_k4_l{L}_g{gamma}.npz`` per file with
     R       float64, flat in MATLAB column-major order, length 125*N_L
     meta    float64 [K_max, L_max, D_max, alpha]
 
+Two of the files (L=2 gamma=0: R in one chunk, the Gaunt COO compact; L=4 gamma=1: R and the
+labels in deflate-compressed chunks) are also copied byte for byte next to the npz files, as data for the
+native .mat reader's test (tests/test_matfile.py).
+
 Run this in a container where the reference is mounted at /root/reference; the fixtures
 are committed so nothing at test time reads the reference.
 """
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -66,10 +71,15 @@ def extract(L, gamma, tmpdir):
                                                    os.path.getsize(out)))
 
 
+MAT_COPIES = ["collisiontensor_k4_l2_gamma0.00.mat", "collisiontensor_k4_l4_gamma1.00.mat"]
+
+
 def main():
     with tempfile.TemporaryDirectory() as tmpdir:
         for L, g in FILES:
             extract(L, g, tmpdir)
+    for fn in MAT_COPIES:
+        shutil.copyfile(os.path.join(REF, fn), os.path.join(HERE, fn))
 
 
 if __name__ == "__main__":
