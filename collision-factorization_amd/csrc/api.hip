@@ -287,6 +287,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if (B > max_batch(p->N_Q, p->K))
         return set_error(CFK_EINVAL, "B=%lld exceeds %lld distributions per call (f must stay below 4 GiB)", (long long)B,
                          (long long)max_batch(p->N_Q, p->K));
+    if (p->N_G == 0) return CFK_OK;  // no Gaunt entries: Q(f) = 0
     const int64_t NK = (int64_t)p->N_Q * p->K;
     const bool natural = layout == CFK_LAYOUT_NATURAL;
     const Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
@@ -345,6 +346,7 @@ int oneshot_sparse(int variant, double *Q, const double *f, const double *labels
     int rc;
     if (!Q || !f) return set_error(CFK_EINVAL, "null Q or f");
     if ((rc = validate_coo(labels, vals, ic_map, N_G, R, R_numel, N_Q, K))) return rc;
+    if (N_G == 0) return CFK_OK;  // empty sum: Q unchanged, as the reference's loops leave it
     if ((rc = check_device())) return rc;
     cfk_plan_s plan;
     CFK_HIP(hipGetDevice(&plan.device));

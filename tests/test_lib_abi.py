@@ -49,6 +49,19 @@ def test_host_validation_rejects_bad_indices_without_a_device():
         cfk.angular_first_collision_kernel_mex(np.zeros((nq, K)), f, lab, [0.5], [1.0], np.ones(K ** 3), nq, K)
 
 
+def test_empty_coo_leaves_q_unchanged_without_a_device():
+    """N_G = 0: the reference's loops add nothing, so Q keeps its contents (no device is touched)."""
+    nq, K = 4, 2
+    before = np.arange(nq * K, dtype=np.float64).reshape(nq, K)
+    Q = np.asfortranarray(before)
+    f = np.ones((nq, K), order="F")
+    empty = np.zeros((0, 3))
+    for fn in (cfk.angular_first_collision_kernel_mex, cfk.radial_first_collision_kernel_mex,
+               cfk.naive_collision_kernel_mex):
+        fn(Q, f, empty, np.zeros(0), np.zeros(0), np.zeros(0), nq, K)
+        assert np.array_equal(Q, before)
+
+
 def test_sumfac_arity_error_mirrors_the_mex():
     # compute_rtensor_sumfac_mex.cpp:82-84
     with pytest.raises(cfk.MexError) as e:
