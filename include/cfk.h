@@ -160,7 +160,10 @@ int cfk_plan_create(const double *labels, const double *vals, const double *ic_m
 int cfk_plan_destroy(cfk_plan plan);
 
 /* Q_dev += Q(f_dev) for B distributions on `stream` (hipStream_t, NULL = default).
- * Asynchronous.  variant: CFK_VARIANT_*; precision: CFK_PREC_*; layout: CFK_LAYOUT_*. */
+ * Asynchronous.  variant: CFK_VARIANT_*; precision: CFK_PREC_*; layout: CFK_LAYOUT_*.
+ * B = 0 and a plan with no Gaunt entries leave Q unchanged.  One call takes at most
+ * (2^32 - 1) / (8 N_Q K) distributions (f addressed through 32-bit buffer offsets; 662,800 at L=8,
+ * K=10); larger batches return CFK_EINVAL before any launch and are split by the caller. */
 int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t layout, double *Q_dev,
                      const double *f_dev, int64_t B, void *stream);
 
@@ -210,7 +213,7 @@ int cfk_multi_destroy(cfk_multi multi);
  * K runs its items (one workgroup per item, dispatched in that order), the kernels of different K
  * on parallel graph branches.  The conversions and the kernels are captured into a hipGraph on the
  * first call for a given set of pointers and sizes; later calls with the same set replay the graph
- * on `stream`. */
+ * on `stream`.  Each B[g] obeys cfk_plan_execute's per-call limit. */
 int cfk_multi_execute(cfk_multi multi, double *const *Q_dev, const double *const *f_dev, const int64_t *B,
                       void *stream);
 
