@@ -181,6 +181,7 @@ def main():
         "kernel_ms": {"mixed": kmix * 1e3, "fp64": k64 * 1e3, "fp64_sym": ksym * 1e3},
         "flop_per_eval": F,
         "tiles_kc": tiles,
+        "waves_per_workgroup": {prec: plan.waves(prec) for prec in ("fp64", "mixed", "fp64_sym")},
         "setup_s": {"geometry": t1 - t0, "rtensor_sumfac_gpu": t2 - t1},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp32"], "traffic": MEASURED_TRAFFIC_BYTES if B == 4096 else None,

@@ -66,6 +66,16 @@ class CollisionPlan:
         """Fix one precision's tile (0 = default); for "mixed", pf=1 runs the radial stage on MFMA."""
         check(lib().cfk_plan_set_tile(self._h, PRECISION[precision], int(kc), int(nb), int(pf)))
 
+    def set_waves(self, precision, wpb=0):
+        """Waves per workgroup (1, 2 or 4 batch tiles of one output row per CU; 0 = default)."""
+        check(lib().cfk_plan_set_waves(self._h, PRECISION[precision], int(wpb)))
+
+    def waves(self, precision):
+        """The waves per workgroup the plan uses for `precision`."""
+        w = ctypes.c_int32(0)
+        check(lib().cfk_plan_get_waves(self._h, PRECISION[precision], ctypes.byref(w)))
+        return int(w.value)
+
     def autotune(self, B, stream=None):
         """Pick the fastest tile per precision for batches of B; returns {precision: (KC, NB, PF)}."""
         t = (ctypes.c_int32 * 9)()

@@ -155,8 +155,14 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
     p->sym_row_work = work;
-    // device copy: last field = 1 on the slices whose partner block is l = 0 (kept with l2 <= l3, so l2 == 0)
-    for (Slice &s : sl) s.q1 = (trip[s.t][1] == 0) ? 1 : 0;
+    // device copy: the last field holds the slice flags (partner block l = 0: kept with l2 <= l3, so l2 == 0;
+    // (k2,k3)-symmetric merged block: l2 == l3) and the length of the slice's first q2 group
+    for (Slice &s : sl) {
+        int32_t g1 = 1;
+        while (s.z0 + g1 < s.z1 && zq[s.z0 + g1].x >= 0) ++g1;
+        s.q1 = (trip[s.t][1] == 0 ? SLICE_PRECISE : 0) | (trip[s.t][1] == trip[s.t][2] ? SLICE_SYMMETRIC : 0) |
+               (g1 << SLICE_G1_SHIFT);
+    }
     p->sym_N_G = (int64_t)zq.size();
     p->sym_S = (int64_t)tq.size();
     for (int i = 0; i < table_padding(); ++i) {  // batch-fetch padding (never used as entries)
@@ -292,7 +298,8 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     const bool natural = layout == CFK_LAYOUT_NATURAL;
     const Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
                   p->nb[precision] > 0 ? p->nb[precision] : 1,
-                  p->pf[precision]};
+                  p->pf[precision],
+                  p->wpb[precision] > 0 ? p->wpb[precision] : 1};
     int rc;
     if (precision != CFK_PREC_FP64) {
         if (variant != CFK_VARIANT_ANGULAR_FIRST)
@@ -471,17 +478,19 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
     for (int prec = 0; prec < 3; ++prec) {
         if (prec != CFK_PREC_FP64 && !p->sym_ok) continue;
         double best = 1e300;
-        Tile bt{default_kc(p->K), 1, 0};
+        Tile bt{default_kc(p->K), 1, 0, 1};
         for (int c = 0; c < 2; ++c) {
             if (c == 1 && kcs[1] == kcs[0]) continue;
             for (int nb = 1; nb <= 2; ++nb)
                 for (int pf = 0; pf <= ((prec == CFK_PREC_FP64 || (prec == CFK_PREC_MIXED && mf_supports_k(p->K))) ? 1 : 0);
-                     ++pf) {
+                     ++pf)
+                for (int wpb = 1; wpb <= 4; wpb *= 2) {
                     // mixed: pf = 1 selects the matrix-core radial stage (fixed chunk, one distribution per lane)
-                    if (prec == CFK_PREC_MIXED && pf == 1 && (nb != 1 || c != 1)) continue;
+                    if (prec == CFK_PREC_MIXED && pf == 1 && (nb != 1 || c != 1 || wpb != 1)) continue;
                     p->kc[prec] = kcs[c];
                     p->nb[prec] = nb;
                     p->pf[prec] = pf;
+                    p->wpb[prec] = wpb;
                     p->timing = false;
                     if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p, B,
                                            st)))
@@ -495,17 +504,18 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
                     int64_t nl = 0;
                     if ((rc = cfk_plan_read_timing(p, &ms, &nl))) return rc;
                     if (std::getenv("CFK_AUTOTUNE_VERBOSE"))
-                        fprintf(stderr, "cfk autotune: precision %d KC %d NB %d PF %d: %.4f ms\n", prec, kcs[c], nb, pf,
-                                nl > 0 ? ms / nl : -1.0);
+                        fprintf(stderr, "cfk autotune: precision %d KC %d NB %d PF %d WPB %d: %.4f ms\n", prec, kcs[c], nb,
+                                pf, wpb, nl > 0 ? ms / nl : -1.0);
                     if (nl > 0 && ms / nl < best) {
                         best = ms / nl;
-                        bt = Tile{kcs[c], nb, pf};
+                        bt = Tile{kcs[c], nb, pf, wpb};
                     }
                 }
         }
         p->kc[prec] = bt.kc;
         p->nb[prec] = bt.nb;
         p->pf[prec] = bt.pf;
+        p->wpb[prec] = bt.wpb;
     }
     p->timing = was_timing;
     CFK_HIP(hipStreamSynchronize(st));
@@ -558,6 +568,22 @@ int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, 
     plan->kc[precision] = kc;
     plan->nb[precision] = nb;
     plan->pf[precision] = pf;
+    return CFK_OK;
+}
+
+int cfk_plan_set_waves(cfk_plan plan, int32_t precision, int32_t wpb) {
+    if (!plan) return set_error(CFK_EINVAL, "null plan");
+    if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
+    if (wpb != 0 && wpb != 1 && wpb != 2 && wpb != 4)
+        return set_error(CFK_EINVAL, "waves per workgroup must be 0, 1, 2 or 4 (got %d)", wpb);
+    plan->wpb[precision] = wpb;
+    return CFK_OK;
+}
+
+int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out) {
+    if (!plan || !wpb_out) return set_error(CFK_EINVAL, "null argument");
+    if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
+    *wpb_out = plan->wpb[precision] > 0 ? plan->wpb[precision] : 1;
     return CFK_OK;
 }
 

@@ -60,6 +60,7 @@ struct DevBuf {
 // per lane, one-entry-ahead prefetch of the next nonzero's f rows (plain tables only).
 struct Tile {
     int kc, nb, pf;
+    int wpb = 1;  // waves per workgroup: batch tiles of one output row that share a CU's scalar cache
 };
 
 // One group of the batched uneven-size kernel (mixed precision, symmetrized tables, batch-fastest).
@@ -86,6 +87,11 @@ inline int64_t max_batch(int N_Q, int K) { return (((int64_t)1 << 32) - 1) / (8 
 struct Slice {
     int32_t t, z0, z1, q1;
 };
+// In the symmetrized device tables the q1 field of a Slice (rows are given by row_ptr there) holds
+// flags instead: the partner block is l = 0 (precise: FP64 R.Phi on the mixed path), the block is
+// (k2,k3)-symmetric (l2 == l3, so R'_t[k1,k2,k3] = R'_t[k1,k3,k2]), and, from bit SLICE_G1_SHIFT up,
+// the number of entries of the slice's first q2 group.
+constexpr int32_t SLICE_PRECISE = 1, SLICE_SYMMETRIC = 2, SLICE_G1_SHIFT = 8;
 
 }  // namespace cfk
 
@@ -126,6 +132,7 @@ struct cfk_plan_s {
     int kc[3] = {0, 0, 0};
     int nb[3] = {0, 0, 0};   // distributions per lane
     int pf[3] = {0, 0, 0};   // FP64: prefetch next nonzero (plain tables); mixed: matrix-core radial stage
+    int wpb[3] = {0, 0, 0};  // waves per workgroup of the angular-first kernels (0: default)
     // kernel timing (cfk_plan_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -18,7 +18,9 @@ namespace cfk {
 
 int table_padding() { return ZB; }
 
+#ifndef CFK_K_CASES
 #define CFK_K_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+#endif
 
 // Device pointers of one set of routing tables.
 template <typename T>
@@ -47,12 +49,54 @@ struct SparseTables {
 // PROJ forces the five collision invariants of the contribution to exactly zero.
 //
 // Mixed precision (T = float): Phi is built in FP32; the R.Phi stage runs in FP32 with an FP32
-// per-slice sum (R32, one FP32 row per (k2,k3)) except on slices flagged precise (Slice.q1 != 0
-// in the symmetrized device tables: the blocks whose partner is the l=0 block, which carry the
-// large, cancelling Maxwellian-times-perturbation terms), where it runs in FP64 as for T = double.
+// per-slice sum (R32, one FP32 row per (k2,k3)) except on slices flagged SLICE_PRECISE (the blocks
+// whose partner is the l=0 block, which carry the large, cancelling Maxwellian-times-perturbation
+// terms), where it runs in FP64 as for T = double.  On slices flagged SLICE_SYMMETRIC (l2 == l3, where
+// the merged block is symmetric in (k2,k3)) the FP32 stage folds Phi onto k2 <= k3 and reads that half
+// of R_t only (K(K+1)/2 instead of K^2 rows).
 // ---------------------------------------------------------------------------------------------
 // One work item of the angular-first contraction: output row `row` for batch tile `tile` (64*NB
 // distributions), lane = threadIdx.x % 64 of a one-wave workgroup.
+
+// FP32 stage R_t : Phi of one k3 chunk: a[i][k1] = sum_{k3 in chunk} sum_{k2} R_t[k1,k2,k3] phi[i][k2][k3-c0],
+// one k3-plane of R_t (K*K contiguous FP32 values, rows k2, k1 fastest) per k3, fetched through the
+// scalar path in batches of RB values that are all in flight together (scalar loads return out of
+// order, so each batch costs one wait), each value feeding NB FMAs.  With SYM only the rows k2 <= k3 of
+// each plane are read (the caller has folded Phi onto that triangle).  a is written by its first product.
+template <int K, int KC, int NB, bool SYM, typename PhiT>
+__device__ __forceinline__ void r_phi_f32(cptr<float> Rt32, const PhiT (&phi)[NB][K][KC], float (&a)[NB][K], int c0) {
+    constexpr int K2 = K * K;
+    constexpr int RB = (K * K + 1) / 2 < 50 ? (K * K + 1) / 2 : 50;  // floats per batch
+#pragma unroll
+    for (int j = 0; j < KC; ++j) {
+        if (c0 + j < K) {
+            const cptr<float> plane = Rt32 + (c0 + j) * K2;
+            const int n = SYM ? (j + 1) * K : K2;  // compile-time after unrolling
+#pragma unroll
+            for (int e0 = 0; e0 < K2; e0 += RB) {
+                if (e0 < n) {
+                    float rr[RB];
+#pragma unroll
+                    for (int e = 0; e < RB; ++e)
+                        if (e0 + e < n) rr[e] = plane[e0 + e];
+#pragma unroll
+                    for (int e = 0; e < RB; ++e) {
+                        if (e0 + e < n) {
+                            const int k2 = (e0 + e) / K, k1 = (e0 + e) % K;
+#pragma unroll
+                            for (int i = 0; i < NB; ++i) {
+                                if (j == 0 && k2 == 0)
+                                    a[i][k1] = rr[e] * (float)phi[i][k2][j];
+                                else
+                                    a[i][k1] = fmaf(rr[e], (float)phi[i][k2][j], a[i][k1]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
 
 template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
 __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__restrict__ QT_in,
@@ -91,29 +135,77 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
         const Slice sl = slices_c[s];
         const cptr<double> Rt = R_c + (int64_t)sl.t * K3;
         const cptr<float> Rt32 = R32_c + (int64_t)sl.t * K3;
-        const bool precise = !std::is_same<T, float>::value || sl.q1 != 0;
+        const bool precise = !std::is_same<T, float>::value || (sl.q1 & SLICE_PRECISE) != 0;
+        // (k2,k3)-symmetric block (l2 == l3): only the k2 <= k3 half of R_t is needed (FP32 stage, one chunk)
+        const bool symm = KC == K && GROUPED && !precise && (sl.q1 & SLICE_SYMMETRIC) != 0;
 #pragma unroll 1
         for (int c0 = 0; c0 < K; c0 += KC) {
+            // Phi and the partial sums below are started by their first product (a multiply) rather
+            // than zero-filled: 0 + x*y rounds exactly as x*y, and the fills were 8% of the
+            // kernel's vector instructions.
             T phi[NB][K][KC];
-#pragma unroll
-            for (int i = 0; i < NB; ++i)
-#pragma unroll
-                for (int k = 0; k < K; ++k)
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) phi[i][k][j] = T(0);
             // Table entries are fetched ZB at a time (one scalar round trip per batch; the device arrays
             // carry ZB padding entries so the batch read never leaves them).
             if (GROUPED) {
                 // A q2 group accumulates h = sum g f3 and is closed (phi += f2 (x) h) by the next
-                // group's first entry or the end of the slice; the slice's first entry opens a group
-                // without closing one.
+                // group's first entry or the end of the slice.  In FP32 the slice's first group (its
+                // length is in the slice flags) is peeled off and its close writes phi with a multiply
+                // (in FP64 the peel costs an occupancy step, so phi is zero-filled there instead).
+                constexpr bool PEEL = std::is_same<T, float>::value;
                 T f2[NB][K], h[NB][KC];
+                int zmain = sl.z0;  // first entry of the main loop
+                if constexpr (PEEL) {
+                    const int zg2 = sl.z0 + (sl.q1 >> SLICE_G1_SHIFT);  // first entry of the second group
+                    const int64_t u2 = (int64_t)(~zq_c[sl.z0].x) * B;
 #pragma unroll
-                for (int i = 0; i < NB; ++i)
+                    for (int i = 0; i < NB; ++i) {
 #pragma unroll
-                    for (int j = 0; j < KC; ++j) h[i][j] = T(0);
+                        for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = fload<T>(frs, lo[i], u2 + k2 * kstride);
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) h[i][j] = T(0);
+                    }
+                    for (int zb = sl.z0; zb < zg2; zb += ZB) {
+                        int2 qv[ZB];
+                        T gv[ZB];
+#pragma unroll
+                        for (int e = 0; e < ZB; ++e) {
+                            qv[e] = zq_c[zb + e];
+                            gv[e] = zg_c[zb + e];
+                        }
+#pragma unroll
+                        for (int e = 0; e < ZB; ++e) {
+                            if (zb + e < zg2) {
+                                const int64_t u3 = (int64_t)qv[e].y * B;
+#pragma unroll
+                                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                                    for (int j = 0; j < KC; ++j)
+                                        if (c0 + j < K) h[i][j] = fma(gv[e], fload<T>(frs, lo[i], u3 + (c0 + j) * kstride), h[i][j]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < NB; ++i)
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                            for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = f2[i][k2] * h[i][j];
+                            h[i][j] = T(0);
+                        }
+                    zmain = zg2;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) {
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+#pragma unroll
+                            for (int j = 0; j < KC; ++j) phi[i][k][j] = T(0);
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) h[i][j] = T(0);
+                    }
+                }
                 bool have = false;  // wave-uniform: a group is open
-                for (int zb = sl.z0; zb < sl.z1; zb += ZB) {
+                for (int zb = zmain; zb < sl.z1; zb += ZB) {
                     int2 qv[ZB];
                     T gv[ZB];
 #pragma unroll
@@ -154,13 +246,14 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                         }
                     }
                 }
-                // every slice has at least one entry, so a group is open here
+                if (have) {
 #pragma unroll
-                for (int i = 0; i < NB; ++i)
+                    for (int i = 0; i < NB; ++i)
 #pragma unroll
-                    for (int j = 0; j < KC; ++j)
+                        for (int j = 0; j < KC; ++j)
 #pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
+                            for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
+                }
             } else if (PF) {
                 // one-entry-ahead prefetch of the next nonzero's f rows
                 int2 qn = zq_c[sl.z0];
@@ -198,12 +291,21 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                                 f3n[i][j] = (c0 + j < K) ? fload<T>(frs, lo[i], u3 + (c0 + j) * kstride) : T(0);
                         }
                     }
+                    if (z == sl.z0) {
 #pragma unroll
-                    for (int i = 0; i < NB; ++i)
+                        for (int i = 0; i < NB; ++i)
 #pragma unroll
-                        for (int j = 0; j < KC; ++j)
+                            for (int j = 0; j < KC; ++j)
 #pragma unroll
-                            for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[i][k2], f3[i][j], phi[i][k2][j]);
+                                for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = gf2[i][k2] * f3[i][j];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < NB; ++i)
+#pragma unroll
+                            for (int j = 0; j < KC; ++j)
+#pragma unroll
+                                for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[i][k2], f3[i][j], phi[i][k2][j]);
+                    }
                 }
             } else {
                 for (int zb = sl.z0; zb < sl.z1; zb += ZB) {
@@ -223,12 +325,21 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                                 T gf2[K];
 #pragma unroll
                                 for (int k2 = 0; k2 < K; ++k2) gf2[k2] = gv[e] * fload<T>(frs, lo[i], u2 + k2 * kstride);
+                                if (zb + e == sl.z0) {
 #pragma unroll
-                                for (int j = 0; j < KC; ++j) {
-                                    if (c0 + j < K) {
-                                        const T f3 = fload<T>(frs, lo[i], u3 + (c0 + j) * kstride);
+                                    for (int j = 0; j < KC; ++j) {
+                                        const T f3 = (c0 + j < K) ? fload<T>(frs, lo[i], u3 + (c0 + j) * kstride) : T(0);
 #pragma unroll
-                                        for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[k2], f3, phi[i][k2][j]);
+                                        for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = gf2[k2] * f3;
+                                    }
+                                } else {
+#pragma unroll
+                                    for (int j = 0; j < KC; ++j) {
+                                        if (c0 + j < K) {
+                                            const T f3 = fload<T>(frs, lo[i], u3 + (c0 + j) * kstride);
+#pragma unroll
+                                            for (int k2 = 0; k2 < K; ++k2) phi[i][k2][j] = fma(gf2[k2], f3, phi[i][k2][j]);
+                                        }
                                     }
                                 }
                             }
@@ -263,32 +374,23 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                     }
                 }
             } else {
-                constexpr int RB = (K * K + 1) / 2 < 50 ? (K * K + 1) / 2 : 50;  // floats per batch
+                // FP32 per-slice sum a[k1], started by the (k2,k3) = (0,c0) products.
                 float a[NB][K];
+                if (symm) {
+                    // R_t is symmetric in (k2,k3) here, so R_t : Phi = sum_{k2<=k3} R_t[.,k2,k3] Phis[k2,k3]
+                    // with Phis = Phi + Phi^T off the diagonal: the rows k2 <= k3 of each k3-plane are
+                    // its first (k3+1)K values.
+                    if constexpr (KC == K) {
 #pragma unroll
-                for (int i = 0; i < NB; ++i)
+                        for (int i = 0; i < NB; ++i)
 #pragma unroll
-                    for (int k1 = 0; k1 < K; ++k1) a[i][k1] = 0.0f;
+                            for (int k3 = 1; k3 < K; ++k3)
 #pragma unroll
-                for (int j = 0; j < KC; ++j) {
-                    if (c0 + j < K) {
-                        const cptr<float> plane = Rt32 + (c0 + j) * K2;
-#pragma unroll
-                        for (int e0 = 0; e0 < K2; e0 += RB) {
-                            float rr[RB];
-#pragma unroll
-                            for (int e = 0; e < RB; ++e)
-                                if (e0 + e < K2) rr[e] = plane[e0 + e];
-#pragma unroll
-                            for (int e = 0; e < RB; ++e) {
-                                if (e0 + e < K2) {
-                                    const int k2 = (e0 + e) / K, k1 = (e0 + e) % K;
-#pragma unroll
-                                    for (int i = 0; i < NB; ++i) a[i][k1] = fmaf(rr[e], (float)phi[i][k2][j], a[i][k1]);
-                                }
-                            }
-                        }
+                                for (int k2 = 0; k2 < k3; ++k2) phi[i][k2][k3] += phi[i][k3][k2];
+                        r_phi_f32<K, KC, NB, true>(Rt32, phi, a, 0);
                     }
+                } else {
+                    r_phi_f32<K, KC, NB, false>(Rt32, phi, a, c0);
                 }
 #pragma unroll
                 for (int i = 0; i < NB; ++i)
@@ -310,14 +412,19 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
     }
 }
 
+// A workgroup holds blockDim.x / 64 waves, each one batch tile of the same output row: the waves run
+// the same slice sequence, so R_t and the table entries they fetch through the scalar path are
+// shared in their CU's scalar cache instead of being fetched by waves of unrelated rows.
 template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
-__global__ __launch_bounds__(WAVE) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
+__global__ __launch_bounds__(4 * WAVE) void af_kernel(const T *__restrict__ fT, double *__restrict__ QT,
                                                   const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
                                                   const T *__restrict__ zg, const double *__restrict__ R,
                                                   const float *__restrict__ R32, int64_t B, int N_Q, int accumulate) {
+    const int64_t tile = (int64_t)blockIdx.x * (blockDim.x / WAVE) + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    if (tile * NB * WAVE >= B) return;  // wave-uniform; the kernel has no barrier
     af_item<T, K, KC, GROUPED, PROJ, NB, PF>(fT, QT, slices, row_ptr, zq, zg, R, R32, B, N_Q, accumulate,
-                                             row_order[blockIdx.y], (int64_t)blockIdx.x);
+                                             row_order[blockIdx.y], tile);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -525,11 +632,14 @@ __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int la
 namespace {
 
 template <typename T, int K, int KC, bool GROUPED, int NB, bool PF>
-void launch_af_v(const SparseTables<T> &tb, int N_Q, const T *fT, double *QT, int64_t B, int acc, hipStream_t st) {
-    dim3 grid((unsigned)((B + WAVE * NB - 1) / (WAVE * NB)), (unsigned)N_Q);
+void launch_af_v(const SparseTables<T> &tb, int N_Q, int wpb, const T *fT, double *QT, int64_t B, int acc,
+                 hipStream_t st) {
+    wpb = wpb == 2 || wpb == 4 ? wpb : 1;
+    const int64_t tiles = (B + WAVE * NB - 1) / (WAVE * NB);
+    dim3 grid((unsigned)((tiles + wpb - 1) / wpb), (unsigned)N_Q);
     // double runs the plain operator (no projection), float the mixed path (always projected)
     constexpr bool PROJ = std::is_same<T, float>::value;
-    hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(WAVE), 0, st, fT, QT, tb.slices,
+    hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(WAVE * wpb), 0, st, fT, QT, tb.slices,
                        tb.row_ptr, tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc);
 }
 
@@ -554,9 +664,9 @@ void launch_af_k(const SparseTables<T> &tb, int N_Q, const Tile &tl, const T *fT
 #define CFK_AF_NB(KCV, PFV)                                                             \
     do {                                                                                \
         if (tl.nb >= 2)                                                                 \
-            launch_af_v<T, K, KCV, GROUPED, 2, PFV>(tb, N_Q, fT, QT, B, acc, st);       \
+            launch_af_v<T, K, KCV, GROUPED, 2, PFV>(tb, N_Q, tl.wpb, fT, QT, B, acc, st);       \
         else                                                                            \
-            launch_af_v<T, K, KCV, GROUPED, 1, PFV>(tb, N_Q, fT, QT, B, acc, st);       \
+            launch_af_v<T, K, KCV, GROUPED, 1, PFV>(tb, N_Q, tl.wpb, fT, QT, B, acc, st);       \
     } while (0)
         if constexpr (GROUPED) {
             (void)pf;
@@ -570,7 +680,7 @@ void launch_af_k(const SparseTables<T> &tb, int N_Q, const Tile &tl, const T *fT
         }
 #undef CFK_AF_NB
     } else {
-        launch_af_v<T, K, KCD, GROUPED, 1, false>(tb, N_Q, fT, QT, B, acc, st);
+        launch_af_v<T, K, KCD, GROUPED, 1, false>(tb, N_Q, tl.wpb, fT, QT, B, acc, st);
     }
 }
 

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(WAVE) void af_mf_kernel(const float *__restrict__ f
     for (int s = s0; s < s1; ++s) {
         const Slice sl = slices_c[s];
         const cptr<double> Rt = R_c + (int64_t)sl.t * K3;
-        const bool precise = sl.q1 != 0;
+        const bool precise = (sl.q1 & SLICE_PRECISE) != 0;
         f32x4 acc[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};

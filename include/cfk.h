@@ -171,7 +171,8 @@ int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t 
  * {K, ceil(K/2)} x distributions per lane NB in {1,2} x a third switch PF: for FP64 the prefetch of
  * the next nonzero's rows, for MIXED the radial stage on the matrix cores (v_mfma_f32_16x16x4_f32,
  * KC = ceil(K/2), NB = 1) -- for each precision on a synthetic batch of B distributions
- * (batch-fastest layout) on `stream` and keeps the fastest per precision in the plan.  When
+ * (batch-fastest layout), each also at 1, 2 and 4 waves per workgroup (cfk_plan_set_waves), on
+ * `stream` and keeps the fastest per precision in the plan.  When
  * tiles_out is not NULL it receives 9 ints: (KC, NB, PF) for FP64, MIXED, FP64_SYM in that order.
  * Allocates temporary device buffers; synchronizes `stream`.  The full variant set exists for K in
  * {6,8,10,12,14}; other K run the default tile. */
@@ -187,6 +188,15 @@ int cfk_plan_linearize(cfk_plan plan, const double *f0, double *J, void *stream)
 /* Sets one precision's tile (KC, NB, PF as above; KC = 0 and NB = 0 select the defaults) without
  * timing anything. */
 int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf);
+
+/* Waves per workgroup of one precision's angular-first kernel: wpb in {1, 2, 4} puts that many batch
+ * tiles of the same output row in one workgroup (one CU), where they share the scalar-cache copies
+ * of R_t and of the routing table; 0 selects the default (1).  The autotuner also picks it (per
+ * precision, among 1, 2 and 4); cfk_plan_set_tile leaves it unchanged.  No reference counterpart
+ * (the MEX kernels are single-threaded). */
+int cfk_plan_set_waves(cfk_plan plan, int32_t precision, int32_t wpb);
+/* The waves per workgroup the plan uses for a precision (1, 2 or 4). */
+int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out);
 
 /* Kernel timing: when enabled, every execute brackets its main contraction kernel (not the
  * layout/precision conversions around it) with a pair of HIP events on the execution stream.

@@ -188,9 +188,11 @@ def test_every_mixed_tile_keeps_parity(config1, torch_cuda):
                                       nthreads=8)
     fd = torch.from_numpy(f).cuda()
     fT = fd.reshape(B, -1).t().contiguous()
-    for kc, nb, pf in ((K, 1, 0), ((K + 1) // 2, 1, 0), (K, 2, 0), ((K + 1) // 2, 1, 1)):
+    for kc, nb, pf, wpb in ((K, 1, 0, 1), ((K + 1) // 2, 1, 0, 1), (K, 2, 0, 1), ((K + 1) // 2, 1, 1, 1),
+                            (K, 1, 0, 2), (K, 1, 0, 4), (K, 2, 0, 4)):
         c["plan"].set_tile("mixed", kc, nb, pf)
-        print("mixed tile", kc, nb, pf, flush=True)
+        c["plan"].set_waves("mixed", wpb)
+        print("mixed tile", kc, nb, pf, wpb, flush=True)
         Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
         c["plan"].execute(Qd, fd, B, precision="mixed")
         QT = torch.zeros_like(fT)
@@ -202,3 +204,26 @@ def test_every_mixed_tile_keeps_parity(config1, torch_cuda):
         Qn = Qn.reshape(B, K, nq)
         assert float(np.abs(Qn[:, 0, :4]).max()) == 0.0 and float(np.abs(Qn[:, 1, 0]).max()) == 0.0
     c["plan"].set_tile("mixed")
+    c["plan"].set_waves("mixed")
+
+
+def test_waves_per_workgroup_keep_parity(config1, torch_cuda):
+    """FP64 and symmetrized FP64 at 1, 2 and 4 waves per workgroup on a ragged batch (the last
+    workgroup holds tiles past the batch end) against the oracle."""
+    torch = torch_cuda
+    c = config1
+    nq, K, B = c["nq"], c["K"], 64 * 5 + 17
+    f = perturbed_maxwellians(c["L"], K, B, seed=97)
+    Qref = oracle.angular_first_batch(np.zeros(B * nq * K), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
+                                      nthreads=8)
+    fd = torch.from_numpy(f).cuda()
+    for prec in ("fp64", "fp64_sym"):
+        for wpb in (1, 2, 4):
+            c["plan"].set_waves(prec, wpb)
+            assert c["plan"].waves(prec) == wpb
+            print("waves", prec, wpb, flush=True)
+            Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
+            c["plan"].execute(Qd, fd, B, precision=prec)
+            torch.cuda.synchronize()
+            assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= TOL64, (prec, wpb)
+        c["plan"].set_waves(prec)
