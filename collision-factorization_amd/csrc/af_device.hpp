@@ -55,13 +55,26 @@ __device__ __forceinline__ T fload(frsrc_t rs, uint32_t lo, int64_t u) {
     else
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, lo, (int)(u * 8), 0));
 }
+// Four consecutive FP32 values at lane offset lo + imm and uniform byte offset soff (one 16-byte load).
+__device__ __forceinline__ float4 fload4(frsrc_t rs, uint32_t lo, int soff) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, lo, soff, 0));
+}
 #else
 typedef const void *frsrc_t;
+__device__ inline float4 fload4(frsrc_t rs, uint32_t lo, int soff) {
+    return *(const float4 *)((const char *)rs + lo + soff);
+}
 __device__ inline frsrc_t make_frsrc(const void *p, uint32_t) { return p; }
 template <typename T>
 __device__ inline T fload(frsrc_t rs, uint32_t lo, int64_t u) {
     return *(const T *)((const char *)rs + lo + u * (int64_t)sizeof(T));
 }
 #endif
+
+// Packed FP32 rows: the single-plan mixed kernel with a full k3 chunk reads f as [N_Q][B][KP] with
+// KP = K rounded up to 4 (zero padding), so the K coefficients of one (q, distribution) pair are one
+// contiguous 16-byte-aligned row: three 16-byte loads per row at one scalar offset, instead of K
+// 4-byte loads each with its own.
+constexpr int kp_of(int K) { return (K + 3) / 4 * 4; }
 
 }  // namespace cfk

@@ -55,6 +55,8 @@ int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, doub
                       hipStream_t st);
 int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t cols, hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
+int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st);
+bool af_f32_packed(int K, const Tile &tl);
 
 // 1-based integral index stored as double, checked against [1, hi].
 static inline bool idx_ok(double v, int64_t hi) { return v >= 1.0 && v <= (double)hi && v == std::floor(v); }
@@ -308,10 +310,15 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     }
     if (precision == CFK_PREC_MIXED) {
         const bool mf = tl.pf != 0 && mf_supports_k(p->K);  // matrix-core radial stage
-        if ((rc = p->fT32.ensure((size_t)(NK * B)))) return rc;
+        // the VALU kernel with a full k3 chunk reads packed rows [N_Q][B][KP]; the others [K][N_Q][B]
+        const bool packed = !mf && af_f32_packed(p->K, tl);
+        const int64_t nf = packed ? (int64_t)p->N_Q * ((p->K + 3) / 4 * 4) * B : NK * B;
+        if ((rc = p->fT32.ensure((size_t)nf))) return rc;
         if (natural) {
             if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
-            if ((rc = launch_transpose_f32(f, p->fT32.p, B, NK, st))) return rc;
+            if ((rc = packed ? launch_pack_rows(f, p->fT32.p, p->N_Q, p->K, B, true, st)
+                             : launch_transpose_f32(f, p->fT32.p, B, NK, st)))
+                return rc;
             KernelTimer tm(p, st);
             if ((rc = mf ? launch_af_mf(p, p->fT32.p, p->QT.p, B, 0, st)
                          : launch_af_sym_f32(p, tl, p->fT32.p, p->QT.p, B, 0, st)))
@@ -319,7 +326,9 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
             tm.stop();
             return launch_transpose(p->QT.p, Q, NK, B, true, st);
         }
-        if ((rc = launch_to_f32(f, p->fT32.p, NK * B, st))) return rc;
+        if ((rc = packed ? launch_pack_rows(f, p->fT32.p, p->N_Q, p->K, B, false, st)
+                         : launch_to_f32(f, p->fT32.p, NK * B, st)))
+            return rc;
         KernelTimer tm(p, st);
         if ((rc = mf ? launch_af_mf(p, p->fT32.p, Q, B, 1, st) : launch_af_sym_f32(p, tl, p->fT32.p, Q, B, 1, st))) return rc;
         tm.stop();

@@ -78,8 +78,13 @@ struct MultiGroupDev {
 };
 constexpr int MULTI_MAX_GROUPS = 8;
 // Largest batch per call: the contraction kernels read f through a buffer resource, which spans
-// at most 4 GiB (8-byte coefficients counted).
-inline int64_t max_batch(int N_Q, int K) { return (((int64_t)1 << 32) - 1) / (8 * (int64_t)N_Q * K); }
+// at most 4 GiB (8-byte coefficients counted), and the mixed kernel's packed FP32 rows (K rounded up
+// to 4 per row) take a signed 32-bit row offset.
+inline int64_t max_batch(int N_Q, int K) {
+    const int64_t a = (((int64_t)1 << 32) - 1) / (8 * (int64_t)N_Q * K);
+    const int64_t b = (((int64_t)1 << 31) - 1) / (4 * (int64_t)N_Q * ((K + 3) / 4 * 4));
+    return a < b ? a : b;
+}
 
 // One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.  In the device copy
 // of the symmetrized tables the last field instead flags the precise slices (partner block l = 0)

@@ -98,7 +98,7 @@ __device__ __forceinline__ void r_phi_f32(cptr<float> Rt32, const PhiT (&phi)[NB
     }
 }
 
-template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF>
+template <typename T, int K, int KC, bool GROUPED, bool PROJ, int NB, bool PF, bool PK = false>
 __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__restrict__ QT_in,
                                         const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
                                         const int2 *__restrict__ zq, const T *__restrict__ zg,
@@ -120,12 +120,26 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
     for (int i = 0; i < NB; ++i) {
         const int64_t b = (tile * NB + i) * WAVE + (threadIdx.x % WAVE);
         act[i] = b < B;
-        lo[i] = act[i] ? (uint32_t)b * (uint32_t)sizeof(T) : 0u;
+        lo[i] = act[i] ? (uint32_t)b * (uint32_t)(PK ? kp_of(K) * 4 : sizeof(T)) : 0u;
         lq[i] = act[i] ? (uint32_t)b * 8u : 0u;
     }
     const int s0 = row_ptr_c[row], s1 = row_ptr_c[row + 1];
     const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
-    const frsrc_t frs = make_frsrc(fT_in, (uint32_t)(K * kstride * (int64_t)sizeof(T)));
+    static_assert(!PK || (std::is_same<T, float>::value && KC == K && GROUPED), "packed rows: FP32, full chunk, grouped");
+    constexpr int KP = kp_of(K);
+    const frsrc_t frs = make_frsrc(fT_in, (uint32_t)((PK ? KP : K) * kstride * (int64_t)sizeof(T)));
+    // PK: the K coefficients of row q (batch-fastest packed layout [N_Q][B][KP]) of lane's distribution i
+    auto load_row = [&](int q, int i, T (&dst)[K]) {
+        const int soff = q * (int)B * (KP * 4);
+#pragma unroll
+        for (int c = 0; c < KP / 4; ++c) {
+            const float4 v = fload4(frs, lo[i] + 16u * c, soff);
+            if (4 * c + 0 < K) dst[4 * c + 0] = v.x;
+            if (4 * c + 1 < K) dst[4 * c + 1] = v.y;
+            if (4 * c + 2 < K) dst[4 * c + 2] = v.z;
+            if (4 * c + 3 < K) dst[4 * c + 3] = v.w;
+        }
+    };
     double q[NB][K];
 #pragma unroll
     for (int i = 0; i < NB; ++i)
@@ -159,8 +173,11 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                     const int64_t u2 = (int64_t)(~zq_c[sl.z0].x) * B;
 #pragma unroll
                     for (int i = 0; i < NB; ++i) {
+                        if constexpr (PK)
+                            load_row(~zq_c[sl.z0].x, i, f2[i]);
+                        else
 #pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = fload<T>(frs, lo[i], u2 + k2 * kstride);
+                            for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = fload<T>(frs, lo[i], u2 + k2 * kstride);
 #pragma unroll
                         for (int j = 0; j < KC; ++j) h[i][j] = T(0);
                     }
@@ -177,10 +194,18 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                             if (zb + e < zg2) {
                                 const int64_t u3 = (int64_t)qv[e].y * B;
 #pragma unroll
-                                for (int i = 0; i < NB; ++i)
+                                for (int i = 0; i < NB; ++i) {
+                                    if constexpr (PK) {
+                                        T f3[K];
+                                        load_row(qv[e].y, i, f3);
 #pragma unroll
-                                    for (int j = 0; j < KC; ++j)
-                                        if (c0 + j < K) h[i][j] = fma(gv[e], fload<T>(frs, lo[i], u3 + (c0 + j) * kstride), h[i][j]);
+                                        for (int j = 0; j < KC; ++j) h[i][j] = fma(gv[e], f3[j], h[i][j]);
+                                    } else {
+#pragma unroll
+                                        for (int j = 0; j < KC; ++j)
+                                            if (c0 + j < K) h[i][j] = fma(gv[e], fload<T>(frs, lo[i], u3 + (c0 + j) * kstride), h[i][j]);
+                                    }
+                                }
                             }
                         }
                     }
@@ -233,16 +258,28 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                                 have = true;
                                 const int64_t u2 = (int64_t)(~qq.x) * B;
 #pragma unroll
-                                for (int i = 0; i < NB; ++i)
+                                for (int i = 0; i < NB; ++i) {
+                                    if constexpr (PK)
+                                        load_row(~qq.x, i, f2[i]);
+                                    else
 #pragma unroll
-                                    for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = fload<T>(frs, lo[i], u2 + k2 * kstride);
+                                        for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = fload<T>(frs, lo[i], u2 + k2 * kstride);
+                                }
                             }
                             const int64_t u3 = (int64_t)qq.y * B;
 #pragma unroll
-                            for (int i = 0; i < NB; ++i)
+                            for (int i = 0; i < NB; ++i) {
+                                if constexpr (PK) {
+                                    T f3[K];
+                                    load_row(qq.y, i, f3);
 #pragma unroll
-                                for (int j = 0; j < KC; ++j)
-                                    if (c0 + j < K) h[i][j] = fma(g, fload<T>(frs, lo[i], u3 + (c0 + j) * kstride), h[i][j]);
+                                    for (int j = 0; j < KC; ++j) h[i][j] = fma(g, f3[j], h[i][j]);
+                                } else {
+#pragma unroll
+                                    for (int j = 0; j < KC; ++j)
+                                        if (c0 + j < K) h[i][j] = fma(g, fload<T>(frs, lo[i], u3 + (c0 + j) * kstride), h[i][j]);
+                                }
+                            }
                         }
                     }
                 }
@@ -423,8 +460,9 @@ __global__ __launch_bounds__(4 * WAVE) void af_kernel(const T *__restrict__ fT, 
                                                   const float *__restrict__ R32, int64_t B, int N_Q, int accumulate) {
     const int64_t tile = (int64_t)blockIdx.x * (blockDim.x / WAVE) + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     if (tile * NB * WAVE >= B) return;  // wave-uniform; the kernel has no barrier
-    af_item<T, K, KC, GROUPED, PROJ, NB, PF>(fT, QT, slices, row_ptr, zq, zg, R, R32, B, N_Q, accumulate,
-                                             row_order[blockIdx.y], tile);
+    constexpr bool PK = std::is_same<T, float>::value && KC == K && GROUPED;
+    af_item<T, K, KC, GROUPED, PROJ, NB, PF, PK>(fT, QT, slices, row_ptr, zq, zg, R, R32, B, N_Q, accumulate,
+                                                 row_order[blockIdx.y], tile);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -565,6 +603,34 @@ __global__ __launch_bounds__(256) void transpose_kernel(const double *__restrict
 __global__ __launch_bounds__(256) void to_f32_kernel(const double *__restrict__ in, float *__restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) out[i] = (float)in[i];
+}
+
+// FP64 f -> packed FP32 rows [N_Q][B][KP] (zero padded), one thread per (q, distribution) row.
+// NATURAL: input [B][K][N_Q] (B MATLAB N_Q x K arrays), thread index q fastest (coalesced reads);
+// otherwise batch-fastest [K][N_Q][B], thread index b fastest.
+template <bool NATURAL>
+__global__ __launch_bounds__(256) void pack_rows_kernel(const double *__restrict__ in, float *__restrict__ out,
+                                                        int N_Q, int K, int64_t B) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)N_Q * B) return;
+    int64_t q, b;
+    if (NATURAL) {
+        b = i / N_Q;
+        q = i - b * N_Q;
+    } else {
+        q = i / B;
+        b = i - q * B;
+    }
+    const int KP = kp_of(K);
+    float4 *o = (float4 *)(out + (q * B + b) * KP);
+    for (int c = 0; c < KP / 4; ++c) {
+        float v[4];
+        for (int r = 0; r < 4; ++r) {
+            const int k = 4 * c + r;
+            v[r] = k < K ? (float)(NATURAL ? in[(b * K + k) * N_Q + q] : in[((int64_t)k * N_Q + q) * B + b]) : 0.0f;
+        }
+        o[c] = make_float4(v[0], v[1], v[2], v[3]);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -772,6 +838,25 @@ int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t col
     hipLaunchKernelGGL((transpose_kernel<float, false>), grid, dim3(256), 0, st, in, out, rows, cols);
     CFK_HIP(hipGetLastError());
     return CFK_OK;
+}
+
+int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st) {
+    const int64_t n = (int64_t)N_Q * B;
+    if (n <= 0) return CFK_OK;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (natural)
+        hipLaunchKernelGGL(pack_rows_kernel<true>, grid, dim3(256), 0, st, in, out, N_Q, K, B);
+    else
+        hipLaunchKernelGGL(pack_rows_kernel<false>, grid, dim3(256), 0, st, in, out, N_Q, K, B);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+// Whether the FP32 angular-first launch for this tile reads packed rows (the kernel with a full k3
+// chunk); mirrors the chunk selection of launch_af_k.
+bool af_f32_packed(int K, const Tile &tl) {
+    const int kc = CFK_TUNED_K(K) ? (tl.kc >= K ? K : kc_half(K)) : kc_default(K);
+    return kc == K;
 }
 
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st) {

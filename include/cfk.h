@@ -162,8 +162,9 @@ int cfk_plan_destroy(cfk_plan plan);
 /* Q_dev += Q(f_dev) for B distributions on `stream` (hipStream_t, NULL = default).
  * Asynchronous.  variant: CFK_VARIANT_*; precision: CFK_PREC_*; layout: CFK_LAYOUT_*.
  * B = 0 and a plan with no Gaunt entries leave Q unchanged.  One call takes at most
- * (2^32 - 1) / (8 N_Q K) distributions (f addressed through 32-bit buffer offsets; 662,800 at L=8,
- * K=10); larger batches return CFK_EINVAL before any launch and are split by the caller. */
+ * min((2^32 - 1) / (8 N_Q K), (2^31 - 1) / (4 N_Q KP)) distributions, KP = K rounded up to a
+ * multiple of 4 (f addressed through 32-bit buffer offsets; 552,336 at L=8, K=10); larger batches
+ * return CFK_EINVAL before any launch and are split by the caller. */
 int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t layout, double *Q_dev,
                      const double *f_dev, int64_t B, void *stream);
 

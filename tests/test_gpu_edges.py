@@ -65,10 +65,10 @@ def test_empty_batch_leaves_q_unchanged(setup):
 
 
 def test_batch_above_the_limit_is_rejected_before_launch(setup):
-    """B beyond (2^32-1)/(8 N_Q K) distributions: CFK_EINVAL from the argument checks; the
-    buffers passed are tiny and never touched."""
+    """B beyond min((2^32-1)/(8 N_Q K), (2^31-1)/(4 N_Q KP)) distributions (KP = K rounded up to
+    4): CFK_EINVAL from the argument checks; the buffers passed are tiny and never touched."""
     torch = setup["torch"]
-    limit = (2 ** 32 - 1) // (8 * NQ * K)
+    limit = min((2 ** 32 - 1) // (8 * NQ * K), (2 ** 31 - 1) // (4 * NQ * ((K + 3) // 4 * 4)))
     Qd = torch.zeros(NQ * K, dtype=torch.float64, device="cuda")
     fd = torch.zeros(NQ * K, dtype=torch.float64, device="cuda")
     for prec in (0, 1, 2):
