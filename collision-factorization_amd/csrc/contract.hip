@@ -162,10 +162,11 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
             // carry ZB padding entries so the batch read never leaves them).
             if (GROUPED) {
                 // A q2 group accumulates h = sum g f3 and is closed (phi += f2 (x) h) by the next
-                // group's first entry or the end of the slice.  In FP32 the slice's first group (its
-                // length is in the slice flags) is peeled off and its close writes phi with a multiply
-                // (in FP64 the peel costs an occupancy step, so phi is zero-filled there instead).
-                constexpr bool PEEL = std::is_same<T, float>::value;
+                // group's first entry or the end of the slice.  In FP32 with a full k3 chunk the slice's
+                // first group (its length is in the slice flags) is peeled off and its close writes phi
+                // with a multiply; elsewhere phi is zero-filled (in FP64 the peel costs an occupancy
+                // step, and with half chunks at K = 14 it made the compiler serialize the f loads).
+                constexpr bool PEEL = std::is_same<T, float>::value && KC == K;
                 T f2[NB][K], h[NB][KC];
                 int zmain = sl.z0;  // first entry of the main loop
                 if constexpr (PEEL) {
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(WAVE) void multi_af_kernel(const MultiGroupDev *__r
     if (it >= i1) return;
     const int4 w = items[it];
     const MultiGroupDev &G = groups[w.x];
-    af_item<float, K, KC, true, true, 1, false>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.R32, G.B, G.N_Q, 1,
+    af_item<float, K, KC, true, true, 1, false, KC == K>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.R32, G.B, G.N_Q, 1,
                                                 w.y, (int64_t)w.z);
 }
 

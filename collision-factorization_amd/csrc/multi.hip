@@ -14,6 +14,7 @@
 namespace cfk {
 int launch_multi(const MultiGroupDev *groups, const int4 *items, int K, int KC, int i0, int i1, hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
+int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st);
 bool multi_supports_k(int K);
 int default_kc(int K);
 }  // namespace cfk
@@ -41,6 +42,7 @@ struct cfk_multi_s {
     };
     std::vector<Launch> launches;  // one item kernel per K-class
     std::vector<std::unique_ptr<DevBuf<float>>> f32;
+    std::vector<char> packed;  // per group: its item kernel (full k3 chunk) reads packed FP32 rows
     int n_items = 0;
     ~cfk_multi_s() {
         reset();
@@ -110,10 +112,14 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
     int rc;
     if ((rc = m->items.upload(items.data(), items.size()))) return rc;
     m->f32.clear();
+    m->packed.assign(G, 0);
     std::vector<MultiGroupDev> gd(G);
     for (int g = 0; g < G; ++g) {
         cfk_plan_s *p = m->plans[g];
-        const int64_t n = (int64_t)p->N_Q * p->K * B[g];
+        // (from the group's own chunk, as its class was formed: cls has been re-sorted since)
+        const int kc = p->kc[CFK_PREC_MIXED] > 0 ? p->kc[CFK_PREC_MIXED] : default_kc(p->K);
+        m->packed[g] = B[g] > 0 && kc >= p->K;
+        const int64_t n = (int64_t)p->N_Q * (m->packed[g] ? (p->K + 3) / 4 * 4 : p->K) * B[g];
         m->f32.emplace_back(new DevBuf<float>());
         if ((rc = m->f32.back()->alloc((size_t)std::max<int64_t>(n, 1)))) return rc;
         gd[g] = MultiGroupDev{m->f32.back()->p, Q[g], p->sym_slices.p, p->sym_row_ptr.p, p->sym_zq.p,
@@ -140,9 +146,12 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
 static int multi_enqueue(cfk_multi_s *m, hipStream_t st) {
     const int G = (int)m->plans.size();
     int rc = CFK_OK;
-    for (int g = 0; g < G && rc == CFK_OK; ++g)
+    for (int g = 0; g < G && rc == CFK_OK; ++g) {
+        const cfk_plan_s *p = m->plans[g];
         if (m->key_B[g] > 0)
-            rc = launch_to_f32(m->key_f[g], m->f32[g]->p, (int64_t)m->plans[g]->N_Q * m->plans[g]->K * m->key_B[g], st);
+            rc = m->packed[g] ? launch_pack_rows(m->key_f[g], m->f32[g]->p, p->N_Q, p->K, m->key_B[g], false, st)
+                              : launch_to_f32(m->key_f[g], m->f32[g]->p, (int64_t)p->N_Q * p->K * m->key_B[g], st);
+    }
     const int nc = (int)m->launches.size();
     if (rc || nc == 0) return rc;
     CFK_HIP(hipEventRecord(m->ev[MULTI_MAX_GROUPS], st));
