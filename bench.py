@@ -182,6 +182,7 @@ def main():
         "flop_per_eval": F,
         "tiles_kc": tiles,
         "waves_per_workgroup": {prec: plan.waves(prec) for prec in ("fp64", "mixed", "fp64_sym")},
+        "row_split": {prec: plan.split(prec) for prec in ("fp64", "mixed", "fp64_sym")},
         "setup_s": {"geometry": t1 - t0, "rtensor_sumfac_gpu": t2 - t1},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp32"], "traffic": MEASURED_TRAFFIC_BYTES if B == 4096 else None,

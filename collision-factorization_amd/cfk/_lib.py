@@ -72,6 +72,8 @@ SIGNATURES = {
     "cfk_plan_set_tile": ([_vp, _i32, _i32, _i32, _i32], ctypes.c_int),
     "cfk_plan_set_waves": ([_vp, _i32, _i32], ctypes.c_int),
     "cfk_plan_get_waves": ([_vp, _i32, ctypes.POINTER(_i32)], ctypes.c_int),
+    "cfk_plan_set_split": ([_vp, _i32, _i32], ctypes.c_int),
+    "cfk_plan_get_split": ([_vp, _i32, ctypes.POINTER(_i32)], ctypes.c_int),
     "cfk_plan_linearize": ([_vp, _dp, _dp, _vp], ctypes.c_int),
     "cfk_plan_set_timing": ([_vp, _i32], ctypes.c_int),
     "cfk_plan_read_timing": ([_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)], ctypes.c_int),

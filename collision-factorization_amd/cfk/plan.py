@@ -76,6 +76,16 @@ class CollisionPlan:
         check(lib().cfk_plan_get_waves(self._h, PRECISION[precision], ctypes.byref(w)))
         return int(w.value)
 
+    def set_split(self, precision, split=0):
+        """Row split between two waves per batch tile (symmetrized precisions; 0 = off)."""
+        check(lib().cfk_plan_set_split(self._h, PRECISION[precision], int(split)))
+
+    def split(self, precision):
+        """Whether the plan splits rows between two waves for `precision`."""
+        v = ctypes.c_int32(0)
+        check(lib().cfk_plan_get_split(self._h, PRECISION[precision], ctypes.byref(v)))
+        return int(v.value)
+
     def autotune(self, B, stream=None):
         """Pick the fastest tile per precision for batches of B; returns {precision: (KC, NB, PF)}."""
         t = (ctypes.c_int32 * 9)()

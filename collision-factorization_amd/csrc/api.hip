@@ -174,6 +174,21 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
     int rc;
     if ((rc = p->sym_slices.upload(sl.data(), sl.size()))) return rc;
     if ((rc = p->sym_row_ptr.upload(row_ptr.data(), row_ptr.size()))) return rc;
+    {
+        // split point of each row's slices for row-split launches: the halves' cost-model work balanced
+        std::vector<int32_t> mid(N_Q);
+        for (int r = 0; r < N_Q; ++r) {
+            double tot = 0.0, acc = 0.0;
+            for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) tot += (double)(sl[k].z1 - sl[k].z0) * K * K + (double)K3;
+            int32_t m = row_ptr[r];
+            while (m < row_ptr[r + 1] && acc + 0.5 * ((double)(sl[m].z1 - sl[m].z0) * K * K + (double)K3) < 0.5 * tot) {
+                acc += (double)(sl[m].z1 - sl[m].z0) * K * K + (double)K3;
+                ++m;
+            }
+            mid[r] = m;
+        }
+        if ((rc = p->sym_row_mid.upload(mid.data(), mid.size()))) return rc;
+    }
     if ((rc = p->sym_row_order.upload(order.data(), order.size()))) return rc;
     if ((rc = p->sym_zq.upload(zq.data(), zq.size()))) return rc;
     if ((rc = p->sym_zg.upload(zg.data(), zg.size()))) return rc;
@@ -301,7 +316,8 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     const Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
                   p->nb[precision] > 0 ? p->nb[precision] : 1,
                   p->pf[precision],
-                  p->wpb[precision] > 0 ? p->wpb[precision] : 1};
+                  p->wpb[precision] > 0 ? p->wpb[precision] : 1,
+                  p->split[precision]};
     int rc;
     if (precision != CFK_PREC_FP64) {
         if (variant != CFK_VARIANT_ANGULAR_FIRST)
@@ -487,19 +503,23 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
     for (int prec = 0; prec < 3; ++prec) {
         if (prec != CFK_PREC_FP64 && !p->sym_ok) continue;
         double best = 1e300;
-        Tile bt{default_kc(p->K), 1, 0, 1};
+        Tile bt{default_kc(p->K), 1, 0, 1, 0};
         for (int c = 0; c < 2; ++c) {
             if (c == 1 && kcs[1] == kcs[0]) continue;
             for (int nb = 1; nb <= 2; ++nb)
                 for (int pf = 0; pf <= ((prec == CFK_PREC_FP64 || (prec == CFK_PREC_MIXED && mf_supports_k(p->K))) ? 1 : 0);
                      ++pf)
-                for (int wpb = 1; wpb <= 4; wpb *= 2) {
+                for (int wpb = 1; wpb <= 8; wpb *= 2) {
+                    // wpb = 8 stands for the row split (symmetrized tables: two waves per batch tile)
+                    const int split = wpb == 8 ? 1 : 0;
+                    if (split && (prec == CFK_PREC_FP64 || pf != 0)) continue;
                     // mixed: pf = 1 selects the matrix-core radial stage (fixed chunk, one distribution per lane)
                     if (prec == CFK_PREC_MIXED && pf == 1 && (nb != 1 || c != 1 || wpb != 1)) continue;
                     p->kc[prec] = kcs[c];
                     p->nb[prec] = nb;
                     p->pf[prec] = pf;
-                    p->wpb[prec] = wpb;
+                    p->wpb[prec] = split ? 1 : wpb;
+                    p->split[prec] = split;
                     p->timing = false;
                     if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p, B,
                                            st)))
@@ -513,11 +533,11 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
                     int64_t nl = 0;
                     if ((rc = cfk_plan_read_timing(p, &ms, &nl))) return rc;
                     if (std::getenv("CFK_AUTOTUNE_VERBOSE"))
-                        fprintf(stderr, "cfk autotune: precision %d KC %d NB %d PF %d WPB %d: %.4f ms\n", prec, kcs[c], nb,
-                                pf, wpb, nl > 0 ? ms / nl : -1.0);
+                        fprintf(stderr, "cfk autotune: precision %d KC %d NB %d PF %d WPB %d SPLIT %d: %.4f ms\n", prec,
+                                kcs[c], nb, pf, split ? 1 : wpb, split, nl > 0 ? ms / nl : -1.0);
                     if (nl > 0 && ms / nl < best) {
                         best = ms / nl;
-                        bt = Tile{kcs[c], nb, pf, wpb};
+                        bt = Tile{kcs[c], nb, pf, split ? 1 : wpb, split};
                     }
                 }
         }
@@ -525,6 +545,7 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
         p->nb[prec] = bt.nb;
         p->pf[prec] = bt.pf;
         p->wpb[prec] = bt.wpb;
+        p->split[prec] = bt.split;
     }
     p->timing = was_timing;
     CFK_HIP(hipStreamSynchronize(st));
@@ -586,6 +607,21 @@ int cfk_plan_set_waves(cfk_plan plan, int32_t precision, int32_t wpb) {
     if (wpb != 0 && wpb != 1 && wpb != 2 && wpb != 4)
         return set_error(CFK_EINVAL, "waves per workgroup must be 0, 1, 2 or 4 (got %d)", wpb);
     plan->wpb[precision] = wpb;
+    return CFK_OK;
+}
+
+int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split) {
+    if (!plan) return set_error(CFK_EINVAL, "null plan");
+    if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
+    if (split != 0 && split != 1) return set_error(CFK_EINVAL, "split must be 0 or 1 (got %d)", split);
+    plan->split[precision] = split;
+    return CFK_OK;
+}
+
+int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out) {
+    if (!plan || !split_out) return set_error(CFK_EINVAL, "null argument");
+    if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
+    *split_out = plan->split[precision];
     return CFK_OK;
 }
 

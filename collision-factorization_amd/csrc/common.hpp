@@ -61,6 +61,7 @@ struct DevBuf {
 struct Tile {
     int kc, nb, pf;
     int wpb = 1;  // waves per workgroup: batch tiles of one output row that share a CU's scalar cache
+    int split = 0;  // symmetrized tables: each row's slices split between two waves (one tile per workgroup)
 };
 
 // One group of the batched uneven-size kernel (mixed precision, symmetrized tables, batch-fastest).
@@ -124,7 +125,7 @@ struct cfk_plan_s {
     std::string sym_why;
     int64_t sym_N_G = 0, sym_S = 0;
     cfk::DevBuf<cfk::Slice> sym_slices;
-    cfk::DevBuf<int32_t> sym_row_ptr, sym_row_order;
+    cfk::DevBuf<int32_t> sym_row_ptr, sym_row_order, sym_row_mid;
     cfk::DevBuf<int2> sym_zq;
     cfk::DevBuf<double> sym_zg, sym_R;
     cfk::DevBuf<float> sym_zg32, sym_R32;
@@ -138,6 +139,7 @@ struct cfk_plan_s {
     int nb[3] = {0, 0, 0};   // distributions per lane
     int pf[3] = {0, 0, 0};   // FP64: prefetch next nonzero (plain tables); mixed: matrix-core radial stage
     int wpb[3] = {0, 0, 0};  // waves per workgroup of the angular-first kernels (0: default)
+    int split[3] = {0, 0, 0};  // row split between two waves (symmetrized tables)
     // kernel timing (cfk_plan_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

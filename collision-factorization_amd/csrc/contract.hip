@@ -32,6 +32,7 @@ struct SparseTables {
     const T *zg;
     const double *R;
     const float *R32;  // FP32 copy of R (mixed path); may be null for double tables
+    const int32_t *row_mid = nullptr;  // split point of each row's slices (row-split launches only)
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -103,7 +104,8 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                                         const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
                                         const int2 *__restrict__ zq, const T *__restrict__ zg,
                                         const double *__restrict__ R, const float *__restrict__ R32, int64_t B,
-                                        int N_Q, int accumulate, int row, int64_t tile) {
+                                        int N_Q, int accumulate, int row, int64_t tile,
+                                        const int32_t *__restrict__ row_mid = nullptr, int half = -1) {
     constexpr int K2 = K * K, K3 = K2 * K;
     const gptr<double> QT = (gptr<double>)QT_in;
     const cptr<Slice> slices_c = (cptr<Slice>)slices;
@@ -123,7 +125,14 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
         lo[i] = act[i] ? (uint32_t)b * (uint32_t)(PK ? kp_of(K) * 4 : sizeof(T)) : 0u;
         lq[i] = act[i] ? (uint32_t)b * 8u : 0u;
     }
-    const int s0 = row_ptr_c[row], s1 = row_ptr_c[row + 1];
+    int s0 = row_ptr_c[row], s1 = row_ptr_c[row + 1];
+    if (half >= 0) {  // row split between the two waves of a workgroup: half 0 [s0, mid), half 1 [mid, s1)
+        const int mid = ((cptr<int32_t>)row_mid)[row];
+        if (half == 0)
+            s1 = mid;
+        else
+            s0 = mid;
+    }
     const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
     static_assert(!PK || (std::is_same<T, float>::value && KC == K && GROUPED), "packed rows: FP32, full chunk, grouped");
     constexpr int KP = kp_of(K);
@@ -437,6 +446,23 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
             }
         }
     }
+    if (half >= 0) {
+        // half 1 hands its partial sums to half 0 through LDS; half 0 adds them (q0 + q1, a fixed order)
+        extern __shared__ double part[];  // [NB][K][64]
+        const int lane = threadIdx.x % WAVE;
+        if (half == 1) {
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+#pragma unroll
+                for (int k1 = 0; k1 < K; ++k1) part[(i * K + k1) * WAVE + lane] = q[i][k1];
+        }
+        __syncthreads();
+        if (half == 1) return;
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+#pragma unroll
+            for (int k1 = 0; k1 < K; ++k1) q[i][k1] += part[(i * K + k1) * WAVE + lane];
+    }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         if (!act[i]) continue;
@@ -458,12 +484,23 @@ __global__ __launch_bounds__(4 * WAVE) void af_kernel(const T *__restrict__ fT, 
                                                   const Slice *__restrict__ slices, const int32_t *__restrict__ row_ptr,
                                                   const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
                                                   const T *__restrict__ zg, const double *__restrict__ R,
-                                                  const float *__restrict__ R32, int64_t B, int N_Q, int accumulate) {
-    const int64_t tile = (int64_t)blockIdx.x * (blockDim.x / WAVE) + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    if (tile * NB * WAVE >= B) return;  // wave-uniform; the kernel has no barrier
+                                                  const float *__restrict__ R32, int64_t B, int N_Q, int accumulate,
+                                                  const int32_t *__restrict__ row_mid) {
     constexpr bool PK = std::is_same<T, float>::value && KC == K && GROUPED;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    int64_t tile;
+    int half = -1;
+    if (row_mid) {
+        // row-split launch: one batch tile per workgroup of two waves, each wave half of the row's slices
+        // (every wave of the workgroup reaches the barrier in af_item: the grid holds no tile past B)
+        tile = blockIdx.x;
+        half = w & 1;
+    } else {
+        tile = (int64_t)blockIdx.x * (blockDim.x / WAVE) + w;
+        if (tile * NB * WAVE >= B) return;  // wave-uniform; no barrier without a row split
+    }
     af_item<T, K, KC, GROUPED, PROJ, NB, PF, PK>(fT, QT, slices, row_ptr, zq, zg, R, R32, B, N_Q, accumulate,
-                                                 row_order[blockIdx.y], tile);
+                                                 row_order[blockIdx.y], tile, row_mid, half);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -699,15 +736,22 @@ __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int la
 namespace {
 
 template <typename T, int K, int KC, bool GROUPED, int NB, bool PF>
-void launch_af_v(const SparseTables<T> &tb, int N_Q, int wpb, const T *fT, double *QT, int64_t B, int acc,
-                 hipStream_t st) {
+void launch_af_v(const SparseTables<T> &tb, int N_Q, int wpb, bool split, const T *fT, double *QT, int64_t B,
+                 int acc, hipStream_t st) {
     wpb = wpb == 2 || wpb == 4 ? wpb : 1;
     const int64_t tiles = (B + WAVE * NB - 1) / (WAVE * NB);
-    dim3 grid((unsigned)((tiles + wpb - 1) / wpb), (unsigned)N_Q);
     // double runs the plain operator (no projection), float the mixed path (always projected)
     constexpr bool PROJ = std::is_same<T, float>::value;
+    if (split && tb.row_mid) {
+        const dim3 grid((unsigned)tiles, (unsigned)N_Q);
+        hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(2 * WAVE),
+                           (size_t)NB * K * WAVE * sizeof(double), st, fT, QT, tb.slices, tb.row_ptr, tb.row_order,
+                           tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, tb.row_mid);
+        return;
+    }
+    const dim3 grid((unsigned)((tiles + wpb - 1) / wpb), (unsigned)N_Q);
     hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(WAVE * wpb), 0, st, fT, QT, tb.slices,
-                       tb.row_ptr, tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc);
+                       tb.row_ptr, tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, (const int32_t *)nullptr);
 }
 
 // k3 chunk candidates: the full width and half of it.
@@ -731,9 +775,9 @@ void launch_af_k(const SparseTables<T> &tb, int N_Q, const Tile &tl, const T *fT
 #define CFK_AF_NB(KCV, PFV)                                                             \
     do {                                                                                \
         if (tl.nb >= 2)                                                                 \
-            launch_af_v<T, K, KCV, GROUPED, 2, PFV>(tb, N_Q, tl.wpb, fT, QT, B, acc, st);       \
+            launch_af_v<T, K, KCV, GROUPED, 2, PFV>(tb, N_Q, tl.wpb, tl.split, fT, QT, B, acc, st);       \
         else                                                                            \
-            launch_af_v<T, K, KCV, GROUPED, 1, PFV>(tb, N_Q, tl.wpb, fT, QT, B, acc, st);       \
+            launch_af_v<T, K, KCV, GROUPED, 1, PFV>(tb, N_Q, tl.wpb, tl.split, fT, QT, B, acc, st);       \
     } while (0)
         if constexpr (GROUPED) {
             (void)pf;
@@ -747,7 +791,7 @@ void launch_af_k(const SparseTables<T> &tb, int N_Q, const Tile &tl, const T *fT
         }
 #undef CFK_AF_NB
     } else {
-        launch_af_v<T, K, KCD, GROUPED, 1, false>(tb, N_Q, tl.wpb, fT, QT, B, acc, st);
+        launch_af_v<T, K, KCD, GROUPED, 1, false>(tb, N_Q, tl.wpb, tl.split, fT, QT, B, acc, st);
     }
 }
 
@@ -809,14 +853,14 @@ int launch_af_sym(const cfk_plan_s *p, const SparseTables<T> &tb, const Tile &tl
 int launch_af_sym_f64(const cfk_plan_s *p, const Tile &tl, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st) {
     const SparseTables<double> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg.p,
-                                  p->sym_R.p, nullptr};
+                                  p->sym_R.p, nullptr, p->sym_row_mid.p};
     return launch_af_sym<double>(p, tb, tl, fT, QT, B, accumulate, st);
 }
 
 int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st) {
     const SparseTables<float> tb{p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg32.p,
-                                 p->sym_R.p, p->sym_R32.p};
+                                 p->sym_R.p, p->sym_R32.p, p->sym_row_mid.p};
     return launch_af_sym<float>(p, tb, tl, fT, QT, B, accumulate, st);
 }
 

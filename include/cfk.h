@@ -172,7 +172,8 @@ int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t 
  * {K, ceil(K/2)} x distributions per lane NB in {1,2} x a third switch PF: for FP64 the prefetch of
  * the next nonzero's rows, for MIXED the radial stage on the matrix cores (v_mfma_f32_16x16x4_f32,
  * KC = ceil(K/2), NB = 1) -- for each precision on a synthetic batch of B distributions
- * (batch-fastest layout), each also at 1, 2 and 4 waves per workgroup (cfk_plan_set_waves), on
+ * (batch-fastest layout), each also at 1, 2 and 4 waves per workgroup (cfk_plan_set_waves) and, for
+ * the symmetrized precisions, with the row split (cfk_plan_set_split), on
  * `stream` and keeps the fastest per precision in the plan.  When
  * tiles_out is not NULL it receives 9 ints: (KC, NB, PF) for FP64, MIXED, FP64_SYM in that order.
  * Allocates temporary device buffers; synchronizes `stream`.  The full variant set exists for K in
@@ -198,6 +199,15 @@ int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, 
 int cfk_plan_set_waves(cfk_plan plan, int32_t precision, int32_t wpb);
 /* The waves per workgroup the plan uses for a precision (1, 2 or 4). */
 int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out);
+
+/* Row split for the symmetrized precisions (CFK_PREC_FP64_SYM, CFK_PREC_MIXED): with split = 1 each
+ * batch tile runs as a workgroup of two waves that take the first and second half (by the cost model)
+ * of every output row's slices; the second hands its partial sums to the first through LDS, which adds
+ * them in that fixed order, so results stay deterministic.  Halves the work item for a better-balanced
+ * end of the launch; the waves-per-workgroup setting is then unused.  Ignored for CFK_PREC_FP64.  The
+ * autotuner also tries it.  No reference counterpart. */
+int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split);
+int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out);
 
 /* Kernel timing: when enabled, every execute brackets its main contraction kernel (not the
  * layout/precision conversions around it) with a pair of HIP events on the execution stream.

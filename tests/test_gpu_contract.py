@@ -227,3 +227,36 @@ def test_waves_per_workgroup_keep_parity(config1, torch_cuda):
             torch.cuda.synchronize()
             assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= TOL64, (prec, wpb)
         c["plan"].set_waves(prec)
+
+
+def test_row_split_keeps_parity_and_is_deterministic(config1, torch_cuda):
+    """Rows split between the two waves of a workgroup (symmetrized FP64 and mixed) on a ragged batch:
+    against the oracle, bit-identical on a second run, and accumulating into a nonzero Q."""
+    torch = torch_cuda
+    c = config1
+    nq, K, B = c["nq"], c["K"], 64 * 3 + 5
+    f = perturbed_maxwellians(c["L"], K, B, seed=96)
+    Qref = oracle.angular_first_batch(np.zeros(B * nq * K), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
+                                      nthreads=8)
+    fd = torch.from_numpy(f).cuda()
+    fT = fd.reshape(B, -1).t().contiguous()
+    for prec, tol in (("fp64_sym", TOL64), ("mixed", TOL_MIXED)):
+        c["plan"].set_split(prec, 1)
+        assert c["plan"].split(prec) == 1
+        print("row split", prec, flush=True)
+        Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
+        c["plan"].execute(Qd, fd, B, precision=prec)
+        Q1 = torch.zeros_like(fT)
+        c["plan"].execute(Q1, fT, B, precision=prec, layout="batch_fastest")
+        Q2 = torch.full_like(fT, 0.25)
+        c["plan"].execute(Q2, fT, B, precision=prec, layout="batch_fastest")
+        torch.cuda.synchronize()
+        assert rel_err(Qd.cpu().numpy(), Qref, nq, K) <= tol, prec
+        Qn = Q1.t().contiguous().cpu().numpy().ravel()
+        assert rel_err(Qn, Qref, nq, K) <= tol, prec
+        Q1b = torch.zeros_like(fT)
+        c["plan"].execute(Q1b, fT, B, precision=prec, layout="batch_fastest")
+        torch.cuda.synchronize()
+        assert bool((Q1b == Q1).all()), prec
+        assert bool(((Q2 - 0.25) - Q1).abs().max() <= 1e-12 * Q1.abs().max()), prec
+        c["plan"].set_split(prec, 0)
