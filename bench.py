@@ -23,7 +23,7 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp32_vector_unpacked": 78.6e12}
 # HBM bytes per launch of the mixed kernel at this workload, from two rocprofv3 PMC passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration (factors 2.0 and 1.0); the commands and
 # raw figures are in profiles/af_kernel_mixed_hs_l8_k10_b4096.txt.  Re-measured when the kernel changes.
-MEASURED_TRAFFIC_BYTES = 84327 * 1024
+MEASURED_TRAFFIC_BYTES = 83691 * 1024
 
 
 def work_model(lab, ic, K):
