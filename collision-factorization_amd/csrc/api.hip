@@ -509,16 +509,18 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
             for (int nb = 1; nb <= 2; ++nb)
                 for (int pf = 0; pf <= ((prec == CFK_PREC_FP64 || (prec == CFK_PREC_MIXED && mf_supports_k(p->K))) ? 1 : 0);
                      ++pf)
-                for (int wpb = 1; wpb <= 8; wpb *= 2) {
-                    // wpb = 8 stands for the row split (symmetrized tables: two waves per batch tile)
-                    const int split = wpb == 8 ? 1 : 0;
+                for (int v = 0; v < 5; ++v) {
+                    // v = 0..2: 1, 2, 4 waves per workgroup; v = 3, 4: row split (symmetrized tables, two
+                    // waves per batch tile) with 1 or 2 tiles per workgroup
+                    const int split = v >= 3 ? 1 : 0;
+                    const int wpb = split ? v - 2 : 1 << v;
                     if (split && (prec == CFK_PREC_FP64 || pf != 0)) continue;
                     // mixed: pf = 1 selects the matrix-core radial stage (fixed chunk, one distribution per lane)
                     if (prec == CFK_PREC_MIXED && pf == 1 && (nb != 1 || c != 1 || wpb != 1)) continue;
                     p->kc[prec] = kcs[c];
                     p->nb[prec] = nb;
                     p->pf[prec] = pf;
-                    p->wpb[prec] = split ? 1 : wpb;
+                    p->wpb[prec] = wpb;
                     p->split[prec] = split;
                     p->timing = false;
                     if ((rc = plan_execute(p, CFK_VARIANT_ANGULAR_FIRST, prec, CFK_LAYOUT_BATCH_FASTEST, Q.p, f.p, B,
@@ -534,10 +536,10 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
                     if ((rc = cfk_plan_read_timing(p, &ms, &nl))) return rc;
                     if (std::getenv("CFK_AUTOTUNE_VERBOSE"))
                         fprintf(stderr, "cfk autotune: precision %d KC %d NB %d PF %d WPB %d SPLIT %d: %.4f ms\n", prec,
-                                kcs[c], nb, pf, split ? 1 : wpb, split, nl > 0 ? ms / nl : -1.0);
+                                kcs[c], nb, pf, wpb, split, nl > 0 ? ms / nl : -1.0);
                     if (nl > 0 && ms / nl < best) {
                         best = ms / nl;
-                        bt = Tile{kcs[c], nb, pf, split ? 1 : wpb, split};
+                        bt = Tile{kcs[c], nb, pf, wpb, split};
                     }
                 }
         }

@@ -448,7 +448,8 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
     }
     if (half >= 0) {
         // half 1 hands its partial sums to half 0 through LDS; half 0 adds them (q0 + q1, a fixed order)
-        extern __shared__ double part[];  // [NB][K][64]
+        extern __shared__ double part_lds[];  // per tile of the workgroup: [NB][K][64]
+        double *const part = part_lds + (threadIdx.x / (2 * WAVE)) * (NB * K * WAVE);
         const int lane = threadIdx.x % WAVE;
         if (half == 1) {
 #pragma unroll
@@ -491,9 +492,10 @@ __global__ __launch_bounds__(4 * WAVE) void af_kernel(const T *__restrict__ fT, 
     int64_t tile;
     int half = -1;
     if (row_mid) {
-        // row-split launch: one batch tile per workgroup of two waves, each wave half of the row's slices
-        // (every wave of the workgroup reaches the barrier in af_item: the grid holds no tile past B)
-        tile = blockIdx.x;
+        // row-split launch: blockDim.x / 128 batch tiles per workgroup, two waves per tile, each wave half
+        // of the row's slices.  Every wave reaches the barrier in af_item (a tile past the batch end runs
+        // with all lanes inactive).
+        tile = (int64_t)blockIdx.x * (blockDim.x / (2 * WAVE)) + (w >> 1);
         half = w & 1;
     } else {
         tile = (int64_t)blockIdx.x * (blockDim.x / WAVE) + w;
@@ -743,10 +745,11 @@ void launch_af_v(const SparseTables<T> &tb, int N_Q, int wpb, bool split, const 
     // double runs the plain operator (no projection), float the mixed path (always projected)
     constexpr bool PROJ = std::is_same<T, float>::value;
     if (split && tb.row_mid) {
-        const dim3 grid((unsigned)tiles, (unsigned)N_Q);
-        hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(2 * WAVE),
-                           (size_t)NB * K * WAVE * sizeof(double), st, fT, QT, tb.slices, tb.row_ptr, tb.row_order,
-                           tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, tb.row_mid);
+        const int tpw = wpb >= 2 ? 2 : 1;  // batch tiles per workgroup (two waves each, at most 4 waves)
+        const dim3 grid((unsigned)((tiles + tpw - 1) / tpw), (unsigned)N_Q);
+        hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(2 * WAVE * tpw),
+                           (size_t)tpw * NB * K * WAVE * sizeof(double), st, fT, QT, tb.slices, tb.row_ptr,
+                           tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, tb.row_mid);
         return;
     }
     const dim3 grid((unsigned)((tiles + wpb - 1) / wpb), (unsigned)N_Q);

@@ -204,8 +204,8 @@ int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out);
  * batch tile runs as a workgroup of two waves that take the first and second half (by the cost model)
  * of every output row's slices; the second hands its partial sums to the first through LDS, which adds
  * them in that fixed order, so results stay deterministic.  Halves the work item for a better-balanced
- * end of the launch; the waves-per-workgroup setting is then unused.  Ignored for CFK_PREC_FP64.  The
- * autotuner also tries it.  No reference counterpart. */
+ * end of the launch; a waves-per-workgroup setting of 2 or more then puts two batch tiles (four waves)
+ * in a workgroup.  Ignored for CFK_PREC_FP64.  The autotuner also tries it.  No reference counterpart. */
 int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split);
 int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out);
 

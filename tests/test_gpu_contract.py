@@ -234,16 +234,18 @@ def test_row_split_keeps_parity_and_is_deterministic(config1, torch_cuda):
     against the oracle, bit-identical on a second run, and accumulating into a nonzero Q."""
     torch = torch_cuda
     c = config1
-    nq, K, B = c["nq"], c["K"], 64 * 3 + 5
+    nq, K, B = c["nq"], c["K"], 64 * 2 + 5   # 3 batch tiles: with 2 tiles per workgroup one is past the end
     f = perturbed_maxwellians(c["L"], K, B, seed=96)
     Qref = oracle.angular_first_batch(np.zeros(B * nq * K), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
                                       nthreads=8)
     fd = torch.from_numpy(f).cuda()
     fT = fd.reshape(B, -1).t().contiguous()
-    for prec, tol in (("fp64_sym", TOL64), ("mixed", TOL_MIXED)):
+    for prec, tol, wpb in (("fp64_sym", TOL64, 1), ("mixed", TOL_MIXED, 1), ("fp64_sym", TOL64, 2),
+                           ("mixed", TOL_MIXED, 2)):
         c["plan"].set_split(prec, 1)
+        c["plan"].set_waves(prec, wpb)   # 2: two batch tiles per workgroup
         assert c["plan"].split(prec) == 1
-        print("row split", prec, flush=True)
+        print("row split", prec, wpb, flush=True)
         Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
         c["plan"].execute(Qd, fd, B, precision=prec)
         Q1 = torch.zeros_like(fT)
@@ -260,3 +262,4 @@ def test_row_split_keeps_parity_and_is_deterministic(config1, torch_cuda):
         assert bool((Q1b == Q1).all()), prec
         assert bool(((Q2 - 0.25) - Q1).abs().max() <= 1e-12 * Q1.abs().max()), prec
         c["plan"].set_split(prec, 0)
+        c["plan"].set_waves(prec)
