@@ -58,6 +58,8 @@ SIGNATURES = {
     "cfk_gauss_legendre": ([_i32, ctypes.c_double, ctypes.c_double, _dp, _dp], ctypes.c_int),
     "cfk_gauss_gen_laguerre": ([_i32, ctypes.c_double, _dp, _dp], ctypes.c_int),
     "cfk_sumfac_set_kernel": ([_i32], ctypes.c_int),
+    "cfk_sumfac_set_chunk_nodes": ([_i32], ctypes.c_int),
+    "cfk_sumfac_last_chunks": ([ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_h5_read_f64": ([ctypes.c_char_p, ctypes.c_char_p, _dp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i32),
                          ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_sumfac_last_timing": ([ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_double),
@@ -67,6 +69,7 @@ SIGNATURES = {
                           ctypes.c_int),
     "cfk_plan_create": ([_dp, _dp, _dp, _i64, _dp, _i64, _i32, _i32, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_plan_destroy": ([_vp], ctypes.c_int),
+    "cfk_plan_get_stats": ([_vp, ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_plan_execute": ([_vp, _i32, _i32, _i32, _vp, _vp, _i64, _vp], ctypes.c_int),
     "cfk_plan_autotune": ([_vp, _i64, _vp, ctypes.POINTER(_i32)], ctypes.c_int),
     "cfk_plan_set_tile": ([_vp, _i32, _i32, _i32, _i32], ctypes.c_int),

@@ -66,6 +66,18 @@ def set_sumfac_kernel(kind):
     check(lib().cfk_sumfac_set_kernel(SUMFAC_KERNELS[kind]))
 
 
+def set_sumfac_chunk_nodes(max_nodes=0):
+    """Cap the x nodes per sumfac chunk (process-wide; 0 = sized by the unit buffers' memory cap)."""
+    check(lib().cfk_sumfac_set_chunk_nodes(int(max_nodes)))
+
+
+def sumfac_last_chunks():
+    """Number of x-node chunks the last sumfac call ran."""
+    n = _i64(0)
+    check(lib().cfk_sumfac_last_chunks(ctypes.byref(n)))
+    return n.value
+
+
 def sumfac_last_timing():
     """Device time of the last sumfac call: {"kernel", "units", "units_ms", "fold_ms"}."""
     k, n, a, b = _i32(0), _i64(0), ctypes.c_double(0.0), ctypes.c_double(0.0)

@@ -2,7 +2,7 @@
 
 A ``CollisionPlan`` uploads the Gaunt routing table and the radial tensor once; ``execute``
 accumulates ``Q += Q(f)`` for a batch of distributions already resident in device memory
-(any object exposing ``data_ptr()``, e.g. torch tensors on ``cuda:0``; the library only sees raw
+(contiguous float64 torch tensors on the GPU, checked before each call; the library only sees raw
 device pointers).
 """
 import ctypes
@@ -14,6 +14,20 @@ from ._lib import LAYOUT, PRECISION, VARIANT, CfkError, _vp, check, dptr, f64, l
 
 def _ptr(t):
     return ctypes.c_void_p(int(t.data_ptr()))
+
+
+def _check_dev_f64(t, need, name):
+    """A device buffer the kernels read or write as `need` contiguous float64 values: a torch tensor
+    must be float64, on the GPU and contiguous (the library sees only its data pointer and would
+    otherwise read or write past its end)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor on the GPU" % name)
+    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("%s must be a contiguous float64 tensor on the GPU (got %s, %s, contiguous=%s)"
+                         % (name, t.dtype, t.device, t.is_contiguous()))
+    if t.numel() < need:
+        raise ValueError("%s too small: %d values, need %d" % (name, t.numel(), need))
 
 
 def _stream_ptr(stream):
@@ -46,8 +60,8 @@ class CollisionPlan:
         n = self.N_Q * self.K
         if B is None:
             B = Q.numel() // n
-        if Q.numel() < B * n or f.numel() < B * n:
-            raise ValueError("Q/f too small for B=%d distributions of %d coefficients" % (B, n))
+        _check_dev_f64(Q, B * n, "Q")
+        _check_dev_f64(f, B * n, "f")
         check(lib().cfk_plan_execute(self._h, VARIANT[variant], PRECISION[precision], LAYOUT[layout], _ptr(Q),
                                      _ptr(f), int(B), _stream_ptr(stream)))
         return Q
@@ -92,6 +106,14 @@ class CollisionPlan:
         check(lib().cfk_plan_autotune(self._h, int(B), _stream_ptr(stream), t))
         return {"fp64": tuple(t[0:3]), "mixed": tuple(t[3:6]), "fp64_sym": tuple(t[6:9])}
 
+    def stats(self):
+        """Routing-table counts: N_G, slices, and for the symmetrized tables nonzeros, slices, q2 groups,
+        symmetric (l2 == l3) slices, precise (l2 == 0) slices; N_L (cfk_plan_get_stats)."""
+        v = (ctypes.c_int64 * 8)()
+        check(lib().cfk_plan_get_stats(self._h, v))
+        keys = ("N_G", "S", "sym_N_G", "sym_S", "sym_groups", "sym_symmetric", "sym_precise", "N_L")
+        return dict(zip(keys, (int(x) for x in v)))
+
     def set_timing(self, enable=True):
         """Bracket each execute's main kernel with HIP events (see cfk_plan_set_timing)."""
         check(lib().cfk_plan_set_timing(self._h, 1 if enable else 0))
@@ -116,11 +138,15 @@ class CollisionPlan:
 
 
 def dense_execute(Q, f, C, N, B=1, stream=None):
+    _check_dev_f64(Q, int(B) * int(N), "Q")
+    _check_dev_f64(f, int(B) * int(N), "f")
+    _check_dev_f64(C, int(N) ** 3, "C")
     check(lib().cfk_dense_execute(_ptr(Q), _ptr(f), _ptr(C), int(N), int(B), _stream_ptr(stream)))
     return Q
 
 
 def conservation_project(Q, B, N_Q, K, layout="natural", stream=None):
+    _check_dev_f64(Q, int(B) * int(N_Q) * int(K), "Q")
     check(lib().cfk_conservation_project(_ptr(Q), int(B), int(N_Q), int(K), LAYOUT[layout], _stream_ptr(stream)))
     return Q
 
@@ -146,20 +172,14 @@ class MultiPlan:
         n = len(self.plans)
         if len(Qs) != n or len(fs) != n or len(Bs) != n:
             raise ValueError("expected %d groups" % n)
-        for p, q, f, b in zip(self.plans, Qs, fs, Bs):
+        for i, (p, q, f, b) in enumerate(zip(self.plans, Qs, fs, Bs)):
             need = p.N_Q * p.K * int(b)
-            for t in (q, f):
-                if t.dtype != torch_float64() or not t.is_cuda or not t.is_contiguous() or t.numel() < need:
-                    raise ValueError("group buffers must be contiguous float64 device tensors of >= %d values" % need)
+            _check_dev_f64(q, need, "Q[%d]" % i)
+            _check_dev_f64(f, need, "f[%d]" % i)
         Qa = (_vp * n)(*[int(q.data_ptr()) for q in Qs])
         fa = (_vp * n)(*[int(f.data_ptr()) for f in fs])
         Ba = (ctypes.c_int64 * n)(*[int(b) for b in Bs])
         check(lib().cfk_multi_execute(self._h, Qa, fa, Ba, _stream_ptr(stream)))
-
-
-def torch_float64():
-    import torch
-    return torch.float64
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -171,3 +191,4 @@ def torch_float64():
             self.close()
         except Exception:
             pass
+

@@ -167,6 +167,12 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
     }
     p->sym_N_G = (int64_t)zq.size();
     p->sym_S = (int64_t)tq.size();
+    p->sym_groups = p->sym_symmetric = p->sym_precise = 0;
+    for (const int2 &e : zq) p->sym_groups += e.x < 0;
+    for (const Slice &s : sl) {
+        p->sym_symmetric += (s.q1 & SLICE_SYMMETRIC) != 0;
+        p->sym_precise += (s.q1 & SLICE_PRECISE) != 0;
+    }
     for (int i = 0; i < table_padding(); ++i) {  // batch-fetch padding (never used as entries)
         zq.push_back(make_int2(0, 0));
         zg.push_back(0.0);
@@ -631,6 +637,19 @@ int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out) {
     if (!plan || !wpb_out) return set_error(CFK_EINVAL, "null argument");
     if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
     *wpb_out = plan->wpb[precision] > 0 ? plan->wpb[precision] : 1;
+    return CFK_OK;
+}
+
+int cfk_plan_get_stats(cfk_plan plan, int64_t stats[8]) {
+    if (!plan || !stats) return set_error(CFK_EINVAL, "null plan or stats");
+    stats[0] = plan->N_G;
+    stats[1] = plan->S;
+    stats[2] = plan->sym_ok ? plan->sym_N_G : 0;
+    stats[3] = plan->sym_ok ? plan->sym_S : 0;
+    stats[4] = plan->sym_ok ? plan->sym_groups : 0;
+    stats[5] = plan->sym_ok ? plan->sym_symmetric : 0;
+    stats[6] = plan->sym_ok ? plan->sym_precise : 0;
+    stats[7] = plan->n_l;
     return CFK_OK;
 }
 

@@ -124,6 +124,7 @@ struct cfk_plan_s {
     bool sym_ok = false;
     std::string sym_why;
     int64_t sym_N_G = 0, sym_S = 0;
+    int64_t sym_groups = 0, sym_symmetric = 0, sym_precise = 0;  // q2 groups; slices with l2 == l3; with l2 == 0
     cfk::DevBuf<cfk::Slice> sym_slices;
     cfk::DevBuf<int32_t> sym_row_ptr, sym_row_order, sym_row_mid;
     cfk::DevBuf<int2> sym_zq;

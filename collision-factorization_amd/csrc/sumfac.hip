@@ -1088,11 +1088,13 @@ int launch_units_mf(int mt, size_t dyn, const SumfacDev &d, int xc, int nx, int6
 }
 
 int g_sumfac_kernel = CFK_SUMFAC_MATRIX_CORE;
+int g_sumfac_chunk_nodes = 0;  // x nodes per chunk (0: sized by the buffer cap)
 
 // Device time of the last quadrature call (HIP events around each launch on its stream).
 struct SumfacTiming {
     int kind = 0;
     int64_t units = 0;
+    int64_t chunks = 0;  // x-node chunks (unit-buffer launches) of the call
     double units_ms = 0.0, fold_ms = 0.0;
 };
 SumfacTiming g_last_timing;
@@ -1235,6 +1237,7 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     const int64_t per_unit = (int64_t)NL * 2 * NK + 2 * (L + 1) * NK;
     const int64_t cap = mf ? ((int64_t)1 << 29) : ((int64_t)1 << 27);
     int64_t xs_per_chunk = std::max<int64_t>(1, cap / (per_unit * Ux));
+    if (g_sumfac_chunk_nodes > 0) xs_per_chunk = std::min<int64_t>(xs_per_chunk, g_sumfac_chunk_nodes);
     if (x_end > x_begin) {
         const int64_t max_units = std::min<int64_t>(xs_per_chunk, x_end - x_begin) * Ux;
         if ((rc = b.net.alloc((size_t)(max_units * NL * 2 * NK)))) return rc;
@@ -1257,6 +1260,7 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     for (int xc = x_begin; xc < x_end; xc += (int)xs_per_chunk) {
         const int nx = (int)std::min<int64_t>(xs_per_chunk, x_end - xc);
         const int64_t n_units = (int64_t)nx * Ux;
+        g_last_timing.chunks += 1;
         if (n_units > 0x7fffffffLL) return set_error(CFK_EINVAL, "sumfac: too many units per chunk");
         if (mf) {
             switch (NK) {
@@ -1375,6 +1379,18 @@ extern "C" int cfk_sumfac_set_kernel(int32_t kind) {
     if (kind != CFK_SUMFAC_MATRIX_CORE && kind != CFK_SUMFAC_PER_CHANNEL)
         return set_error(CFK_EINVAL, "unknown sumfac kernel %d", kind);
     g_sumfac_kernel = kind;
+    return CFK_OK;
+}
+
+extern "C" int cfk_sumfac_set_chunk_nodes(int32_t max_nodes) {
+    if (max_nodes < 0) return set_error(CFK_EINVAL, "sumfac chunk: %d x nodes", max_nodes);
+    g_sumfac_chunk_nodes = max_nodes;
+    return CFK_OK;
+}
+
+extern "C" int cfk_sumfac_last_chunks(int64_t *n_chunks) {
+    if (!n_chunks) return set_error(CFK_EINVAL, "null pointer");
+    *n_chunks = g_last_timing.chunks;
     return CFK_OK;
 }
 

@@ -106,6 +106,14 @@ int cfk_compute_rtensor_sumfac(double *R_out, int32_t K_max, int32_t N_L, int32_
 #define CFK_SUMFAC_MATRIX_CORE 0
 #define CFK_SUMFAC_PER_CHANNEL 1
 int cfk_sumfac_set_kernel(int32_t kind);
+/* Largest number of x nodes the sumfac processes per chunk (one stage-A and one fold launch over
+ * per-unit buffers of that many nodes; process-wide).  0 (the default) sizes chunks by the unit
+ * buffers' memory cap (at L=12, K=12, 48 nodes per axis: 4 chunks of 12 nodes).  Results do not
+ * depend on the chunking beyond rounding (the fold adds each chunk's units into R).  No reference
+ * counterpart (the reference's OpenMP threads each take x nodes, compute_rtensor_sumfac_mex.cpp:134). */
+int cfk_sumfac_set_chunk_nodes(int32_t max_nodes);
+/* Number of x-node chunks of the last sumfac call. */
+int cfk_sumfac_last_chunks(int64_t *n_chunks);
 /* Device time of the last sumfac call: the kernel that ran (tables whose valid q_i are not all of
  * degree l1 run per channel), the units processed, and the summed HIP-event durations of the
  * unit (stage A) and fold (stage B) launches.  Any pointer may be NULL.  No reference counterpart. */
@@ -208,6 +216,12 @@ int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out);
  * in a workgroup.  Ignored for CFK_PREC_FP64.  The autotuner also tries it.  No reference counterpart. */
 int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split);
 int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out);
+
+/* Routing-table statistics of a plan, for work models: stats[0] N_G, [1] S (distinct (t,q1) slices),
+ * and for the symmetrized tables (0 when unavailable) [2] nonzeros kept (q2 <= q3), [3] slices,
+ * [4] q2 groups (distinct (t,q1,q2)), [5] (k2,k3)-symmetric slices (l2 == l3), [6] slices whose
+ * partner block is l = 0 (FP64 radial stage on the mixed path); [7] N_L.  No reference counterpart. */
+int cfk_plan_get_stats(cfk_plan plan, int64_t stats[8]);
 
 /* Kernel timing: when enabled, every execute brackets its main contraction kernel (not the
  * layout/precision conversions around it) with a pair of HIP events on the execution stream.

@@ -74,3 +74,26 @@ def test_single_hip_runtime_in_process():
     from cfk._lib import hip_runtimes_loaded
     lib()
     assert len(hip_runtimes_loaded()) == 1, hip_runtimes_loaded()
+
+
+def test_multiplan_and_plan_release_their_handles():
+    """close() and __del__ are methods of both wrappers (the native objects own graphs, streams and
+    device buffers that must be freed)."""
+    for cls in (cfk.CollisionPlan, cfk.MultiPlan):
+        assert callable(getattr(cls, "close", None)), cls
+        assert callable(getattr(cls, "__del__", None)), cls
+
+
+def test_device_buffer_checks_run_before_the_library():
+    """Q/f buffers are checked for dtype, device, contiguity and size on the host: a float32, host or
+    strided tensor of the right element count would otherwise be read past its end."""
+    import torch
+    from cfk.plan import _check_dev_f64
+    with pytest.raises(TypeError):
+        _check_dev_f64(np.zeros(8), 8, "Q")
+    with pytest.raises(ValueError):
+        _check_dev_f64(torch.zeros(8, dtype=torch.float64), 8, "Q")   # not on the GPU
+    with pytest.raises(ValueError):
+        _check_dev_f64(torch.zeros(8, dtype=torch.float32), 8, "f")
+    with pytest.raises(ValueError):
+        _check_dev_f64(torch.zeros(4, 4, dtype=torch.float64).t(), 16, "f")
