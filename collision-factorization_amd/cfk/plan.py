@@ -76,8 +76,11 @@ class CollisionPlan:
                                        _stream_ptr(stream)))
         return J
 
-    def set_tile(self, precision, kc=0, nb=0, pf=0):
-        """Fix one precision's tile (0 = default); for "mixed", pf=1 runs the radial stage on MFMA."""
+    def set_tile(self, precision, kc=0, nb=0, pf=None):
+        """Fix one precision's tile (0 = default).  For "mixed", pf=1 runs the radial stage on the matrix
+        cores, pf=0 on the VALU, pf=None (-1) picks by K (matrix cores from K = 12 up)."""
+        if pf is None:
+            pf = -1 if precision == "mixed" else 0
         check(lib().cfk_plan_set_tile(self._h, PRECISION[precision], int(kc), int(nb), int(pf)))
 
     def set_waves(self, precision, wpb=0):
@@ -90,8 +93,9 @@ class CollisionPlan:
         check(lib().cfk_plan_get_waves(self._h, PRECISION[precision], ctypes.byref(w)))
         return int(w.value)
 
-    def set_split(self, precision, split=0):
-        """Row split between two waves per batch tile (symmetrized precisions; 0 = off)."""
+    def set_split(self, precision, split=-1):
+        """Row split between two waves per batch tile (symmetrized precisions; 0 = off, 1 = on, -1 = the
+        selected kernel's default)."""
         check(lib().cfk_plan_set_split(self._h, PRECISION[precision], int(split)))
 
     def split(self, precision):

@@ -47,8 +47,6 @@ int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStr
 int default_kc(int K);
 int table_padding();
 int half_kc(int K);
-bool mf_supports_k(int K);
-int launch_af_mf(cfk_plan_s *p, const float *fT, double *QT, int64_t B, int accumulate, hipStream_t st);
 int launch_af_sym_f64(const cfk_plan_s *p, const Tile &tl, const double *fT, double *QT, int64_t B, int accumulate,
                       hipStream_t st);
 int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, double *QT, int64_t B, int accumulate,
@@ -57,6 +55,21 @@ int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t col
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
 int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st);
 bool af_f32_packed(int K, const Tile &tl);
+bool mx_supports_k(int K);
+int mx_chunk(int K);
+int build_mx_tables(cfk_plan_s *p, std::vector<float> &out, int64_t &n_groups);
+int launch_af_mx(const cfk_plan_s *p, const float *Rmx, const Tile &tl, const float *fT, double *QT, int64_t B,
+                 int accumulate, hipStream_t st);
+
+// A operands of the matrix-core kernel, built on first use.
+int ensure_mx_tables(cfk_plan_s *p) {
+    if (p->sym_Rmx.p) return CFK_OK;
+    std::vector<float> t;
+    int64_t ng = 0;
+    int rc = build_mx_tables(p, t, ng);
+    if (rc) return rc;
+    return p->sym_Rmx.upload(t.data(), t.size());
+}
 
 // 1-based integral index stored as double, checked against [1, hi].
 static inline bool idx_ok(double v, int64_t hi) { return v >= 1.0 && v <= (double)hi && v == std::floor(v); }
@@ -300,6 +313,27 @@ static int default_tile_kc(int precision, int K) {
     return (precision != CFK_PREC_MIXED && K >= 8) ? half_kc(K) : default_kc(K);
 }
 
+// The tile a plan runs for a precision, defaults resolved.  The mixed path's default kernel follows
+// the measured crossover (batch 1024-4096 on MI355X): the radial stage on the matrix cores with the
+// row split from K = 12 up (K = 14: 0.97 vs 1.44 ms for 1024 distributions at L = 12), the VALU
+// kernel below (K = 6, 8: the VALU kernel is faster; K = 10: within a few percent).
+Tile plan_tile(const cfk_plan_s *p, int precision) {
+    Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
+            p->nb[precision] > 0 ? p->nb[precision] : 1, p->pf[precision],
+            p->wpb[precision] > 0 ? p->wpb[precision] : 1, p->split[precision]};
+    if (precision == CFK_PREC_MIXED) {
+        if (tl.pf < 0) tl.pf = (p->K >= 12 && mx_supports_k(p->K)) ? 1 : 0;
+        if (tl.pf == 1 && !mx_supports_k(p->K)) tl.pf = 0;
+        if (tl.split < 0) tl.split = tl.pf == 1 ? 1 : 0;
+        if (tl.pf == 1) {
+            tl.kc = mx_chunk(p->K);
+            tl.nb = 1;
+        }
+    }
+    if (tl.split < 0) tl.split = 0;
+    return tl;
+}
+
 int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *Q, const double *f, int64_t B,
                  hipStream_t st) {
     if (B < 0) return set_error(CFK_EINVAL, "B must be >= 0");
@@ -319,11 +353,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if (p->N_G == 0) return CFK_OK;  // no Gaunt entries: Q(f) = 0
     const int64_t NK = (int64_t)p->N_Q * p->K;
     const bool natural = layout == CFK_LAYOUT_NATURAL;
-    const Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
-                  p->nb[precision] > 0 ? p->nb[precision] : 1,
-                  p->pf[precision],
-                  p->wpb[precision] > 0 ? p->wpb[precision] : 1,
-                  p->split[precision]};
+    const Tile tl = plan_tile(p, precision);
     int rc;
     if (precision != CFK_PREC_FP64) {
         if (variant != CFK_VARIANT_ANGULAR_FIRST)
@@ -331,9 +361,10 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
         if (!p->sym_ok) return set_error(CFK_EUNSUP, "symmetrized tables unavailable: %s", p->sym_why.c_str());
     }
     if (precision == CFK_PREC_MIXED) {
-        const bool mf = tl.pf != 0 && mf_supports_k(p->K);  // matrix-core radial stage
-        // the VALU kernel with a full k3 chunk reads packed rows [N_Q][B][KP]; the others [K][N_Q][B]
-        const bool packed = !mf && af_f32_packed(p->K, tl);
+        const bool mx = tl.pf == 1;  // radial stage on the matrix cores
+        if (mx && (rc = ensure_mx_tables(p))) return rc;
+        // kernels with a full k3 chunk read packed rows [N_Q][B][KP]; the others [K][N_Q][B]
+        const bool packed = mx ? mx_chunk(p->K) == p->K : af_f32_packed(p->K, tl);
         const int64_t nf = packed ? (int64_t)p->N_Q * ((p->K + 3) / 4 * 4) * B : NK * B;
         if ((rc = p->fT32.ensure((size_t)nf))) return rc;
         if (natural) {
@@ -342,7 +373,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
                              : launch_transpose_f32(f, p->fT32.p, B, NK, st)))
                 return rc;
             KernelTimer tm(p, st);
-            if ((rc = mf ? launch_af_mf(p, p->fT32.p, p->QT.p, B, 0, st)
+            if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, p->fT32.p, p->QT.p, B, 0, st)
                          : launch_af_sym_f32(p, tl, p->fT32.p, p->QT.p, B, 0, st)))
                 return rc;
             tm.stop();
@@ -352,7 +383,9 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
                          : launch_to_f32(f, p->fT32.p, NK * B, st)))
             return rc;
         KernelTimer tm(p, st);
-        if ((rc = mf ? launch_af_mf(p, p->fT32.p, Q, B, 1, st) : launch_af_sym_f32(p, tl, p->fT32.p, Q, B, 1, st))) return rc;
+        if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, p->fT32.p, Q, B, 1, st)
+                     : launch_af_sym_f32(p, tl, p->fT32.p, Q, B, 1, st)))
+            return rc;
         tm.stop();
         return CFK_OK;
     }
@@ -509,20 +542,21 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
     for (int prec = 0; prec < 3; ++prec) {
         if (prec != CFK_PREC_FP64 && !p->sym_ok) continue;
         double best = 1e300;
-        Tile bt{default_kc(p->K), 1, 0, 1, 0};
+        Tile bt = plan_tile(p, prec);
         for (int c = 0; c < 2; ++c) {
             if (c == 1 && kcs[1] == kcs[0]) continue;
             for (int nb = 1; nb <= 2; ++nb)
-                for (int pf = 0; pf <= ((prec == CFK_PREC_FP64 || (prec == CFK_PREC_MIXED && mf_supports_k(p->K))) ? 1 : 0);
+                for (int pf = 0; pf <= ((prec == CFK_PREC_FP64 || (prec == CFK_PREC_MIXED && mx_supports_k(p->K))) ? 1 : 0);
                      ++pf)
                 for (int v = 0; v < 5; ++v) {
                     // v = 0..2: 1, 2, 4 waves per workgroup; v = 3, 4: row split (symmetrized tables, two
                     // waves per batch tile) with 1 or 2 tiles per workgroup
                     const int split = v >= 3 ? 1 : 0;
                     const int wpb = split ? v - 2 : 1 << v;
-                    if (split && (prec == CFK_PREC_FP64 || pf != 0)) continue;
-                    // mixed: pf = 1 selects the matrix-core radial stage (fixed chunk, one distribution per lane)
-                    if (prec == CFK_PREC_MIXED && pf == 1 && (nb != 1 || c != 1 || wpb != 1)) continue;
+                    if (split && prec == CFK_PREC_FP64) continue;
+                    // mixed: pf = 1 selects the matrix-core radial stage (its own k3 chunk, one distribution
+                    // per lane), timed once
+                    if (prec == CFK_PREC_MIXED && pf == 1 && (nb != 1 || c != 0)) continue;
                     p->kc[prec] = kcs[c];
                     p->nb[prec] = nb;
                     p->pf[prec] = pf;
@@ -545,7 +579,7 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
                                 kcs[c], nb, pf, wpb, split, nl > 0 ? ms / nl : -1.0);
                     if (nl > 0 && ms / nl < best) {
                         best = ms / nl;
-                        bt = Tile{kcs[c], nb, pf, wpb, split};
+                        bt = plan_tile(p, prec);
                     }
                 }
         }
@@ -559,9 +593,10 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *kc_out) {
     CFK_HIP(hipStreamSynchronize(st));
     if (kc_out)
         for (int i = 0; i < 3; ++i) {
-            kc_out[3 * i] = p->kc[i] > 0 ? p->kc[i] : default_tile_kc(i, p->K);
-            kc_out[3 * i + 1] = p->nb[i] > 0 ? p->nb[i] : 1;
-            kc_out[3 * i + 2] = p->pf[i];
+            const Tile t = plan_tile(p, i);
+            kc_out[3 * i] = t.kc;
+            kc_out[3 * i + 1] = t.nb;
+            kc_out[3 * i + 2] = t.pf;
         }
     return CFK_OK;
 }
@@ -599,9 +634,9 @@ int cfk_plan_linearize(cfk_plan plan, const double *f0, double *J, void *stream)
 int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf) {
     if (!plan) return set_error(CFK_EINVAL, "null plan");
     if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
-    if (kc < 0 || kc > plan->K || nb < 0 || nb > 2 || pf < 0 || pf > 1)
+    if (kc < 0 || kc > plan->K || nb < 0 || nb > 2 || pf < -1 || pf > 1 || (pf == -1 && precision != CFK_PREC_MIXED))
         return set_error(CFK_EINVAL, "tile (%d, %d, %d) out of range", kc, nb, pf);
-    if (precision == CFK_PREC_MIXED && pf == 1 && !mf_supports_k(plan->K))
+    if (precision == CFK_PREC_MIXED && pf == 1 && !mx_supports_k(plan->K))
         return set_error(CFK_EUNSUP, "matrix-core radial stage: K=%d not in {6,8,10,12,14}", plan->K);
     plan->kc[precision] = kc;
     plan->nb[precision] = nb;
@@ -621,7 +656,7 @@ int cfk_plan_set_waves(cfk_plan plan, int32_t precision, int32_t wpb) {
 int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split) {
     if (!plan) return set_error(CFK_EINVAL, "null plan");
     if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
-    if (split != 0 && split != 1) return set_error(CFK_EINVAL, "split must be 0 or 1 (got %d)", split);
+    if (split < -1 || split > 1) return set_error(CFK_EINVAL, "split must be -1, 0 or 1 (got %d)", split);
     plan->split[precision] = split;
     return CFK_OK;
 }
@@ -629,7 +664,7 @@ int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split) {
 int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out) {
     if (!plan || !split_out) return set_error(CFK_EINVAL, "null argument");
     if (precision < 0 || precision > 2) return set_error(CFK_EINVAL, "unknown precision %d", precision);
-    *split_out = plan->split[precision];
+    *split_out = plan_tile(plan, precision).split;
     return CFK_OK;
 }
 

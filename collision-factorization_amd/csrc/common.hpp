@@ -74,6 +74,8 @@ struct MultiGroupDev {
     const float *zg;
     const double *R;
     const float *R32;
+    const float *Rmx;         // A operands of the matrix-core kernel (null for the VALU kernel)
+    const int32_t *row_mid;   // row split points (matrix-core kernel with the split)
     int64_t B;
     int N_Q, K;
 };
@@ -130,7 +132,7 @@ struct cfk_plan_s {
     cfk::DevBuf<int2> sym_zq;
     cfk::DevBuf<double> sym_zg, sym_R;
     cfk::DevBuf<float> sym_zg32, sym_R32;
-    cfk::DevBuf<float> sym_Rmf;  // R' in the matrix-core operand order (built on first use)
+    cfk::DevBuf<float> sym_Rmx;  // R' as A operands of the matrix-core kernel, 4-step groups (built on first use)
     std::vector<double> sym_row_work;  // cost model per output row (symmetrized tables)
     // scratch for the natural layout (batch-fastest staging)
     cfk::DevBuf<double> fT, QT;
@@ -138,9 +140,9 @@ struct cfk_plan_s {
     // tile choice per precision (index = CFK_PREC_*), set by the autotuner; 0 = default
     int kc[3] = {0, 0, 0};
     int nb[3] = {0, 0, 0};   // distributions per lane
-    int pf[3] = {0, 0, 0};   // FP64: prefetch next nonzero (plain tables); mixed: matrix-core radial stage
+    int pf[3] = {0, -1, 0};  // FP64: prefetch next nonzero (plain tables); mixed: matrix-core radial stage (-1: by K)
     int wpb[3] = {0, 0, 0};  // waves per workgroup of the angular-first kernels (0: default)
-    int split[3] = {0, 0, 0};  // row split between two waves (symmetrized tables)
+    int split[3] = {0, -1, 0};  // row split between two waves (symmetrized tables; -1: default of the kernel)
     // kernel timing (cfk_plan_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

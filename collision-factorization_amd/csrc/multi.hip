@@ -1,7 +1,9 @@
 // Batched uneven-size execution: several plans (different L and K) in one hipGraph, captured once
 // and replayed (cfk_multi_* in include/cfk.h).  The graph converts each group's f to FP32, then forks
 // one item kernel per radial size K onto parallel branches, so the kernels of different K share the
-// GPU and fill each other's tails.
+// GPU and fill each other's tails.  Each group runs the item kernel its plan's mixed-precision tile
+// selects: the VALU kernel (contract.hip) or the matrix-core one (contract_mx.hip, with or without
+// the row split).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,10 +15,13 @@
 
 namespace cfk {
 int launch_multi(const MultiGroupDev *groups, const int4 *items, int K, int KC, int i0, int i1, hipStream_t st);
+int launch_multi_mx(const MultiGroupDev *groups, const int4 *items, int K, bool split, int i0, int i1,
+                    hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
 int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st);
 bool multi_supports_k(int K);
-int default_kc(int K);
+Tile plan_tile(const cfk_plan_s *p, int precision);
+int ensure_mx_tables(cfk_plan_s *p);
 }  // namespace cfk
 
 using namespace cfk;
@@ -39,6 +44,7 @@ struct cfk_multi_s {
     bool prepared = false;
     struct Launch {
         int K, KC, i0, i1;
+        bool mx, split;  // radial stage on the matrix cores; two waves per item
     };
     std::vector<Launch> launches;  // one item kernel per K-class
     std::vector<std::unique_ptr<DevBuf<float>>> f32;
@@ -64,21 +70,29 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
     const int G = (int)m->plans.size();
     m->reset();
     m->prepared = false;
-    // K-classes: groups that share (K, KC) run in one item kernel
+    // K-classes: groups that share the item kernel -- (K, KC) on the VALU kernel, (K, split) on the
+    // matrix-core one -- as each plan's mixed-precision tile selects it
     struct Cls {
         int K, KC;
+        bool mx, split;
         double work = 0.0;
         std::vector<int4> items;
     };
     std::vector<Cls> cls;
     std::vector<int> cls_of(G, -1);
+    std::vector<Tile> tiles(G);
+    int rc;
     for (int g = 0; g < G; ++g) {
+        cfk_plan_s *p = m->plans[g];
+        tiles[g] = plan_tile(p, CFK_PREC_MIXED);
         if (B[g] <= 0) continue;
-        const cfk_plan_s *p = m->plans[g];
-        const int kc = p->kc[CFK_PREC_MIXED] > 0 ? p->kc[CFK_PREC_MIXED] : default_kc(p->K);
+        const Tile &t = tiles[g];
+        const bool mx = t.pf == 1, split = mx && t.split;
+        if (mx && (rc = ensure_mx_tables(p))) return rc;
         int c = 0;
-        while (c < (int)cls.size() && !(cls[c].K == p->K && cls[c].KC == kc)) ++c;
-        if (c == (int)cls.size()) cls.push_back(Cls{p->K, kc});
+        while (c < (int)cls.size() && !(cls[c].K == p->K && cls[c].KC == t.kc && cls[c].mx == mx && cls[c].split == split))
+            ++c;
+        if (c == (int)cls.size()) cls.push_back(Cls{p->K, t.kc, mx, split});
         cls_of[g] = c;
     }
     // work items of each class, heaviest first: cost of (group,row) from the plan's cost model, one
@@ -109,21 +123,22 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
         off.push_back((int)items.size());
     }
     m->n_items = (int)items.size();
-    int rc;
     if ((rc = m->items.upload(items.data(), items.size()))) return rc;
     m->f32.clear();
     m->packed.assign(G, 0);
     std::vector<MultiGroupDev> gd(G);
     for (int g = 0; g < G; ++g) {
         cfk_plan_s *p = m->plans[g];
-        // (from the group's own chunk, as its class was formed: cls has been re-sorted since)
-        const int kc = p->kc[CFK_PREC_MIXED] > 0 ? p->kc[CFK_PREC_MIXED] : default_kc(p->K);
-        m->packed[g] = B[g] > 0 && kc >= p->K;
+        // (from the group's own tile, as its class was formed: cls has been re-sorted since); kernels
+        // with a full k3 chunk read packed FP32 rows
+        const bool mx = tiles[g].pf == 1;
+        m->packed[g] = B[g] > 0 && tiles[g].kc >= p->K;
         const int64_t n = (int64_t)p->N_Q * (m->packed[g] ? (p->K + 3) / 4 * 4 : p->K) * B[g];
         m->f32.emplace_back(new DevBuf<float>());
         if ((rc = m->f32.back()->alloc((size_t)std::max<int64_t>(n, 1)))) return rc;
         gd[g] = MultiGroupDev{m->f32.back()->p, Q[g], p->sym_slices.p, p->sym_row_ptr.p, p->sym_zq.p,
-                              p->sym_zg32.p, p->sym_R.p, p->sym_R32.p, B[g], p->N_Q, p->K};
+                              p->sym_zg32.p, p->sym_R.p, p->sym_R32.p, mx ? p->sym_Rmx.p : nullptr,
+                              p->sym_row_mid.p, B[g], p->N_Q, p->K};
     }
     if ((rc = m->groups.upload(gd.data(), gd.size()))) return rc;
     const int nc = (int)cls.size();
@@ -131,7 +146,7 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
     for (int c = 0; c < nc; ++c) {
         if (!m->branch[c]) CFK_HIP(hipStreamCreateWithFlags(&m->branch[c], hipStreamNonBlocking));
         if (!m->ev[c]) CFK_HIP(hipEventCreateWithFlags(&m->ev[c], hipEventDisableTiming));
-        m->launches.push_back(cfk_multi_s::Launch{cls[c].K, cls[c].KC, off[c], off[c + 1]});
+        m->launches.push_back(cfk_multi_s::Launch{cls[c].K, cls[c].KC, off[c], off[c + 1], cls[c].mx, cls[c].split});
     }
     if (!m->ev[MULTI_MAX_GROUPS]) CFK_HIP(hipEventCreateWithFlags(&m->ev[MULTI_MAX_GROUPS], hipEventDisableTiming));
     m->key_f.assign(f, f + G);
@@ -158,7 +173,9 @@ static int multi_enqueue(cfk_multi_s *m, hipStream_t st) {
     for (int c = 0; c < nc; ++c) {
         const cfk_multi_s::Launch &l = m->launches[c];
         CFK_HIP(hipStreamWaitEvent(m->branch[c], m->ev[MULTI_MAX_GROUPS], 0));
-        if ((rc = launch_multi(m->groups.p, m->items.p, l.K, l.KC, l.i0, l.i1, m->branch[c]))) return rc;
+        if ((rc = l.mx ? launch_multi_mx(m->groups.p, m->items.p, l.K, l.split, l.i0, l.i1, m->branch[c])
+                       : launch_multi(m->groups.p, m->items.p, l.K, l.KC, l.i0, l.i1, m->branch[c])))
+            return rc;
         CFK_HIP(hipEventRecord(m->ev[c], m->branch[c]));
         CFK_HIP(hipStreamWaitEvent(st, m->ev[c], 0));
     }

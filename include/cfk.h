@@ -178,8 +178,9 @@ int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t 
 
 /* Autotuner: times every tile variant of the angular-first kernels -- k3 chunk width KC in
  * {K, ceil(K/2)} x distributions per lane NB in {1,2} x a third switch PF: for FP64 the prefetch of
- * the next nonzero's rows, for MIXED the radial stage on the matrix cores (v_mfma_f32_16x16x4_f32,
- * KC = ceil(K/2), NB = 1) -- for each precision on a synthetic batch of B distributions
+ * the next nonzero's rows, for MIXED the radial stage on the matrix cores (v_mfma_f32_16x16x4_f32
+ * with the operands moved between lanes by register transposes; its own chunk: KC = K up to K = 10,
+ * ceil(K/2) above; NB = 1) -- for each precision on a synthetic batch of B distributions
  * (batch-fastest layout), each also at 1, 2 and 4 waves per workgroup (cfk_plan_set_waves) and, for
  * the symmetrized precisions, with the row split (cfk_plan_set_split), on
  * `stream` and keeps the fastest per precision in the plan.  When
@@ -196,7 +197,8 @@ int cfk_plan_autotune(cfk_plan plan, int64_t B, void *stream, int32_t *tiles_out
 int cfk_plan_linearize(cfk_plan plan, const double *f0, double *J, void *stream);
 
 /* Sets one precision's tile (KC, NB, PF as above; KC = 0 and NB = 0 select the defaults) without
- * timing anything. */
+ * timing anything.  For MIXED, PF = -1 (the default of a new plan) selects the kernel by K: the
+ * matrix-core radial stage with the row split from K = 12 up, the VALU kernel below. */
 int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf);
 
 /* Waves per workgroup of one precision's angular-first kernel: wpb in {1, 2, 4} puts that many batch
@@ -215,6 +217,8 @@ int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out);
  * end of the launch; a waves-per-workgroup setting of 2 or more then puts two batch tiles (four waves)
  * in a workgroup.  Ignored for CFK_PREC_FP64.  The autotuner also tries it.  No reference counterpart. */
 int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split);
+/* (split = -1: the default of the selected kernel -- on for the matrix-core MIXED kernel, else off;
+ * cfk_plan_get_split reports the resolved value.) */
 int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out);
 
 /* Routing-table statistics of a plan, for work models: stats[0] N_G, [1] S (distinct (t,q1) slices),

@@ -185,8 +185,11 @@ def test_multi_rejects_unsupported_groups(torch_cuda):
     p.close()
 
 
-def test_matrix_core_radial_stage_per_k(groups, torch_cuda):
-    """The mixed path with the radial stage on the matrix cores, for every supported K, against the
+@pytest.mark.parametrize("pf,split", [(1, 0), (1, 1), (0, 0), (0, 1)],
+                         ids=["matrix_cores", "matrix_cores_split", "valu", "valu_split"])
+def test_each_mixed_kernel_per_k(groups, torch_cuda, pf, split):
+    """The single-plan mixed path on each kernel -- radial stage on the matrix cores (pf=1) or on the
+    VALU (pf=0), with and without the row split -- for every K of the batched path, against the
     oracle on a sample and the FP64 GPU operator on the whole ragged batch."""
     torch = torch_cuda
     for i, k in enumerate(CONFIG2 + EXTRA):
@@ -195,14 +198,53 @@ def test_matrix_core_radial_stage_per_k(groups, torch_cuda):
         f, fT = _batch(torch, g, B, 200 + i)
         ref = torch.zeros_like(fT)
         g["plan"].execute(ref, fT, B, precision="fp64", layout="batch_fastest")
-        g["plan"].set_tile("mixed", 0, 1, 1)
-        print("matrix-core radial stage L=%d K=%d" % (g["L"], g["K"]), flush=True)
+        g["plan"].set_tile("mixed", 0, 1, pf)
+        g["plan"].set_split("mixed", split)
+        print("mixed kernel pf=%d split=%d L=%d K=%d" % (pf, split, g["L"], g["K"]), flush=True)
         Q = torch.zeros_like(fT)
         g["plan"].execute(Q, fT, B, precision="mixed", layout="batch_fastest")
         torch.cuda.synchronize()
         g["plan"].set_tile("mixed")
+        g["plan"].set_split("mixed")
         Qr = _rows(Q)
         assert rel_rows(Qr, _rows(ref)) <= TOL_MIXED, k
         pick = np.r_[0:4, B - 4:B]
         assert rel_rows(Qr[pick], oracle_q(g, f[pick])) <= TOL_MIXED, k
         assert invariants_zero(Qr, g["nq"]), k
+
+
+@pytest.mark.parametrize("pf,split", [(0, 0), (1, 0), (1, 1)], ids=["valu", "matrix_cores", "matrix_cores_split"])
+def test_multi_each_item_kernel(groups, torch_cuda, pf, split):
+    """The batched path with every group forced onto one item kernel (VALU, or matrix cores with and
+    without the row split) against the FP64 GPU operator and, on the first and last tiles, the oracle."""
+    torch = torch_cuda
+    keys = CONFIG2 + EXTRA
+    Bs = [100, 70, 129, 65, 66]
+    fs, Qs, refs, orefs = [], [], [], []
+    for i, (k, B) in enumerate(zip(keys, Bs)):
+        g = groups[k]
+        g["plan"].set_tile("mixed", 0, 1, pf)
+        g["plan"].set_split("mixed", split)
+        f, fT = _batch(torch, g, B, 400 + i)
+        ref = torch.zeros_like(fT)
+        g["plan"].execute(ref, fT, B, precision="fp64", layout="batch_fastest")
+        fs.append(fT)
+        Qs.append(torch.zeros_like(fT))
+        refs.append(_rows(ref))
+        pick = np.r_[0:2, B - 2:B]
+        orefs.append((pick, oracle_q(g, f[pick])))
+    try:
+        mp = cfk.MultiPlan([groups[k]["plan"] for k in keys])
+        print("multi, every group pf=%d split=%d" % (pf, split), flush=True)
+        mp.execute(Qs, fs, Bs)
+        torch.cuda.synchronize()
+        for k, Q, ref, (pick, oref) in zip(keys, Qs, refs, orefs):
+            Qr = _rows(Q)
+            assert rel_rows(Qr, ref) <= TOL_MIXED, k
+            assert rel_rows(Qr[pick], oref) <= TOL_MIXED, k
+            assert invariants_zero(Qr, groups[k]["nq"]), k
+        mp.close()
+    finally:
+        for k in keys:
+            groups[k]["plan"].set_tile("mixed")
+            groups[k]["plan"].set_split("mixed")

@@ -166,6 +166,7 @@ def test_autotuned_tiles_keep_parity(config1, torch_cuda):
     tiles = c["plan"].autotune(B)
     assert set(tiles) == {"fp64", "mixed", "fp64_sym"}
     assert all(kc in (K, (K + 1) // 2) and nb in (1, 2) and pf in (0, 1) for kc, nb, pf in tiles.values()), tiles
+    assert c["plan"].split("mixed") in (0, 1)
     f = perturbed_maxwellians(c["L"], K, B, seed=99)
     Qref = oracle.angular_first_batch(np.zeros(B * nq * K), f, c["lab"], c["val"], c["ic"], c["R"], nq, K,
                                       nthreads=8)
@@ -188,10 +189,11 @@ def test_every_mixed_tile_keeps_parity(config1, torch_cuda):
                                       nthreads=8)
     fd = torch.from_numpy(f).cuda()
     fT = fd.reshape(B, -1).t().contiguous()
-    for kc, nb, pf, wpb in ((K, 1, 0, 1), ((K + 1) // 2, 1, 0, 1), (K, 2, 0, 1), ((K + 1) // 2, 1, 1, 1),
-                            (K, 1, 0, 2), (K, 1, 0, 4), (K, 2, 0, 4)):
+    for kc, nb, pf, wpb in ((K, 1, 0, 1), ((K + 1) // 2, 1, 0, 1), (K, 2, 0, 1), (K, 1, 0, 2), (K, 1, 0, 4),
+                            (K, 2, 0, 4), (0, 1, 1, 1), (0, 1, 1, 2), (0, 1, 1, 4)):
         c["plan"].set_tile("mixed", kc, nb, pf)
         c["plan"].set_waves("mixed", wpb)
+        c["plan"].set_split("mixed", 0)
         print("mixed tile", kc, nb, pf, wpb, flush=True)
         Qd = torch.zeros(B * nq * K, dtype=torch.float64, device="cuda")
         c["plan"].execute(Qd, fd, B, precision="mixed")
@@ -205,6 +207,7 @@ def test_every_mixed_tile_keeps_parity(config1, torch_cuda):
         assert float(np.abs(Qn[:, 0, :4]).max()) == 0.0 and float(np.abs(Qn[:, 1, 0]).max()) == 0.0
     c["plan"].set_tile("mixed")
     c["plan"].set_waves("mixed")
+    c["plan"].set_split("mixed")
 
 
 def test_waves_per_workgroup_keep_parity(config1, torch_cuda):
@@ -240,8 +243,10 @@ def test_row_split_keeps_parity_and_is_deterministic(config1, torch_cuda):
                                       nthreads=8)
     fd = torch.from_numpy(f).cuda()
     fT = fd.reshape(B, -1).t().contiguous()
-    for prec, tol, wpb in (("fp64_sym", TOL64, 1), ("mixed", TOL_MIXED, 1), ("fp64_sym", TOL64, 2),
-                           ("mixed", TOL_MIXED, 2)):
+    for prec, tol, wpb, pf in (("fp64_sym", TOL64, 1, 0), ("mixed", TOL_MIXED, 1, 0), ("fp64_sym", TOL64, 2, 0),
+                               ("mixed", TOL_MIXED, 2, 0), ("mixed", TOL_MIXED, 1, 1), ("mixed", TOL_MIXED, 2, 1)):
+        if prec == "mixed":
+            c["plan"].set_tile(prec, 0, 1, pf)   # pf = 1: radial stage on the matrix cores
         c["plan"].set_split(prec, 1)
         c["plan"].set_waves(prec, wpb)   # 2: two batch tiles per workgroup
         assert c["plan"].split(prec) == 1
@@ -261,5 +266,6 @@ def test_row_split_keeps_parity_and_is_deterministic(config1, torch_cuda):
         torch.cuda.synchronize()
         assert bool((Q1b == Q1).all()), prec
         assert bool(((Q2 - 0.25) - Q1).abs().max() <= 1e-12 * Q1.abs().max()), prec
-        c["plan"].set_split(prec, 0)
+        c["plan"].set_split(prec)
         c["plan"].set_waves(prec)
+        c["plan"].set_tile(prec)
