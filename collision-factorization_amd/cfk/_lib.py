@@ -69,6 +69,7 @@ SIGNATURES = {
                           ctypes.c_int),
     "cfk_plan_create": ([_dp, _dp, _dp, _i64, _dp, _i64, _i32, _i32, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_plan_destroy": ([_vp], ctypes.c_int),
+    "cfk_plan_execute_static": ([_vp, _i32, _i32, _i32, _vp, _vp, _i64, _vp], ctypes.c_int),
     "cfk_plan_get_stats": ([_vp, ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_plan_execute": ([_vp, _i32, _i32, _i32, _vp, _vp, _i64, _vp], ctypes.c_int),
     "cfk_plan_autotune": ([_vp, _i64, _vp, ctypes.POINTER(_i32)], ctypes.c_int),

@@ -66,6 +66,20 @@ class CollisionPlan:
                                      _ptr(f), int(B), _stream_ptr(stream)))
         return Q
 
+    def execute_static(self, Q, f, B=None, variant="angular_first", precision="fp64", layout="natural", stream=None):
+        """As execute, through cfk_plan_execute_static: no device allocation, staging in the library's
+        fixed per-GPU area (CFK_STATIC_SCRATCH_BYTES); overlapping calls on one GPU race on it."""
+        if self._h is None:
+            raise CfkError(-1, "plan destroyed")
+        n = self.N_Q * self.K
+        if B is None:
+            B = Q.numel() // n
+        _check_dev_f64(Q, B * n, "Q")
+        _check_dev_f64(f, B * n, "f")
+        check(lib().cfk_plan_execute_static(self._h, VARIANT[variant], PRECISION[precision], LAYOUT[layout], _ptr(Q),
+                                            _ptr(f), int(B), _stream_ptr(stream)))
+        return Q
+
     def linearize(self, f0=None, stream=None):
         """J = dQ/dc at f0 (default: the absolute Maxwellian), [N, N] with N = N_Q*K, index k*N_Q + q."""
         n = self.N_Q * self.K

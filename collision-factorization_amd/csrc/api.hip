@@ -334,8 +334,27 @@ Tile plan_tile(const cfk_plan_s *p, int precision) {
     return tl;
 }
 
+// Device staging an execute needs, in elements: the FP32 copy of f (mixed precision: packed rows
+// [N_Q][B][KP] for kernels with a full k3 chunk, else [K][N_Q][B]) and, for the natural layout, the
+// batch-fastest FP64 copies of f (FP64 precisions) and of Q.
+StagingNeed plan_staging_need(const cfk_plan_s *p, int precision, int layout, int64_t B) {
+    StagingNeed n{0, 0, 0};
+    const int64_t NK = (int64_t)p->N_Q * p->K;
+    const bool natural = layout == CFK_LAYOUT_NATURAL;
+    if (precision == CFK_PREC_MIXED) {
+        const Tile tl = plan_tile(p, precision);
+        const bool packed = tl.pf == 1 ? mx_chunk(p->K) == p->K : af_f32_packed(p->K, tl);
+        n.f32 = packed ? (int64_t)p->N_Q * ((p->K + 3) / 4 * 4) * B : NK * B;
+        if (natural) n.q64 = NK * B;
+    } else if (natural) {
+        n.f64 = NK * B;
+        n.q64 = NK * B;
+    }
+    return n;
+}
+
 int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *Q, const double *f, int64_t B,
-                 hipStream_t st) {
+                 hipStream_t st, const Staging *stg) {
     if (B < 0) return set_error(CFK_EINVAL, "B must be >= 0");
     if (B == 0) return CFK_OK;
     if (!Q || !f) return set_error(CFK_EINVAL, "null Q or f");
@@ -355,6 +374,15 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     const bool natural = layout == CFK_LAYOUT_NATURAL;
     const Tile tl = plan_tile(p, precision);
     int rc;
+    // staging: the caller's (fixed area, no allocation) or the plan's own buffers, grown as needed
+    const StagingNeed need = plan_staging_need(p, precision, layout, B);
+    float *s_f32 = nullptr;
+    double *s_f64 = nullptr, *s_q64 = nullptr;
+    if (stg) {
+        s_f32 = stg->f32;
+        s_f64 = stg->f64;
+        s_q64 = stg->q64;
+    }
     if (precision != CFK_PREC_FP64) {
         if (variant != CFK_VARIANT_ANGULAR_FIRST)
             return set_error(CFK_EUNSUP, "symmetrized/mixed precision is implemented for the angular-first variant only");
@@ -362,53 +390,65 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     }
     if (precision == CFK_PREC_MIXED) {
         const bool mx = tl.pf == 1;  // radial stage on the matrix cores
-        if (mx && (rc = ensure_mx_tables(p))) return rc;
+        if (mx && !p->sym_Rmx.p) {
+            if (stg) return set_error(CFK_EUNSUP, "matrix-core operand tables missing");
+            if ((rc = ensure_mx_tables(p))) return rc;
+        }
         // kernels with a full k3 chunk read packed rows [N_Q][B][KP]; the others [K][N_Q][B]
         const bool packed = mx ? mx_chunk(p->K) == p->K : af_f32_packed(p->K, tl);
-        const int64_t nf = packed ? (int64_t)p->N_Q * ((p->K + 3) / 4 * 4) * B : NK * B;
-        if ((rc = p->fT32.ensure((size_t)nf))) return rc;
+        if (!stg) {
+            if ((rc = p->fT32.ensure((size_t)need.f32))) return rc;
+            s_f32 = p->fT32.p;
+            if (natural) {
+                if ((rc = p->QT.ensure((size_t)need.q64))) return rc;
+                s_q64 = p->QT.p;
+            }
+        }
         if (natural) {
-            if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
-            if ((rc = packed ? launch_pack_rows(f, p->fT32.p, p->N_Q, p->K, B, true, st)
-                             : launch_transpose_f32(f, p->fT32.p, B, NK, st)))
+            if ((rc = packed ? launch_pack_rows(f, s_f32, p->N_Q, p->K, B, true, st)
+                             : launch_transpose_f32(f, s_f32, B, NK, st)))
                 return rc;
-            KernelTimer tm(p, st);
-            if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, p->fT32.p, p->QT.p, B, 0, st)
-                         : launch_af_sym_f32(p, tl, p->fT32.p, p->QT.p, B, 0, st)))
+            KernelTimer tm(p, st, stg == nullptr);
+            if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, s_f32, s_q64, B, 0, st)
+                         : launch_af_sym_f32(p, tl, s_f32, s_q64, B, 0, st)))
                 return rc;
             tm.stop();
-            return launch_transpose(p->QT.p, Q, NK, B, true, st);
+            return launch_transpose(s_q64, Q, NK, B, true, st);
         }
-        if ((rc = packed ? launch_pack_rows(f, p->fT32.p, p->N_Q, p->K, B, false, st)
-                         : launch_to_f32(f, p->fT32.p, NK * B, st)))
+        if ((rc = packed ? launch_pack_rows(f, s_f32, p->N_Q, p->K, B, false, st)
+                         : launch_to_f32(f, s_f32, NK * B, st)))
             return rc;
-        KernelTimer tm(p, st);
-        if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, p->fT32.p, Q, B, 1, st)
-                     : launch_af_sym_f32(p, tl, p->fT32.p, Q, B, 1, st)))
+        KernelTimer tm(p, st, stg == nullptr);
+        if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, s_f32, Q, B, 1, st)
+                     : launch_af_sym_f32(p, tl, s_f32, Q, B, 1, st)))
             return rc;
         tm.stop();
         return CFK_OK;
     }
     const bool sym = precision == CFK_PREC_FP64_SYM;
     if (!natural) {
-        KernelTimer tm(p, st);
+        KernelTimer tm(p, st, stg == nullptr);
         rc = sym ? launch_af_sym_f64(p, tl, f, Q, B, 1, st)
                  : launch_sparse_fp64(p, variant, tl, f, Q, B, 1, st);
         if (rc) return rc;
         tm.stop();
         return CFK_OK;
     }
-    if ((rc = p->fT.ensure((size_t)(NK * B)))) return rc;
-    if ((rc = p->QT.ensure((size_t)(NK * B)))) return rc;
-    if ((rc = launch_transpose(f, p->fT.p, B, NK, false, st))) return rc;
+    if (!stg) {
+        if ((rc = p->fT.ensure((size_t)need.f64))) return rc;
+        if ((rc = p->QT.ensure((size_t)need.q64))) return rc;
+        s_f64 = p->fT.p;
+        s_q64 = p->QT.p;
+    }
+    if ((rc = launch_transpose(f, s_f64, B, NK, false, st))) return rc;
     {
-        KernelTimer tm(p, st);
-        rc = sym ? launch_af_sym_f64(p, tl, p->fT.p, p->QT.p, B, 0, st)
-                 : launch_sparse_fp64(p, variant, tl, p->fT.p, p->QT.p, B, 0, st);
+        KernelTimer tm(p, st, stg == nullptr);
+        rc = sym ? launch_af_sym_f64(p, tl, s_f64, s_q64, B, 0, st)
+                 : launch_sparse_fp64(p, variant, tl, s_f64, s_q64, B, 0, st);
         if (rc) return rc;
         tm.stop();
     }
-    return launch_transpose(p->QT.p, Q, NK, B, true, st);
+    return launch_transpose(s_q64, Q, NK, B, true, st);
 }
 
 // One-shot host path shared by the three sparse MEX equivalents.
@@ -508,6 +548,11 @@ int cfk_plan_create(const double *labels, const double *vals, const double *ic_m
         return set_error(CFK_EHIP, "hipGetDevice failed");
     }
     if ((rc = build_plan(labels, vals, ic_map, N_G, R, R_numel, N_Q, K, p))) {
+        delete p;
+        return rc;
+    }
+    // the matrix-core kernel's operand tables are built here, so that no execute has to allocate them
+    if (p->sym_ok && mx_supports_k(K) && (rc = ensure_mx_tables(p))) {
         delete p;
         return rc;
     }

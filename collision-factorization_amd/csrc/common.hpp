@@ -155,17 +155,32 @@ struct cfk_plan_s {
 };
 
 namespace cfk {
-// Brackets one kernel launch with HIP events when the plan's timing is enabled.
+// Device staging of one plan execute (element counts from plan_staging_need; any may be zero).
+struct StagingNeed {
+    int64_t f32, f64, q64;
+};
+struct Staging {
+    float *f32;
+    double *f64, *q64;
+};
+StagingNeed plan_staging_need(const cfk_plan_s *p, int precision, int layout, int64_t B);
+int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *Q, const double *f, int64_t B,
+                 hipStream_t st, const Staging *stg = nullptr);
+
+// Brackets one kernel launch with HIP events when the plan's timing is enabled (and `allowed`: the
+// execute variant that allocates nothing creates no events).
 struct KernelTimer {
     cfk_plan_s *p;
     hipStream_t st;
+    bool on;
     hipEvent_t a = nullptr, b = nullptr;
-    KernelTimer(cfk_plan_s *plan, hipStream_t stream) : p(plan), st(stream) {
-        if (p->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+    KernelTimer(cfk_plan_s *plan, hipStream_t stream, bool allowed = true)
+        : p(plan), st(stream), on(allowed && plan->timing) {
+        if (on && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
             (void)hipEventRecord(a, st);
     }
     void stop() {
-        if (p->timing && a && b) {
+        if (on && a && b) {
             (void)hipEventRecord(b, st);
             p->events.emplace_back(a, b);
             a = b = nullptr;

@@ -176,6 +176,22 @@ int cfk_plan_destroy(cfk_plan plan);
 int cfk_plan_execute(cfk_plan plan, int32_t variant, int32_t precision, int32_t layout, double *Q_dev,
                      const double *f_dev, int64_t B, void *stream);
 
+/* Fixed device area, per GPU, of cfk_plan_execute_static. */
+#define CFK_STATIC_SCRATCH_BYTES ((size_t)64 * 1024 * 1024)
+
+/* cfk_plan_execute without any allocation: no hipMalloc, hipFree or allocating library call, so it may
+ * be used where a caller cannot allocate -- inside HIP graph capture, in tight serving loops, under
+ * harnesses that forbid allocation.  No counterpart in the reference (MATLAB allocates every MEX
+ * output itself).  Same arguments, kernels and results as cfk_plan_execute; the staging it needs (the
+ * mixed path's FP32 copy of f, and for CFK_LAYOUT_NATURAL the batch-fastest FP64 copies of f and Q)
+ * lives in one fixed device array of CFK_STATIC_SCRATCH_BYTES per GPU, held until the process ends;
+ * a call whose staging exceeds it returns CFK_EINVAL before launching anything (64 MiB covers L=8,
+ * K=10 at 4096 distributions in every precision and layout).  The batch-fastest FP64 precisions need no
+ * staging.  Calls that overlap on one GPU (several streams or threads) race on the shared array.
+ * Kernel timing (cfk_plan_set_timing) is not recorded by this entry point. */
+int cfk_plan_execute_static(cfk_plan plan, int32_t variant, int32_t precision, int32_t layout, double *Q_dev,
+                            const double *f_dev, int64_t B, void *stream);
+
 /* Autotuner: times every tile variant of the angular-first kernels -- k3 chunk width KC in
  * {K, ceil(K/2)} x distributions per lane NB in {1,2} x a third switch PF: for FP64 the prefetch of
  * the next nonzero's rows, for MIXED the radial stage on the matrix cores (v_mfma_f32_16x16x4_f32
