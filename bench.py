@@ -19,11 +19,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "collision-factorization_amd"))
 
 METRIC = "Q(f) evals/sec at HS Lmax=8 batch 4096 vs FP64 GPU path @<=1e-6 rel err"
+# MI355X published peaks: FP64 vector/matrix 78.6 TF/s; FP32 157.3 TF/s (matrix, and vector with packed
+# FP32, which this build does not use); FP32 vector without packed math 78.6 TF/s
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp32_vector_unpacked": 78.6e12}
-# HBM bytes per launch of the mixed kernel at this workload, from two rocprofv3 PMC passes
-# (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration (factors 2.0 and 1.0); the commands and
-# raw figures are in profiles/af_kernel_mixed_hs_l8_k10_b4096.txt.  Re-measured when the kernel changes.
-MEASURED_TRAFFIC_BYTES = 83691 * 1024
+# HBM bytes per launch of each mixed kernel at this workload (4096 distributions, batch-fastest), from
+# two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration; the commands and
+# raw figures are in the named profile.  A kernel without an entry here reports traffic as null.
+PROFILED_TRAFFIC = {
+    "valu": (83691 * 1024, "profiles/af_kernel_mixed_hs_l8_k10_b4096.txt"),
+}
 
 
 def work_model(lab, ic, K):
@@ -31,6 +35,24 @@ def work_model(lab, ic, K):
     n_g = lab.shape[0]
     S = len(set(zip(np.asarray(ic).astype(int).tolist(), np.asarray(lab)[:, 0].astype(int).tolist())))
     return 2.0 * (n_g * K * K + S * K ** 3), n_g, S
+
+
+def executed_flops(st, K, matrix_cores):
+    """FLOP per evaluation the mixed kernel executes on the symmetrized tables (plan statistics st):
+    stage 1 (h sums over the kept nonzeros, one outer product per q2 group) and the FP64 radial stage of
+    the precise slices on the VALU; the radial stage of the other slices on the VALU (full blocks K^3,
+    symmetric blocks K^2 (K+1)/2) or on the matrix cores (16x16x4 steps of 4 values of (k2,k3), rows k1
+    padded to 16; executed, not useful, FLOPs)."""
+    plain = st["sym_S"] - st["sym_symmetric"] - st["sym_precise"]
+    valu = 2.0 * (st["sym_N_G"] * K + st["sym_groups"] * K * K + st["sym_precise"] * K ** 3)
+    if not matrix_cores:
+        valu += 2.0 * (plain * K ** 3 + st["sym_symmetric"] * K * K * (K + 1) / 2)
+        return valu, 0.0
+    kc = K if K <= 10 else (K + 1) // 2
+    steps_full = (K * kc + 3) // 4 * ((K + kc - 1) // kc)
+    steps_sym = (K * (K + 1) // 2 + 3) // 4 if kc == K else steps_full
+    mfma = (plain * steps_full + st["sym_symmetric"] * steps_sym) * 4 * 16 * 2.0
+    return valu, mfma
 
 
 def cpu_baseline(lab, val, ic, R, nq, K, f, budget_s):
@@ -152,11 +174,15 @@ def main():
 
     value = B / tmix
     achieved = F * B / kmix
-    # composite bound of the work model: the Phi stage (sparse gather, VALU) at the FP32 vector rate
-    # without packed math, the R.Phi stage (dense) at the FP32 matrix rate
-    F_phi = 2.0 * n_g * K * K
-    F_rphi = 2.0 * S * K ** 3
-    t_comp = (F_phi / PEAK["fp32_vector_unpacked"] + F_rphi / PEAK["fp32"]) * B
+    matrix_cores = tiles["mixed"][2] == 1
+    kind = "mfma" if matrix_cores else "valu"
+    st = plan.stats()
+    e_valu, e_mfma = executed_flops(st, K, matrix_cores)
+    # bound on the executed work: VALU FLOPs at the FP32 vector rate without packed math plus MFMA FLOPs
+    # at the FP32 matrix rate, added (on gfx950 FP32 MFMAs and FP32 VALU work of other waves on the same
+    # SIMD do not overlap: measured, 4.65 ms for both vs 4.71 / 4.33 ms for each alone)
+    t_exec = (e_valu / PEAK["fp32_vector_unpacked"] + e_mfma / PEAK["fp32"]) * B
+    traffic, traffic_src = PROFILED_TRAFFIC.get(kind, (None, None)) if B == 4096 else (None, None)
     out = {
         "metric": METRIC,
         "value": value,
@@ -178,16 +204,24 @@ def main():
         "fp64_evals_per_s": B / t64,
         "fp64_sym_evals_per_s": B / tsym,
         "speedup_vs_fp64": t64 / tmix,
+        "speedup_vs_fp64_sym": tsym / tmix,
         "kernel_ms": {"mixed": kmix * 1e3, "fp64": k64 * 1e3, "fp64_sym": ksym * 1e3},
         "flop_per_eval": F,
         "tiles_kc": tiles,
         "waves_per_workgroup": {prec: plan.waves(prec) for prec in ("fp64", "mixed", "fp64_sym")},
         "row_split": {prec: plan.split(prec) for prec in ("fp64", "mixed", "fp64_sym")},
         "setup_s": {"geometry": t1 - t0, "rtensor_sumfac_gpu": t2 - t1},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp32"], "traffic": MEASURED_TRAFFIC_BYTES if B == 4096 else None,
-                     "fp64_path_frac": F * B / k64 / PEAK["fp64"],
-                     "composite_bound_ms": t_comp * 1e3, "composite_frac": t_comp / kmix},
+        "roofline": {"bound": "valu" if kind == "valu" else "mfma", "achieved": achieved / 1e12,
+                     "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
+                     "achieved_basis": "work-model FLOPs 2(N_G K^2 + S K^3) per eval over the kernel's HIP-event "
+                                       "time; the kernel executes fewer (executed)",
+                     "kernel": "matrix-core radial stage" if matrix_cores else "VALU",
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "executed": {"valu_flop_per_eval": e_valu, "mfma_flop_per_eval": e_mfma,
+                                  "tflops": (e_valu + e_mfma) * B / kmix / 1e12,
+                                  "valu_frac_of_unpacked_fp32": e_valu * B / kmix / PEAK["fp32_vector_unpacked"],
+                                  "bound_ms": t_exec * 1e3, "frac_of_bound": t_exec / kmix},
+                     "fp64_path_frac": F * B / k64 / PEAK["fp64"]},
         "cpu_baseline": None,
     }
     if max_rel > 1e-6:

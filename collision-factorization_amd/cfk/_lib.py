@@ -62,6 +62,8 @@ SIGNATURES = {
     "cfk_sumfac_last_chunks": ([ctypes.POINTER(_i64)], ctypes.c_int),
     "cfk_h5_read_f64": ([ctypes.c_char_p, ctypes.c_char_p, _dp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i32),
                          ctypes.POINTER(_i64)], ctypes.c_int),
+    "cfk_mat_write_f64": ([ctypes.c_char_p, _i32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_i32),
+                           ctypes.POINTER(_i64), ctypes.POINTER(_dp)], ctypes.c_int),
     "cfk_sumfac_last_timing": ([ctypes.POINTER(_i32), ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cfk_sumfac_grid_sizes": ([_i32, _i32, _i32, _i32, ctypes.POINTER(_i32)], ctypes.c_int),
@@ -86,6 +88,9 @@ SIGNATURES = {
     "cfk_multi_set_graph": ([_vp, _i32], ctypes.c_int),
     "cfk_multi_execute": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "cfk_dense_execute": ([_vp, _vp, _vp, _i64, _i64, _vp], ctypes.c_int),
+    "cfk_basis_evaluate": ([_vp, _vp, _i64, _i32, _i32, _vp], ctypes.c_int),
+    "cfk_moments": ([_vp, _vp, _i64, _i32, _i32, _i32, _vp], ctypes.c_int),
+    "cfk_moment_weights": ([_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cfk_conservation_project": ([_vp, _i64, _i32, _i32, _i32, _vp], ctypes.c_int),
     "cfk_mlp_create": ([_i32, _i32, _i32, _i32, ctypes.c_uint64, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_mlp_destroy": ([_vp], ctypes.c_int),

@@ -10,6 +10,8 @@
 // against the file size.
 #include <zlib.h>
 
+#include <algorithm>
+#include <cctype>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -370,4 +372,246 @@ extern "C" int cfk_h5_read_f64(const char *path, const char *dataset, double *ou
                                int32_t *rank, int64_t dims[8]) {
     if (!path || !dataset) return cfk::set_error(CFK_EINVAL, "null path or dataset name");
     return cfk::h5_read(path, dataset, out, capacity, numel, rank, dims);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Writer: float64 arrays as the top-level variables of a MATLAB v7.3 .mat file.
+//
+// Layout (what MATLAB's load reads for a plain double array): a 512-byte user block opening with the
+// 128-byte MAT-file header (116 bytes of text, 8 bytes of subsystem offset, version 0x0200, "IM"),
+// then an HDF5 file whose superblock (version 0, 8-byte offsets and lengths, base address 512) roots a
+// symbol-table group: a local heap with the variable names, one v1 B-tree node whose children are
+// symbol-table nodes of up to 8 entries sorted by name, and one dataset per variable (version-1 object
+// header: dataspace with the MATLAB dimensions reversed, IEEE F64LE datatype, fill-value, contiguous
+// layout, and the string attribute MATLAB_class = "double"), its data the array in column-major order.
+// File addresses are relative to the superblock; the end-of-file address is the file's size.
+// ------------------------------------------------------------------------------------------------
+namespace cfk {
+namespace {
+
+constexpr uint64_t kUndef = ~0ull;
+constexpr int kLeafK = 4, kNodeK = 16;  // symbol-table nodes hold 2*4 entries; a B-tree node 2*16 children
+
+struct Out {
+    std::vector<uint8_t> b;  // the HDF5 part (addresses are offsets into b)
+    size_t pos() const { return b.size(); }
+    void u(uint64_t v, int n) {
+        for (int i = 0; i < n; ++i) b.push_back((uint8_t)(v >> (8 * i)));
+    }
+    void bytes(const void *p, size_t n) { b.insert(b.end(), (const uint8_t *)p, (const uint8_t *)p + n); }
+    void zeros(size_t n) { b.insert(b.end(), n, 0); }
+    void align8() {
+        while (b.size() % 8) b.push_back(0);
+    }
+    void put(size_t off, uint64_t v, int n) {
+        for (int i = 0; i < n; ++i) b[off + i] = (uint8_t)(v >> (8 * i));
+    }
+};
+
+// Header message (version-1 object header): type, size (a multiple of 8), flags, 3 reserved bytes.
+void msg_header(Out &o, int type, size_t size) {
+    o.u((uint64_t)type, 2);
+    o.u((uint64_t)size, 2);
+    o.u(0, 4);
+}
+
+void f64_datatype(Out &o) {  // 20 bytes, padded to 24 by the caller's message size
+    o.u(0x11, 1);                   // version 1, class 1 (floating point)
+    o.u(0x20, 1);                   // little endian, implied mantissa MSB
+    o.u(63, 1);                     // sign bit position
+    o.u(0, 1);
+    o.u(8, 4);                      // size in bytes
+    o.u(0, 2);                      // bit offset
+    o.u(64, 2);                     // bit precision
+    o.u(52, 1);                     // exponent location
+    o.u(11, 1);                     // exponent size
+    o.u(0, 1);                      // mantissa location
+    o.u(52, 1);                     // mantissa size
+    o.u(1023, 4);                   // exponent bias
+}
+
+struct Var {
+    std::string name;
+    std::vector<int64_t> dims;  // MATLAB order
+    const double *data;
+    uint64_t numel;
+    size_t heap_off = 0, ohdr = 0;
+};
+
+}  // namespace
+}  // namespace cfk
+
+extern "C" int cfk_mat_write_f64(const char *path, int32_t n_vars, const char *const *names, const int32_t *ranks,
+                                 const int64_t *dims, const double *const *data) {
+    using namespace cfk;
+    if (!path || n_vars < 1 || !names || !ranks || !dims || !data) return set_error(CFK_EINVAL, "null or empty argument");
+    if (n_vars > 2 * kLeafK * 2 * kNodeK)
+        return set_error(CFK_EUNSUP, "%d variables (at most %d)", n_vars, 2 * kLeafK * 2 * kNodeK);
+    std::vector<Var> vars((size_t)n_vars);
+    size_t dpos = 0;
+    for (int i = 0; i < n_vars; ++i) {
+        Var &v = vars[(size_t)i];
+        if (!names[i] || !data[i]) return set_error(CFK_EINVAL, "variable %d: null name or data", i);
+        v.name = names[i];
+        bool ok = !v.name.empty() && v.name.size() <= 63 && std::isalpha((unsigned char)v.name[0]);
+        for (char c : v.name) ok = ok && (std::isalnum((unsigned char)c) || c == '_');
+        if (!ok) return set_error(CFK_EINVAL, "variable %d: '%s' is not a MATLAB identifier", i, v.name.c_str());
+        if (ranks[i] < 1 || ranks[i] > 8) return set_error(CFK_EINVAL, "variable %s: rank %d", v.name.c_str(), ranks[i]);
+        v.numel = 1;
+        for (int r = 0; r < ranks[i]; ++r) {
+            const int64_t d = dims[dpos++];
+            if (d < 1) return set_error(CFK_EINVAL, "variable %s: dimension %lld", v.name.c_str(), (long long)d);
+            v.dims.push_back(d);
+            v.numel *= (uint64_t)d;
+        }
+        if (v.dims.size() == 1) v.dims.push_back(1);  // MATLAB arrays have at least two dimensions
+        v.data = data[i];
+    }
+    std::vector<int> order((size_t)n_vars);
+    for (int i = 0; i < n_vars; ++i) order[(size_t)i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return vars[(size_t)a].name < vars[(size_t)b].name; });
+    for (size_t i = 1; i < order.size(); ++i)
+        if (vars[(size_t)order[i]].name == vars[(size_t)order[i - 1]].name)
+            return set_error(CFK_EINVAL, "variable '%s' given twice", vars[(size_t)order[i]].name.c_str());
+
+    Out o;
+    // superblock (version 0); addresses patched below
+    const uint8_t sig[8] = {0x89, 'H', 'D', 'F', '\r', '\n', 0x1a, '\n'};
+    o.bytes(sig, 8);
+    o.u(0, 1); o.u(0, 1); o.u(0, 1); o.u(0, 1); o.u(0, 1);  // versions: superblock, free space, root entry, -, shared msgs
+    o.u(8, 1); o.u(8, 1); o.u(0, 1);                        // size of offsets, of lengths, reserved
+    o.u(kLeafK, 2); o.u(kNodeK, 2); o.u(0, 4);              // group leaf / internal node K, flags
+    o.u(512, 8);                                            // base address (absolute)
+    o.u(kUndef, 8);                                         // free-space info
+    const size_t eof_at = o.pos();
+    o.u(0, 8);                                              // end of file (patched)
+    o.u(kUndef, 8);                                         // driver info
+    // root group symbol-table entry
+    o.u(0, 8);                                              // link name offset ("")
+    const size_t root_ohdr_at = o.pos();
+    o.u(0, 8);                                              // root object header (patched)
+    o.u(1, 4); o.u(0, 4);                                   // cache type 1: scratch holds B-tree and heap
+    const size_t root_scratch = o.pos();
+    o.u(0, 8); o.u(0, 8);
+
+    // local heap: "" then each name, null-terminated, padded to 8
+    std::vector<uint8_t> heap(8, 0);
+    for (Var &v : vars) {
+        v.heap_off = heap.size();
+        heap.insert(heap.end(), v.name.begin(), v.name.end());
+        heap.push_back(0);
+        while (heap.size() % 8) heap.push_back(0);
+    }
+    const size_t heap_hdr = o.pos();
+    o.bytes("HEAP", 4); o.u(0, 1); o.zeros(3);
+    o.u(heap.size(), 8); o.u(1, 8);  // data segment size; free list: none (HDF5 stores 1 for that)
+    const size_t heap_data_at = o.pos();
+    o.u(0, 8);
+    o.put(heap_data_at, o.pos(), 8);
+    o.bytes(heap.data(), heap.size());
+    o.align8();
+
+    // symbol-table nodes of up to 2 kLeafK entries, in name order
+    const int per = 2 * kLeafK;
+    const int n_snod = (n_vars + per - 1) / per;
+    std::vector<size_t> snod_at((size_t)n_snod), entry_ohdr;  // patch positions of the object header addresses
+    for (int s = 0; s < n_snod; ++s) {
+        snod_at[(size_t)s] = o.pos();
+        const int i0 = s * per, i1 = std::min(n_vars, i0 + per);
+        o.bytes("SNOD", 4); o.u(1, 1); o.u(0, 1); o.u((uint64_t)(i1 - i0), 2);
+        for (int e = 0; e < per; ++e) {
+            if (i0 + e < i1) {
+                const Var &v = vars[(size_t)order[(size_t)(i0 + e)]];
+                o.u(v.heap_off, 8);
+                entry_ohdr.push_back(o.pos());
+                o.u(0, 8);
+            } else {
+                o.u(0, 8); o.u(0, 8);
+            }
+            o.u(0, 4); o.u(0, 4); o.zeros(16);  // cache type 0, reserved, scratch
+        }
+    }
+    // one B-tree node (group nodes, level 0) over the symbol-table nodes; keys are heap offsets of the
+    // last name of each child (key 0: the empty name)
+    const size_t btree_at = o.pos();
+    o.bytes("TREE", 4); o.u(0, 1); o.u(0, 1); o.u((uint64_t)n_snod, 2);
+    o.u(kUndef, 8); o.u(kUndef, 8);
+    o.u(0, 8);
+    for (int s = 0; s < 2 * kNodeK; ++s) {
+        if (s < n_snod) {
+            const int last = std::min(n_vars, (s + 1) * per) - 1;
+            o.u(snod_at[(size_t)s], 8);
+            o.u(vars[(size_t)order[(size_t)last]].heap_off, 8);
+        } else {
+            o.u(0, 8); o.u(0, 8);
+        }
+    }
+    o.put(root_scratch, btree_at, 8);
+    o.put(root_scratch + 8, heap_hdr, 8);
+
+    // root group object header: one symbol-table message
+    o.align8();
+    o.put(root_ohdr_at, o.pos(), 8);
+    o.u(1, 1); o.u(0, 1); o.u(1, 2); o.u(1, 4); o.u(24, 4); o.u(0, 4);
+    msg_header(o, 0x11, 16);
+    o.u(btree_at, 8); o.u(heap_hdr, 8);
+
+    // datasets
+    std::vector<size_t> layout_addr_at((size_t)n_vars);
+    for (int i = 0; i < n_vars; ++i) {
+        Var &v = vars[(size_t)i];
+        const int r = (int)v.dims.size();
+        o.align8();
+        v.ohdr = o.pos();
+        const size_t ds_size = 8 + 16 * (size_t)r, dt_size = 24, fv_size = 8, ly_size = 24, at_size = 48;
+        o.u(1, 1); o.u(0, 1); o.u(5, 2); o.u(1, 4);
+        o.u(5 * 8 + ds_size + dt_size + fv_size + ly_size + at_size, 4); o.u(0, 4);
+        // dataspace v1: current and maximum dimensions, HDF5 order = MATLAB order reversed
+        msg_header(o, 0x01, ds_size);
+        o.u(1, 1); o.u((uint64_t)r, 1); o.u(1, 1); o.u(0, 1); o.u(0, 4);
+        for (int k = r - 1; k >= 0; --k) o.u((uint64_t)v.dims[(size_t)k], 8);
+        for (int k = r - 1; k >= 0; --k) o.u((uint64_t)v.dims[(size_t)k], 8);
+        msg_header(o, 0x03, dt_size);
+        f64_datatype(o);
+        o.zeros(4);
+        // fill value v2: allocated early, written if set, no value defined
+        msg_header(o, 0x05, fv_size);
+        o.u(2, 1); o.u(1, 1); o.u(2, 1); o.u(0, 1); o.zeros(4);
+        // layout v3: contiguous
+        msg_header(o, 0x08, ly_size);
+        o.u(3, 1); o.u(1, 1);
+        layout_addr_at[(size_t)i] = o.pos();
+        o.u(0, 8); o.u(v.numel * 8, 8); o.zeros(6);
+        // attribute v1 MATLAB_class = "double" (string of 6 bytes, null-terminated padding, scalar)
+        msg_header(o, 0x0C, at_size);
+        o.u(1, 1); o.u(0, 1); o.u(13, 2); o.u(8, 2); o.u(8, 2);
+        o.bytes("MATLAB_class\0\0\0\0", 16);
+        o.u(0x13, 1); o.u(0, 1); o.u(0, 1); o.u(0, 1); o.u(6, 4);  // string, null terminated, ASCII
+        o.u(1, 1); o.u(0, 1); o.u(0, 1); o.u(0, 1); o.u(0, 4);    // scalar dataspace v1
+        o.bytes("double\0\0", 8);
+    }
+    for (size_t i = 0; i < entry_ohdr.size(); ++i) o.put(entry_ohdr[i], vars[(size_t)order[i]].ohdr, 8);
+    for (int i = 0; i < n_vars; ++i) {
+        o.align8();
+        o.put(layout_addr_at[(size_t)i], o.pos(), 8);
+        o.bytes(vars[(size_t)i].data, vars[(size_t)i].numel * 8);
+    }
+    o.put(eof_at, 512 + o.pos(), 8);
+
+    // MAT-file header in the user block
+    char head[512];
+    std::memset(head, 0, sizeof(head));
+    char text[117];
+    std::snprintf(text, sizeof(text), "MATLAB 7.3 MAT-file, Platform: GLNXA64, Created on: libcfk HDF5 schema 1.00 .");
+    std::memset(head, ' ', 116);
+    std::memcpy(head, text, std::strlen(text));
+    head[124] = 0x00;
+    head[125] = 0x02;
+    head[126] = 'I';
+    head[127] = 'M';
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return set_error(CFK_EINVAL, "cannot open %s for writing", path);
+    const bool ok = std::fwrite(head, 1, 512, fp) == 512 && std::fwrite(o.b.data(), 1, o.b.size(), fp) == o.b.size();
+    if (std::fclose(fp) != 0 || !ok) return set_error(CFK_EINVAL, "writing %s failed", path);
+    return CFK_OK;
 }

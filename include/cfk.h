@@ -156,6 +156,16 @@ int cfk_build_rtensor(double *R_out, int32_t K_max, int32_t L, double alpha, con
 int cfk_h5_read_f64(const char *path, const char *dataset, double *out, int64_t capacity, int64_t *numel,
                     int32_t *rank, int64_t dims[8]);
 
+/* Write n_vars float64 arrays as the top-level variables of a MATLAB v7.3 .mat file (an HDF5 file
+ * behind the 512-byte MAT-file header), the format precompute_collision_operator.m:61 saves with
+ * save(..., '-v7.3') and MATLAB's load reads: names[i] distinct MATLAB identifiers, ranks[i] the
+ * number of MATLAB dimensions of variable i (1..8), dims those dimensions for all variables back to back,
+ * data[i] the column-major values.  Stored uncompressed and contiguous with MATLAB_class "double";
+ * at most 256 variables; cfk_h5_read_f64 reads them back.  Plain variables, not MATLAB class objects:
+ * the reference saves its TensorObj, this writes the arrays a MATLAB host rebuilds it from.  Host only. */
+int cfk_mat_write_f64(const char *path, int32_t n_vars, const char *const *names, const int32_t *ranks,
+                      const int64_t *dims, const double *const *data);
+
 /* ---------------------------------------------------------------------------------------
  * Device-resident plans: tables uploaded once, many batched executions.
  * ------------------------------------------------------------------------------------- */
@@ -369,6 +379,31 @@ typedef struct cfk_relax_stats {
  * not checked).  Waits for the stream. */
 int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, int32_t n_steps, int32_t mode, double tol,
                         int32_t sample_stride, double *residual_hist, cfk_relax_stats *stats, void *stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Basis evaluation and moment diagnostics.
+ * ------------------------------------------------------------------------------------- */
+
+/* Psi = Basis.evaluate(v) (src/SpectralBasis.m:25-58): the basis functions
+ * phi_{klm}(v) = N_kl L_k^{(l+1/2)}(|v|^2) |v|^l Y_lm(v/|v|) at n_pts velocities, K radial modes (K_max+1),
+ * degrees 0..L (L <= 16).  v_dev is MATLAB's n_pts x 3 array (column-major: x block, y block, z block);
+ * Psi_dev is n_pts x (K (L+1)^2) column-major, column k*N_Q + q.  Device pointers; waits for the
+ * stream. */
+int cfk_basis_evaluate(double *Psi_dev, const double *v_dev, int64_t n_pts, int32_t K, int32_t L, void *stream);
+
+/* Moments of f(v) = e^{-|v|^2} sum c phi(v) for B coefficient vectors (N_Q x K each, layout
+ * CFK_LAYOUT_*; N_Q >= 9, K >= 2): out_dev [B][CFK_N_MOMENTS] = (n, u_x, u_y, u_z, T, P_xx, P_yy, P_zz,
+ * P_xy, P_xz, P_yz) with n = int f, n u = int v f, P_ij = int (v_i - u_i)(v_j - u_j) f and
+ * T = 2 (P_xx + P_yy + P_zz) / (3 n) (a Maxwellian n (pi T)^{-3/2} exp(-|v-u|^2/T) has temperature T).
+ * Mass, momentum and energy are the invariant coefficients the conservation projection keeps
+ * (tutorial_collisions_spectral_properties.m:138-144); the traceless stress is the l = 2 part that
+ * relaxes in tutorial_stress_relaxation.m.  Asynchronous. */
+#define CFK_N_MOMENTS 11
+int cfk_moments(double *out_dev, const double *c_dev, int64_t B, int32_t N_Q, int32_t K, int32_t layout, void *stream);
+/* The ten linear functionals behind cfk_moments (int f, int v_d f, int v_i v_j f for xx, yy, zz, xy, xz,
+ * yz): functional j is sum_t w[4j+t] c[idx[4j+t]] over the terms with idx >= 0 (index k*N_Q + q).
+ * Host only. */
+int cfk_moment_weights(int32_t N_Q, int32_t K, int32_t idx[40], double w[40]);
 
 /* dense variant on device: Q_dev[b] += C f_b f_b for B distributions of length N (layout natural). */
 int cfk_dense_execute(double *Q_dev, const double *f_dev, const double *C_dev, int64_t N, int64_t B, void *stream);
