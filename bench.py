@@ -26,7 +26,8 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp32_vector_unpacked": 78.6e12}
 # two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration; the commands and
 # raw figures are in the named profile.  A kernel without an entry here reports traffic as null.
 PROFILED_TRAFFIC = {
-    "valu": (83691 * 1024, "profiles/af_kernel_mixed_hs_l8_k10_b4096.txt"),
+    "valu": (80620 * 1024, "profiles/af_kernel_mixed_hs_l8_k10_b4096.txt"),
+    "mfma": (80640 * 1024, "profiles/af_mx_kernel_mixed_hs_l8_k10_b4096.txt"),
 }
 
 
@@ -90,6 +91,10 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-kernel", default="auto", choices=["auto", "valu", "mfma"],
+                    help="hs-l8-k10: the mixed-precision kernel -- auto: the autotuner's pick; valu / mfma: the VALU or "
+                         "the matrix-core kernel with the row split and two batch tiles per workgroup (no autotuning "
+                         "of the mixed path), e.g. to profile one kernel")
     ap.add_argument("--no-compare", action="store_true",
                     help="rtensor: skip the per-channel kernel run kept beside the matrix-core one")
     ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate", "copy"],
@@ -129,6 +134,11 @@ def main():
     t2 = time.perf_counter()
     plan = cfk.CollisionPlan(ls, vs, ics, R, nq, K)
     tiles = plan.autotune(B)
+    if args.mixed_kernel != "auto":
+        plan.set_tile("mixed", 0, 1, 1 if args.mixed_kernel == "mfma" else 0)
+        plan.set_split("mixed", 1)
+        plan.set_waves("mixed", 2)
+        tiles["mixed"] = (K, 1, 1 if args.mixed_kernel == "mfma" else 0)
     F, n_g, S = work_model(ls, ics, K)
 
     f_host = perturbed_maxwellians(L, K, B, seed=0)
