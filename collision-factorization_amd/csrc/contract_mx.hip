@@ -25,7 +25,10 @@
 // (l2 == l3) Phi is folded onto k2 <= k3 first and only those p are stepped (14 instead of 25 steps at
 // K = 10).
 //
-// The matrix unit executes stage 2 of one wave while the vector unit runs stage 1 of the others.
+// On gfx950 FP32 MFMAs of one wave and FP32 VALU work of another wave on the same SIMD do not run
+// concurrently (measured with a register-only probe), so the gain over the VALU kernel is the smaller
+// instruction count of the radial stage, not overlap; at K = 10 the two kernels are equally fast,
+// from K = 12 this one is faster (the 16-row tiles are fuller and the VALU kernel needs half chunks).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
