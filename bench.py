@@ -326,7 +326,9 @@ def sumfac_flops(L, NK, sizes, S):
 
 def bench_rtensor(args, cfk):
     """GPU tensor generation at L=12, K=12 radial modes, 48 nodes per axis, gamma = 0 and 1, on the
-    matrix-core quadrature; the per-channel kernel (reference summation order) timed beside it."""
+    default kernel (exact-sphere: this rule is exact for the gain integrand); the matrix-core
+    quadrature and (unless --no-compare) the per-channel kernel (reference summation order) timed
+    beside it on gamma = 1."""
     L, K_max = 12, 11
     NK = K_max + 1
     sizes = (48,) * 7
@@ -334,18 +336,25 @@ def bench_rtensor(args, cfk):
     S = len({(int(t), int(q)) for t, q in zip(geo[2], geo[0][:, 0])})
     res = {}
     timing = {}
-    runs = [("matrix_core", (0.0, 1.0))] + ([] if args.no_compare else [("per_channel", (1.0,))])
-    for kind, gammas in runs:
-        cfk.set_sumfac_kernel(kind)
-        for gamma in gammas:
-            t0 = time.perf_counter()
-            R = cfk.build_rtensor(K_max, L, gamma, sizes, geometry=geo)
-            key = "%s_gamma%d" % (kind, int(gamma))
-            res[key + "_s"] = time.perf_counter() - t0
-            res[key + "_max_abs_R"] = float(np.max(np.abs(R)))
-            timing[key] = cfk.sumfac_last_timing()
-            print("%s: %.3f s, kernels %s" % (key, res[key + "_s"], timing[key]), file=sys.stderr, flush=True)
-    cfk.set_sumfac_kernel("matrix_core")
+    Rs = {}
+    runs = [("auto", (0.0, 1.0)), ("matrix_core", (1.0,))] + ([] if args.no_compare else [("per_channel", (1.0,))])
+    try:
+        for kind, gammas in runs:
+            cfk.set_sumfac_kernel(kind)
+            for gamma in gammas:
+                t0 = time.perf_counter()
+                R = cfk.build_rtensor(K_max, L, gamma, sizes, geometry=geo)
+                key = "%s_gamma%d" % (kind, int(gamma))
+                res[key + "_s"] = time.perf_counter() - t0
+                res[key + "_max_abs_R"] = float(np.max(np.abs(R)))
+                timing[key] = cfk.sumfac_last_timing()
+                if gamma == 1.0:
+                    Rs[kind] = R
+                print("%s: %.3f s, kernels %s" % (key, res[key + "_s"], timing[key]), file=sys.stderr, flush=True)
+    finally:
+        cfk.set_sumfac_kernel("auto")
+    scale = float(np.max(np.abs(Rs["matrix_core"])))
+    diffs = {k: float(np.max(np.abs(v - Rs["matrix_core"])) / scale) for k, v in Rs.items() if k != "matrix_core"}
     cpu = None
     if not args.no_cpu:
         # the oracle's sumfac (plain-C restatement of compute_rtensor_sumfac_mex, OpenMP over the
@@ -363,21 +372,25 @@ def bench_rtensor(args, cfk):
                          "MEX, OpenMP over quadrature units as the reference)" % dt}
     f_units, f_fold = sumfac_flops(L, NK, sizes, S)
     tm = timing["matrix_core_gamma1"]
-    achieved = f_units / (tm["units_ms"] * 1e-3) / 1e12
+    ts = timing["auto_gamma1"]
     gain_points = 48 * (48 * 48 + 48 * 48) * 48 * 48
-    out = {"metric": "tensor generation seconds, L=12, K=12, 48 nodes per axis", "value": res["matrix_core_gamma1_s"],
+    out = {"metric": "tensor generation seconds, L=12, K=12, 48 nodes per axis", "value": res["auto_gamma1_s"],
            "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "compute_rtensor_sumfac, L=12, K=12, 48^7, matrix-core quadrature",
-                      "gain_points": gain_points, "slots": S},
-           "gain_points_per_s": gain_points / res["matrix_core_gamma1_s"],
+           "config": {"workload": "compute_rtensor_sumfac, L=12, K=12, 48^7, gamma 1, default kernel (%s)"
+                                  % ts["kernel"], "gain_points": gain_points, "slots": S},
            "kernel_ms": timing,
-           "speedup_vs_per_channel": (res["per_channel_gamma1_s"] / res["matrix_core_gamma1_s"]
+           "max_rel_diff_vs_matrix_core": diffs,
+           "speedup_vs_matrix_core": res["matrix_core_gamma1_s"] / res["auto_gamma1_s"],
+           "speedup_vs_per_channel": (res["per_channel_gamma1_s"] / res["auto_gamma1_s"]
                                       if "per_channel_gamma1_s" in res else None),
            "cpu_baseline": cpu,
-           "roofline": {"bound": "mfma", "kernel": "sumfac_units_mf_kernel", "achieved": achieved, "peak": 78.6,
-                        "unit": "TFLOP/s", "frac": achieved / 78.6, "flop_units": f_units, "flop_fold": f_fold,
-                        "fold_tflops": f_fold / (tm["fold_ms"] * 1e-3) / 1e12,
-                        "traffic": None},
+           "matrix_core_roofline": {"bound": "mfma", "kernel": "sumfac_units_mf_kernel",
+                                    "achieved": f_units / (tm["units_ms"] * 1e-3) / 1e12, "peak": 78.6,
+                                    "unit": "TFLOP/s", "frac": f_units / (tm["units_ms"] * 1e-3) / 1e12 / 78.6,
+                                    "flop_units": f_units},
+           "fold": {"kernel": "sumfac_fold_mf_kernel", "flop": f_fold,
+                    "tflops": f_fold / (ts["fold_ms"] * 1e-3) / 1e12,
+                    "frac": f_fold / (ts["fold_ms"] * 1e-3) / 1e12 / 78.6},
            **res}
     print(json.dumps(out))
 

@@ -54,15 +54,18 @@ def sumfac_grid_sizes(K_max, L, radial_pad=40, angular_pad=0):
     return tuple(int(v) for v in s)
 
 
-SUMFAC_KERNELS = {"matrix_core": 0, "per_channel": 1}
+SUMFAC_KERNELS = {"matrix_core": 0, "per_channel": 1, "exact_sphere": 2, "auto": 3}
 
 
 def set_sumfac_kernel(kind):
     """Select the GPU quadrature kernel for later sumfac calls (process-wide).
 
-    "matrix_core" (default): gain-point sums once per SH degree on FP64 matrix cores, then the
-    P_gain slot contraction; "per_channel": one lane per t_chan in the reference's summation order
-    (compute_rtensor_sumfac_mex.cpp:219-234).  Both FP64; they agree to rounding."""
+    "auto" (default): "exact_sphere" where the (chi, eps) rule is exact for the gain integrand, else
+    "matrix_core".  "exact_sphere": the gain sum of each unit step as the exact surface integral
+    (SH at one direction per side and 1-D Gauss-Legendre integrals); raises where the rule is not
+    exact.  "matrix_core": gain-point sums once per SH degree on FP64 matrix cores, then the P_gain
+    slot contraction; "per_channel": one lane per t_chan in the reference's summation order
+    (compute_rtensor_sumfac_mex.cpp:219-234).  All FP64; they agree to rounding."""
     check(lib().cfk_sumfac_set_kernel(SUMFAC_KERNELS[kind]))
 
 

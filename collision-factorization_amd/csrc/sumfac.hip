@@ -1002,6 +1002,235 @@ __global__ __launch_bounds__(64) void sumfac_fold_mf_kernel(int NL, int L, const
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Exact-sphere variant (the default where it applies).
+//
+// For one (unit, inner t, side) the gain points are v'(omega) = c + rho R omega with c = U / 2,
+// rho = |u| / 2, R the rotation taking z to the relative velocity's direction and omega on the unit
+// sphere, sampled by the product rule (mu_chi, W_chi) x (eps, W_eps)
+// (compute_rtensor_sumfac_mex.cpp:186-213).  A slot's integrand
+//     Rad(|v'|, l, k) Y_q(v'/|v'|) = RadN SH_Norm L_k^{(l+1/2)}(|v'|^2) S_q(v'),
+// S_q(v) = |v|^l P_l^|m|(v_z/|v|) {cos, sin}(|m| phi) a harmonic polynomial of degree l, agrees on
+// the sphere with a polynomial of degree l + k in omega (|v'|^2 = |c|^2 + rho^2 + 2 rho c . R omega
+// is linear there).  When the rule integrates every polynomial of degree L + N_K - 1 over the
+// sphere exactly (checked on the host; the grid sizes of src/GeneralCollisionTensor.m:63-64 are
+// larger), the gain sum is the surface integral.  On each
+// circle omega . c_hat = t the factor L_k(|v'|^2) is constant and the mean of S_q over the circle is
+// S_q(c_hat) Z_l(t), with Z_l = |v'|^l P_l(cos angle(v', c)) the zonal solid harmonic about c (the
+// mean over rotations about c_hat of a degree-l harmonic is its zonal part), so
+//     sum_p w_p Rad Y = 2 pi B Jac_t RadN SH_Norm S_q(c_hat) int_{-1}^{1} L_k(s(t)) Z_l(t) dt,
+//     s(t) = |c + rho t c_hat|^2 + rho^2 (1 - t^2),
+// a polynomial of degree k + l in t, integrated exactly by Gauss-Legendre with ceil((L + N_K) / 2)
+// nodes.  A unit step then needs the SH at two directions and 2 (L+1) N_K one-dimensional integrals
+// instead of 2 N_chi N_eps basis evaluations, and the t_chan sums factor over k:
+//     Phi_gain[k][tc] += G[l1][k] * sum_{s in slots(tc)} P(s) SH_Norm S_{q_s}(c_hat).
+// The result equals the reference's quadrature up to rounding.
+//
+// One workgroup per unit; thread tc owns t_chan tc (both sides, all k) in registers across the
+// inner-t loop.  Per step: the integrals on 16-lane groups (one lane per node, a group per (side, l),
+// summed across the group by lane shuffles), the SH at c_hat on one lane per (side, m), then the
+// slot sums.  The geometry of all inner steps is formed first, one thread per step.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int SP_NODES = 16;  // lanes per (side, l) group; nodes ceil((L + N_K) / 2) <= 14
+
+struct SphereRule {
+    double t[SP_NODES], w[SP_NODES];  // Gauss-Legendre on [-1, 1], padded with zero weights
+};
+constexpr int SP_MAX_THREADS = 640;  // sphere_threads at L <= 12
+
+// Threads the kernel needs at degree L with NL t_chans: the integral groups, the SH lanes, the
+// t_chan lanes.
+inline int sphere_threads(int L, int NL) {
+    const int ev = 2 * (L + 1) * SP_NODES + 2 * (L + 1);
+    const int n = std::max(ev, NL);
+    return (n + 63) / 64 * 64;
+}
+
+template <int NK>
+__global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(SumfacDev d,
+                                                                             const SphereRule *__restrict__ rule,
+                                                                             int x0, int nx,
+                                                                   double *__restrict__ net,
+                                                                   double *__restrict__ rad) {
+    __shared__ double sY[2][(MAXL + 1) * (MAXL + 1)];
+    __shared__ double sGk[2][(MAXL + 1) * NK];
+    __shared__ double sRadU[2][(MAXL + 1) * MAXK];
+    __shared__ TGeom sG[MAXG];  // geometry of every inner step of the unit
+
+    const int L = d.L, NL = d.NL;
+    const int Ux = d.Nu1 + d.Ny2 * d.Nt2;
+    const int b = blockIdx.x;
+    int xl, r;
+    if (b < nx * d.Nu1) {
+        xl = b / d.Nu1;
+        r = b % d.Nu1;
+    } else {
+        const int bb = b - nx * d.Nu1;
+        xl = bb / (d.Ny2 * d.Nt2);
+        r = d.Nu1 + bb % (d.Ny2 * d.Nt2);
+    }
+    const int64_t unit = (int64_t)xl * Ux + r;
+    const int xi = x0 + xl;
+    const bool patch1 = r < d.Nu1;
+    const int tid = threadIdx.x;
+    const double x_i = d.x[xi];
+
+    double v_large, v_small, Jac_unit;
+    int u_idx = 0, y_idx = 0, t2_idx = 0;
+    if (patch1) {
+        u_idx = r;
+        const double u = d.u1[u_idx];
+        v_large = sqrt(x_i / 2.0) * (1.0 + u);
+        v_small = sqrt(x_i / 2.0) * (1.0 - u);
+        Jac_unit = 0.5 * d.Wx[xi] * d.Wu1[u_idx] * u * exp(-x_i * u * u) * (v_large * v_large * v_small * v_small);
+    } else {
+        y_idx = (r - d.Nu1) / d.Nt2;
+        t2_idx = (r - d.Nu1) % d.Nt2;
+        const double y = d.y2[y_idx];
+        Jac_unit = 0.5 * d.Wx[xi] * d.Wy2[y_idx] * y * (4.0 * y);
+        const double u = y * d.t2[t2_idx];
+        v_large = sqrt(x_i / 2.0) * (1.0 + u);
+        v_small = sqrt(x_i / 2.0) * (1.0 - u);
+    }
+    // geometry of inner step it (as the matrix-core kernel)
+    auto tgeom = [&](int it, TGeom &g) {
+        if (patch1) {
+            const double u = d.u1[u_idx], t = d.t1[it];
+            const double y = u * t;
+            const double Jac_t = d.Wt1[it] * (4.0 * u * t);
+            const double c_beta = 1.0 - 2.0 * y * y;
+            const double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, EPS_SAFE));
+            const double u_mag = sqrt(2.0 * x_i) * u * sqrt(1.0 + (1.0 - u * u) * t * t);
+            make_tgeom(g, x_i, v_large, v_small, c_beta, s_beta, u_mag, Jac_t, d.alpha);
+        } else {
+            const double y = d.y2[y_idx], t = d.t2[t2_idx];
+            const double u = y * t;
+            const double c_beta = 1.0 - 2.0 * y * y;
+            const double s_beta = sqrt(fmax(1.0 - c_beta * c_beta, EPS_SAFE));
+            const double Jac_t = d.Wt2[t2_idx] * exp(-x_i * u * u) * (v_large * v_large * v_small * v_small);
+            const double u_mag = sqrt(2.0 * x_i) * y * sqrt(t * t + 1.0 - y * y * t * t);
+            make_tgeom(g, x_i, v_large, v_small, c_beta, s_beta, u_mag, Jac_t, d.alpha);
+        }
+    };
+    if (tid < 2 * (L + 1)) {
+        const int side = tid / (L + 1), l = tid % (L + 1);
+        double tmp[NK];
+        radial_all_k<NK>(l, side ? v_small : v_large, d.radnorm, tmp);
+#pragma unroll
+        for (int k = 0; k < NK; ++k) sRadU[side][l * MAXK + k] = tmp[k];
+    }
+    const int n_inner_t = patch1 ? d.Nt1 : 1;
+    for (int it = tid; it < n_inner_t; it += blockDim.x) tgeom(it, sG[it]);
+
+    // this thread's t_chan
+    const bool own = tid < NL;
+    const int tc = own ? tid : 0;
+    const int s0 = d.qi_off[tc], s1 = own ? d.qi_off[tc + 1] : s0;
+    const int l1 = d.trip[3 * tc];
+    double acc0[NK], acc1[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        acc0[k] = 0.0;
+        acc1[k] = 0.0;
+    }
+    const int n_ev = 2 * (L + 1) * SP_NODES;
+    const int ns_all = d.qi_off[NL];
+    for (int it = 0; it < n_inner_t; ++it) {
+        __syncthreads();  // geometry written; the previous step's sY / sGk consumed
+        const TGeom &g = sG[it];
+        if (tid < n_ev) {
+            // integral task: group (side, l), lane i = node
+            const int grp = tid / SP_NODES, i = tid % SP_NODES;
+            const int side = grp / (L + 1), l = grp % (L + 1);
+            const double Ux_ = side ? g.U_x2 : g.U_x1, Uz_ = side ? g.U_z2 : g.U_z1;
+            const double cm = 0.5 * sqrt(Ux_ * Ux_ + Uz_ * Uz_);
+            const double rho = 0.5 * (side ? g.u_mag2 : g.u_mag1);
+            const double ti = rule->t[i], wi = rule->w[i];
+            const double z = cm + rho * ti;
+            const double s = z * z + rho * rho * ((1.0 - ti) * (1.0 + ti));
+            double z0 = 1.0, z1 = z;  // Z_{j-1}, Z_j
+            for (int j = 1; j < l; ++j) {
+                const double zn = ((2 * j + 1) * z * z1 - j * s * z0) * c_rinv[j + 1];
+                z0 = z1;
+                z1 = zn;
+            }
+            const double Zl = l == 0 ? 1.0 : z1;
+            const double al = (double)l + 0.5;
+            const double wz = wi * Zl;
+            double P0 = 1.0, P1 = 1.0 + al - s;
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                double Lk;
+                if (k == 0)
+                    Lk = 1.0;
+                else if (k == 1)
+                    Lk = P1;
+                else {
+                    Lk = ((2.0 * (k - 1) + 1.0 + al - s) * P1 - ((k - 1) + al) * P0) * (1.0 / (double)k);
+                    P0 = P1;
+                    P1 = Lk;
+                }
+                double v = wz * Lk;
+                v += __shfl_xor(v, 8);
+                v += __shfl_xor(v, 4);
+                v += __shfl_xor(v, 2);
+                v += __shfl_xor(v, 1);
+                if (i == 0) sGk[side][l * NK + k] = v * (2.0 * M_PI * g.B_val * g.Jac_t) * d.radnorm[k + NK * l];
+            }
+        } else if (tid < n_ev + 2 * (L + 1)) {
+            // SH task: (side, m) at c_hat
+            const int j = tid - n_ev;
+            const int side = j / (L + 1), m = j % (L + 1);
+            const double Ux_ = side ? g.U_x2 : g.U_x1, Uz_ = side ? g.U_z2 : g.U_z1;
+            const double cn = sqrt(Ux_ * Ux_ + Uz_ * Uz_);
+            const double ct = cn > 0.0 ? fmax(fmin(Uz_ / cn, 1.0), -1.0) : 1.0;
+            const double cphi = Ux_ < 0.0 ? -1.0 : 1.0;   // c lies in the x-z plane
+            sh_order_m(L, m, ct, cphi, 0.0, d.shnorm, sY[side]);
+        }
+        __syncthreads();
+        if (own) {
+            const double *Pv =
+                patch1 ? d.Pg1 + ((int64_t)u_idx * d.Nt1 + it) * ns_all : d.Pg2 + (int64_t)y_idx * ns_all;
+            double d0 = 0.0, d1 = 0.0;
+            for (int sl = s0; sl < s1; ++sl) {
+                const double P = Pv[sl];
+                const int q = d.qi[sl];
+                d0 += P * sY[0][q];
+                d1 += P * sY[1][q];
+            }
+            const double *G0 = sGk[0] + l1 * NK, *G1 = sGk[1] + l1 * NK;
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                acc0[k] += d0 * G0[k];
+                acc1[k] += d1 * G1[k];
+            }
+        }
+    }
+    __syncthreads();
+    if (own) {
+        // loss: the radial factor is fixed per unit, so only loss_weight * P_loss is summed over t
+        double ls = 0.0;
+        if (patch1) {
+            for (int it = 0; it < d.Nt1; ++it)
+                ls += sG[it].loss_weight * d.Pl1[((int64_t)u_idx * d.Nt1 + it) * NL + tc];
+        } else {
+            ls = sG[0].loss_weight * d.Pl2[(int64_t)y_idx * NL + tc];
+        }
+        double *o = net + (unit * NL + tc) * 2 * NK;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            o[k] = Jac_unit * (acc0[k] - sRadU[0][l1 * MAXK + k] * ls);
+            o[NK + k] = Jac_unit * (acc1[k] - sRadU[1][l1 * MAXK + k] * ls);
+        }
+    }
+    double *ro = rad + unit * 2 * (L + 1) * NK;
+    for (int i = tid; i < 2 * (L + 1) * NK; i += blockDim.x) {
+        const int side = i / ((L + 1) * NK), rem = i % ((L + 1) * NK);
+        ro[i] = sRadU[side][(rem / NK) * MAXK + rem % NK];
+    }
+}
+
 // Compact P tables on the device (src/GeneralCollisionTensor.m:106-174), from a SumfacTablePlan.
 // One workgroup per quadrature node: the patch-1 (u,t) nodes first ([N_u1][N_t1] order), then the
 // patch-2 y nodes.  The real SH at phi = 0 of the node's polar angle (mu = 1 - 2 y^2, clamped) go to
@@ -1087,7 +1316,71 @@ int launch_units_mf(int mt, size_t dyn, const SumfacDev &d, int xc, int nx, int6
     return launch_units_mf_t<NK, 5>(dyn, d, xc, nx, n_units, net, rad);
 }
 
-int g_sumfac_kernel = CFK_SUMFAC_MATRIX_CORE;
+int g_sumfac_kernel = CFK_SUMFAC_AUTO;
+
+// True when the product rule (mu_chi, W_chi) x (eps, W_eps) integrates every polynomial of degree
+// deg over the unit sphere exactly, to rounding: the chi rule integrates mu^j (j <= deg) over
+// [-1, 1] and the eps rule integrates cos(m eps) and sin(m eps) (m <= deg) over [0, 2 pi).  The
+// surface integrand of a monomial of degree deg in omega splits into such factors.
+bool sphere_rule_exact(const SumfacGrids &G, int deg) {
+    const size_t nc = G.mu_chi.size(), ne = G.eps.size();
+    for (int j = 0; j <= deg; ++j) {
+        double s = 0.0, a = 0.0;
+        for (size_t n = 0; n < nc; ++n) {
+            double p = 1.0;
+            for (int e = 0; e < j; ++e) p *= G.mu_chi[n];
+            s += G.W_chi[n] * p;
+            a += std::fabs(G.W_chi[n] * p);
+        }
+        const double ex = (j & 1) ? 0.0 : 2.0 / (j + 1);
+        if (!(std::fabs(s - ex) <= 1e-12 * std::max(a, ex))) return false;
+    }
+    double wsum = 0.0, wabs = 0.0;
+    for (size_t e = 0; e < ne; ++e) {
+        wsum += G.W_eps[e];
+        wabs += std::fabs(G.W_eps[e]);
+    }
+    if (!(std::fabs(wsum - 2.0 * M_PI) <= 1e-12 * 2.0 * M_PI)) return false;
+    for (int m = 1; m <= deg; ++m) {
+        double c = 0.0, sn = 0.0;
+        for (size_t e = 0; e < ne; ++e) {
+            c += G.W_eps[e] * std::cos(m * G.eps[e]);
+            sn += G.W_eps[e] * std::sin(m * G.eps[e]);
+        }
+        if (!(std::fabs(c) <= 1e-12 * wabs && std::fabs(sn) <= 1e-12 * wabs)) return false;
+    }
+    return true;
+}
+
+// Smallest |u|^2 (squared relative speed of make_tgeom, both sides) over the grid's units and inner
+// steps of x nodes [x_begin, x_end).  Below EPS_SAFE the reference clamps it, and the rotation
+// built from it is no longer orthogonal.
+double min_relative_speed2(const SumfacGrids &G, int x_begin, int x_end) {
+    double mn = INFINITY;
+    auto side2 = [&](double vl, double vs, double c_beta, double s_beta) {
+        const double ux1 = -vs * s_beta, uz1 = vl - vs * c_beta;
+        const double ux2 = -vl * s_beta, uz2 = vs - vl * c_beta;
+        mn = std::min(mn, std::min(ux1 * ux1 + uz1 * uz1, ux2 * ux2 + uz2 * uz2));
+    };
+    for (int xi = x_begin; xi < x_end; ++xi) {
+        const double x = G.x[xi];
+        for (size_t a = 0; a < G.u1.size(); ++a)
+            for (size_t t = 0; t < G.t1.size(); ++t) {
+                const double u = G.u1[a], y = u * G.t1[t];
+                const double c_beta = 1.0 - 2.0 * y * y;
+                const double s_beta = std::sqrt(std::max(1.0 - c_beta * c_beta, EPS_SAFE));
+                side2(std::sqrt(x / 2.0) * (1.0 + u), std::sqrt(x / 2.0) * (1.0 - u), c_beta, s_beta);
+            }
+        for (size_t a = 0; a < G.y2.size(); ++a)
+            for (size_t t = 0; t < G.t2.size(); ++t) {
+                const double y = G.y2[a], u = y * G.t2[t];
+                const double c_beta = 1.0 - 2.0 * y * y;
+                const double s_beta = std::sqrt(std::max(1.0 - c_beta * c_beta, EPS_SAFE));
+                side2(std::sqrt(x / 2.0) * (1.0 + u), std::sqrt(x / 2.0) * (1.0 - u), c_beta, s_beta);
+            }
+    }
+    return mn;
+}
 int g_sumfac_chunk_nodes = 0;  // x nodes per chunk (0: sized by the buffer cap)
 
 // Device time of the last quadrature call (HIP events around each launch on its stream).
@@ -1214,9 +1507,29 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     for (int t = 0; t < NL && slots_on_l1; ++t)
         for (int s = T.qi_off[t]; s < T.qi_off[t + 1]; ++s)
             if (l_of_q0(T.qi[s]) != T.trip[3 * t]) slots_on_l1 = false;
-    const bool mf = g_sumfac_kernel == CFK_SUMFAC_MATRIX_CORE && slots_on_l1;
+    // The exact-sphere form needs the gain rule to be exact for the integrand's degree and every
+    // relative speed above the reference's clamp.
+    const int kind_req = g_sumfac_kernel;
+    bool sphere = false;
+    if ((kind_req == CFK_SUMFAC_AUTO || kind_req == CFK_SUMFAC_EXACT_SPHERE) && slots_on_l1 &&
+        (L + NK + 1) / 2 <= SP_NODES && d.Nt1 <= MAXG)
+        sphere = sphere_rule_exact(G, L + NK - 1) && min_relative_speed2(G, x_begin, x_end) > EPS_SAFE;
+    if (kind_req == CFK_SUMFAC_EXACT_SPHERE && !sphere)
+        return set_error(CFK_EUNSUP,
+                         "sumfac: the exact-sphere kernel needs a (chi, eps) rule exact to degree %d on the sphere, "
+                         "every valid q_i of a t_chan of degree l1 and relative speeds above the clamp",
+                         L + NK - 1);
+    const bool mf = !sphere && (kind_req == CFK_SUMFAC_MATRIX_CORE || kind_req == CFK_SUMFAC_AUTO) && slots_on_l1;
+    DevBuf<double> rule_dev;  // SphereRule on the device
+    if (sphere) {
+        SphereRule rule{};
+        gauss_legendre((L + NK + 1) / 2, -1.0, 1.0, rule.t, rule.w);
+        if ((rc = rule_dev.upload(rule.t, 2 * SP_NODES))) return rc;
+        if (sphere_threads(L, NL) > SP_MAX_THREADS)
+            return set_error(CFK_EUNSUP, "sumfac: %d t_chans exceed the exact-sphere kernel's workgroup", NL);
+    }
     g_last_timing = SumfacTiming{};
-    g_last_timing.kind = mf ? CFK_SUMFAC_MATRIX_CORE : CFK_SUMFAC_PER_CHANNEL;
+    g_last_timing.kind = sphere ? CFK_SUMFAC_EXACT_SPHERE : mf ? CFK_SUMFAC_MATRIX_CORE : CFK_SUMFAC_PER_CHANNEL;
     hipEvent_t ev[3];
     for (auto &e : ev) CFK_HIP(hipEventCreate(&e));
     struct EvGuard {
@@ -1235,7 +1548,7 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     // chunk of x nodes so the per-unit buffers stay below 1 GiB (per-channel kernel) or 4 GiB
     // (matrix-core kernel: more patch-1 units per launch balance the workgroups)
     const int64_t per_unit = (int64_t)NL * 2 * NK + 2 * (L + 1) * NK;
-    const int64_t cap = mf ? ((int64_t)1 << 29) : ((int64_t)1 << 27);
+    const int64_t cap = (mf || sphere) ? ((int64_t)1 << 29) : ((int64_t)1 << 27);
     int64_t xs_per_chunk = std::max<int64_t>(1, cap / (per_unit * Ux));
     if (g_sumfac_chunk_nodes > 0) xs_per_chunk = std::min<int64_t>(xs_per_chunk, g_sumfac_chunk_nodes);
     if (x_end > x_begin) {
@@ -1262,12 +1575,19 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
         const int64_t n_units = (int64_t)nx * Ux;
         g_last_timing.chunks += 1;
         if (n_units > 0x7fffffffLL) return set_error(CFK_EINVAL, "sumfac: too many units per chunk");
-        if (mf) {
+        if (mf || sphere) {
             switch (NK) {
 #define X(KK)                                                                                                    \
     case KK: {                                                                                                   \
         CFK_HIP(hipEventRecord(ev[0], nullptr));                                                                 \
-        if ((rc = launch_units_mf<KK>(mf_tiles_per_wave(L), mf_dyn, d, xc, nx, n_units, b.net.p, b.rad.p)))      \
+        if (sphere) {                                                                                            \
+            hipLaunchKernelGGL(sumfac_units_sphere_kernel<KK>, dim3((unsigned)n_units),                          \
+                               dim3((unsigned)sphere_threads(L, NL)), 0, nullptr, d,                              \
+                               reinterpret_cast<const SphereRule *>(rule_dev.p), xc, nx, b.net.p,                \
+                               b.rad.p);                                                                         \
+            CFK_HIP(hipGetLastError());                                                                          \
+        } else if ((rc = launch_units_mf<KK>(mf_tiles_per_wave(L), mf_dyn, d, xc, nx, n_units, b.net.p,         \
+                                             b.rad.p)))                                                          \
             return rc;                                                                                           \
         CFK_HIP(hipEventRecord(ev[1], nullptr));                                                                 \
         constexpr int nblk = (KK * KK + 15) / 16;                                                                \
@@ -1376,7 +1696,8 @@ extern "C" int cfk_compute_rtensor_sumfac(
 }
 
 extern "C" int cfk_sumfac_set_kernel(int32_t kind) {
-    if (kind != CFK_SUMFAC_MATRIX_CORE && kind != CFK_SUMFAC_PER_CHANNEL)
+    if (kind != CFK_SUMFAC_MATRIX_CORE && kind != CFK_SUMFAC_PER_CHANNEL && kind != CFK_SUMFAC_EXACT_SPHERE &&
+        kind != CFK_SUMFAC_AUTO)
         return set_error(CFK_EINVAL, "unknown sumfac kernel %d", kind);
     g_sumfac_kernel = kind;
     return CFK_OK;

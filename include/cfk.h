@@ -97,14 +97,26 @@ int cfk_compute_rtensor_sumfac(double *R_out, int32_t K_max, int32_t N_L, int32_
                                const double *P_loss_p2, const double *P_gain_p2, int32_t x_begin, int32_t x_end);
 
 /* Kernel for the sumfac quadrature (process-wide; set it before the calls it should govern).
- * CFK_SUMFAC_MATRIX_CORE (default): per inner-t step, the gain-point sum is taken once per SH
- *   degree on FP64 matrix cores (M(l,k,q) = sum_p w Rad(p,l,k) Y(p,q)) and then contracted with
- *   each t_chan's P_gain slots; the unit fold is a matrix-core product over units.
+ * CFK_SUMFAC_AUTO (default): CFK_SUMFAC_EXACT_SPHERE where it applies, else CFK_SUMFAC_MATRIX_CORE
+ *   (else CFK_SUMFAC_PER_CHANNEL for tables whose valid q_i are not all of degree l1).
+ * CFK_SUMFAC_EXACT_SPHERE: when the (chi, eps) product rule integrates polynomials of degree
+ *   L + K_max over the sphere exactly (the degree of the gain integrand there; the reference's own
+ *   grid sizing, GeneralCollisionTensor.m:63-64, and larger Gauss-Legendre x uniform rules qualify),
+ *   the gain sum of one unit step equals its surface
+ *   integral, which reduces by rotational symmetry about the collision's centre of mass to the SH at
+ *   one direction per side and one-dimensional Gauss-Legendre integrals, exact for the polynomial
+ *   integrand; fails with CFK_EUNSUP where the rule is not exact.
+ * CFK_SUMFAC_MATRIX_CORE: per inner-t step, the gain-point sum is taken once per SH degree on FP64
+ *   matrix cores (M(l,k,q) = sum_p w Rad(p,l,k) Y(p,q)) and then contracted with each t_chan's
+ *   P_gain slots.
  * CFK_SUMFAC_PER_CHANNEL: one lane per t_chan sums over the gain points in the reference's order
  *   (compute_rtensor_sumfac_mex.cpp:219-234), on the vector ALUs.
- * Both are FP64 and agree to rounding.  No counterpart in the reference (one code path there). */
+ * All are FP64 and agree with the reference to rounding; the unit fold of the first three is a
+ * matrix-core product over units.  No counterpart in the reference (one code path there). */
 #define CFK_SUMFAC_MATRIX_CORE 0
 #define CFK_SUMFAC_PER_CHANNEL 1
+#define CFK_SUMFAC_EXACT_SPHERE 2
+#define CFK_SUMFAC_AUTO 3
 int cfk_sumfac_set_kernel(int32_t kind);
 /* Largest number of x nodes the sumfac processes per chunk (one stage-A and one fold launch over
  * per-unit buffers of that many nodes; process-wide).  0 (the default) sizes chunks by the unit

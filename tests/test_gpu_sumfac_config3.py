@@ -20,19 +20,32 @@ def geometry12():
 
 @pytest.mark.parametrize("gamma", [0.0, 1.0])
 def test_config3_sumfac_matches_oracle_on_one_x_node(geometry12, gamma):
+    """The exact-sphere kernel (the default for this rule) and the matrix-core quadrature against the
+    oracle on one x node."""
     L, K_max = 12, 11
     lab, val, ic = geometry12
     grids = G.sumfac_grids(gamma, (48,) * 7)
     args = G.sumfac_prep(K_max, L, gamma, lab, val, ic, grids)
     prhs = [args[k] for k in oracle.SUMFAC_ORDER]
     xi = 20
-    print("config3 gamma=%g: GPU x node %d" % (gamma, xi), flush=True)
-    R_gpu = cfk.compute_rtensor_sumfac_mex(*prhs, x_range=(xi, xi + 1)).ravel(order="F")
+    out = {}
+    try:
+        for kind in ("auto", "matrix_core"):
+            print("config3 gamma=%g: GPU x node %d, kernel %s" % (gamma, xi, kind), flush=True)
+            cfk.set_sumfac_kernel(kind)
+            out[kind] = cfk.compute_rtensor_sumfac_mex(*prhs, x_range=(xi, xi + 1)).ravel(order="F")
+            if kind == "auto":
+                assert cfk.sumfac_last_timing()["kernel"] == "exact_sphere"
+    finally:
+        cfk.set_sumfac_kernel("auto")
     print("config3 gamma=%g: oracle" % gamma, flush=True)
     R_ref = oracle.sumfac(args, xi, xi + 1, nthreads=min(16, os.cpu_count() or 1))
     scale = np.max(np.abs(R_ref))
     assert scale > 0
-    assert np.max(np.abs(R_gpu - R_ref)) <= 1e-12 * scale
+    for kind, R in out.items():
+        err = np.max(np.abs(R - R_ref)) / scale
+        print("  %s: %.3e" % (kind, err), flush=True)
+        assert err <= 1e-12, kind
 
 
 @pytest.fixture
@@ -42,13 +55,13 @@ def chunk_cap():
     cfk.set_sumfac_chunk_nodes(0)
 
 
-@pytest.mark.parametrize("kind", ["matrix_core", "per_channel"])
+@pytest.mark.parametrize("kind", ["matrix_core", "per_channel", "exact_sphere"])
 def test_chunked_x_nodes_match_oracle(chunk_cap, kind):
     """The x nodes split into 3 chunks (each its own stage-A and fold launches over the unit buffers,
     compute_rtensor_sumfac_mex.cpp:126-134 sums them in one loop) against the oracle at 1e-12, and
     against the same tensor computed in one chunk."""
     L, K_max, gamma = 4, 3, 1.0
-    sizes = (7, 4, 3, 4, 3, 5, 6)
+    sizes = (7, 4, 3, 4, 3, 6, 11)   # (chi, eps) rule exact for the gain integrand (degree L + K_max = 7)
     lab, val, ic = G.real_gaunt_geometry(L)
     args = G.sumfac_prep(K_max, L, gamma, lab, val, ic, G.sumfac_grids(gamma, sizes))
     prhs = [args[k] for k in oracle.SUMFAC_ORDER]
@@ -63,7 +76,7 @@ def test_chunked_x_nodes_match_oracle(chunk_cap, kind):
         three = cfk.compute_rtensor_sumfac_mex(*prhs).ravel(order="F")
         assert cfk.sumfac_last_chunks() == 3
     finally:
-        cfk.set_sumfac_kernel("matrix_core")
+        cfk.set_sumfac_kernel("auto")
     assert np.max(np.abs(three - ref)) <= 1e-12 * scale
     assert np.max(np.abs(three - one)) <= 1e-13 * scale
 
