@@ -1231,6 +1231,146 @@ __global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(Sum
     }
 }
 
+// Stage B as one product per (l2, l3) class over the units:
+//     R[(tc,k1), (k2,k3)] += sum_u net1[u][tc][k1] B1[u][(k2,k3)] + net2[u][tc][k1] B2[u][(k2,k3)],
+//     B1 = r2l (x) r3s, B2 = r2s (x) r3l at the class's degrees,
+// shared by every t_chan of the class.  Rows (tc, k1) of a class are cut into tiles of 16 (a row
+// table built on the host; padding rows -1), columns are the N_K^2 pairs in blocks of 16, and one
+// wave computes one row tile x all column blocks for one range of units, the accumulators of the
+// blocks held in registers across the range: per 4 units one load of each net term per lane feeds
+// 2 N_blocks v_mfma_f64_16x16x4_f64 (first all blocks' net1 products, then all net2 products, so
+// no MFMA waits on the one before it), the next step's net operands already in flight.  The units'
+// radial rows at l2 and l3 are staged in LDS 32 units at a time, (r2l, r2s, r3l, r3s) per k in one
+// 32-byte entry.  Each unit range writes its partial R; sumfac_reduce_kernel adds the ranges in
+// order (deterministic).
+constexpr int FOLD_SUB = 32;  // units staged per LDS round
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+template <int NK>
+__global__ __launch_bounds__(64) void sumfac_fold_tiles_kernel(int NL, int L, const int *__restrict__ tile_rows,
+                                                               const int *__restrict__ tile_l23, int n_tiles,
+                                                               int64_t n_units, int64_t units_per_range,
+                                                               const double *__restrict__ net,
+                                                               const double *__restrict__ rad,
+                                                               double *__restrict__ partial) {
+    constexpr int NPAIR = NK * NK, NBLK = (NPAIR + 15) / 16;
+    constexpr int64_t NR1 = (int64_t)NK * NK * NK;
+    __shared__ double4v sR[FOLD_SUB][NK + 1];  // per unit and k: (r2l, r2s, r3l, r3s); k = NK pads
+    const int tile = blockIdx.x % n_tiles, range = blockIdx.x / n_tiles;
+    const int lane = threadIdx.x, i16 = lane & 15, g = lane >> 4;
+    const int l2 = tile_l23[2 * tile], l3 = tile_l23[2 * tile + 1];
+    const int row = tile_rows[16 * tile + i16];
+    const int64_t u_begin = (int64_t)range * units_per_range;
+    const int64_t u_end = u_begin + units_per_range < n_units ? u_begin + units_per_range : n_units;
+    const double arow = row >= 0 ? 1.0 : 0.0;
+    const int64_t nstride = (int64_t)NL * 2 * NK;
+    const double *nbase = net + (row >= 0 ? (int64_t)(row / NK) * 2 * NK + row % NK : 0);
+    int k2o[NBLK], k3o[NBLK];
+#pragma unroll
+    for (int j = 0; j < NBLK; ++j) {
+        const int p = 16 * j + i16;
+        k2o[j] = p < NPAIR ? p / NK : NK;  // NK: padding (b = 0)
+        k3o[j] = p < NPAIR ? p % NK : 0;
+    }
+    v4d acc[NBLK];
+#pragma unroll
+    for (int j = 0; j < NBLK; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int rstride = 2 * (L + 1) * NK;
+    // net operands of one 4-unit step (lane: unit g, row i16), fetched one step ahead
+    auto fetch_net = [&](int64_t u, double &a1, double &a2) {
+        const bool ok = u < u_end;
+        const double *nu = nbase + (ok ? u : u_begin) * nstride;
+        const double z = ok ? arow : 0.0;
+        a1 = nu[0] * z;
+        a2 = nu[NK] * z;
+    };
+    double a1n = 0.0, a2n = 0.0;
+    if (u_begin < u_end) fetch_net(u_begin + g, a1n, a2n);
+    if (lane < FOLD_SUB) sR[lane][NK] = double4v{0.0, 0.0, 0.0, 0.0};  // the padding pairs read these
+    for (int64_t s0 = u_begin; s0 < u_end; s0 += FOLD_SUB) {
+        const int ns = u_end - s0 < FOLD_SUB ? (int)(u_end - s0) : FOLD_SUB;
+        __syncthreads();  // the previous round's rows consumed
+        for (int i = lane; i < FOLD_SUB * NK; i += 64) {
+            const int uu = i / NK, k = i % NK;
+            const double *ru = rad + (s0 + (uu < ns ? uu : 0)) * rstride;
+            const double z = uu < ns ? 1.0 : 0.0;
+            sR[uu][k] = double4v{ru[l2 * NK + k] * z, ru[(L + 1) * NK + l2 * NK + k] * z, ru[l3 * NK + k] * z,
+                                 ru[(L + 1) * NK + l3 * NK + k] * z};
+        }
+        __syncthreads();
+        for (int q = 0; q < ns; q += 4) {
+            const double a1 = a1n, a2 = a2n;
+            fetch_net(s0 + q + 4 + g, a1n, a2n);
+            const double4v *r = sR[q + g];
+            double b1[NBLK], b2[NBLK];
+#pragma unroll
+            for (int j = 0; j < NBLK; ++j) {
+                const double4v x2 = r[k2o[j]], x3 = r[k3o[j]];
+                b1[j] = x2[0] * x3[3];  // r2l r3s
+                b2[j] = x2[1] * x3[2];  // r2s r3l
+            }
+#pragma unroll
+            for (int j = 0; j < NBLK; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1[j], acc[j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < NBLK; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2[j], acc[j], 0, 0, 0);
+        }
+    }
+    // D layout: register rr of a lane holds row 4 rr + lane / 16, column lane % 16
+    double *pout = partial + (int64_t)range * NR1 * NL;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        const int orow = tile_rows[16 * tile + 4 * rr + g];
+        if (orow < 0) continue;
+        const int64_t obase = (int64_t)(orow / NK) * NR1 + orow % NK;
+#pragma unroll
+        for (int j = 0; j < NBLK; ++j) {
+            const int p = 16 * j + i16;
+            if (p < NPAIR) pout[obase + NK * (p / NK) + (int64_t)NK * NK * (p % NK)] = acc[j][rr];
+        }
+    }
+}
+
+// R[o] += sum over unit ranges r (in order) of partial[r][o].
+__global__ __launch_bounds__(256) void sumfac_reduce_kernel(int64_t nR, int n_ranges, const double *__restrict__ partial,
+                                                            double *__restrict__ R) {
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= nR) return;
+    double s = R[o];
+    for (int r = 0; r < n_ranges; ++r) s += partial[(int64_t)r * nR + o];
+    R[o] = s;
+}
+
+// Row tiles of the class-level fold: t_chans grouped by (l2, l3) in first-appearance order, rows
+// (tc, k1) of a class in tiles of 16 (tile_rows: tc * NK + k1, -1 for padding), tile_l23 per tile.
+void fold_tiles(const SumfacTables &T, int NK, std::vector<int> &tile_rows, std::vector<int> &tile_l23) {
+    const int NL = T.n_l;
+    std::vector<std::pair<int, int>> classes;
+    std::vector<std::vector<int>> members;
+    for (int tc = 0; tc < NL; ++tc) {
+        const std::pair<int, int> key(T.trip[3 * tc + 1], T.trip[3 * tc + 2]);
+        size_t c = 0;
+        while (c < classes.size() && classes[c] != key) ++c;
+        if (c == classes.size()) {
+            classes.push_back(key);
+            members.emplace_back();
+        }
+        members[c].push_back(tc);
+    }
+    tile_rows.clear();
+    tile_l23.clear();
+    for (size_t c = 0; c < classes.size(); ++c) {
+        const int n_rows = (int)members[c].size() * NK;
+        for (int r0 = 0; r0 < n_rows; r0 += 16) {
+            for (int i = 0; i < 16; ++i) {
+                const int r = r0 + i;
+                tile_rows.push_back(r < n_rows ? members[c][r / NK] * NK + r % NK : -1);
+            }
+            tile_l23.push_back(classes[c].first);
+            tile_l23.push_back(classes[c].second);
+        }
+    }
+}
+
 // Compact P tables on the device (src/GeneralCollisionTensor.m:106-174), from a SumfacTablePlan.
 // One workgroup per quadrature node: the patch-1 (u,t) nodes first ([N_u1][N_t1] order), then the
 // patch-2 y nodes.  The real SH at phi = 0 of the node's polar angle (mu = 1 - 2 y^2, clamped) go to
@@ -1556,6 +1696,20 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
         if ((rc = b.net.alloc((size_t)(max_units * NL * 2 * NK)))) return rc;
         if ((rc = b.rad.alloc((size_t)(max_units * 2 * (L + 1) * NK)))) return rc;
     }
+    // class-level fold (matrix-core and exact-sphere forms): row tiles, unit ranges for about 4 waves
+    // per SIMD, one partial R per range
+    DevBuf<int> fold_rows, fold_l23;
+    DevBuf<double> fold_part;
+    int n_tiles = 0, n_ranges = 1;
+    if (mf || sphere) {
+        std::vector<int> rows, l23;
+        fold_tiles(T, NK, rows, l23);
+        n_tiles = (int)(l23.size() / 2);
+        n_ranges = std::max(1, std::min(16, (4096 + n_tiles - 1) / n_tiles));
+        if ((rc = fold_rows.upload(rows.data(), rows.size())) || (rc = fold_l23.upload(l23.data(), l23.size())) ||
+            (rc = fold_part.alloc((size_t)n_ranges * (size_t)nR)))
+            return rc;
+    }
     size_t mf_dyn = 0;
     if (mf) {
         const MfLayout lay = mf_layout(L, NK, d.Nt1, d.Nchi, d.Neps);
@@ -1590,11 +1744,13 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
                                              b.rad.p)))                                                          \
             return rc;                                                                                           \
         CFK_HIP(hipEventRecord(ev[1], nullptr));                                                                 \
-        constexpr int nblk = (KK * KK + 15) / 16;                                                                \
-        const int ntg = (NL + N_XCD - 1) / N_XCD;                                                               \
-        hipLaunchKernelGGL(sumfac_fold_mf_kernel<KK>, dim3((unsigned)(ntg * N_XCD * nblk)), dim3(64), 0, nullptr, \
-                           NL, L,                                                                                 \
-                           b.trip.p, n_units, b.net.p, b.rad.p, b.R.p);                                          \
+        const int64_t per_range = (n_units + n_ranges - 1) / n_ranges;                                          \
+        hipLaunchKernelGGL(sumfac_fold_tiles_kernel<KK>, dim3((unsigned)(n_tiles * n_ranges)), dim3(64), 0,       \
+                           nullptr, NL, L, fold_rows.p, fold_l23.p, n_tiles, n_units, per_range, b.net.p,        \
+                           b.rad.p, fold_part.p);                                                                 \
+        CFK_HIP(hipGetLastError());                                                                              \
+        hipLaunchKernelGGL(sumfac_reduce_kernel, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, nullptr, nR,    \
+                           n_ranges, fold_part.p, b.R.p);                                                        \
         CFK_HIP(hipGetLastError());                                                                              \
         CFK_HIP(hipEventRecord(ev[2], nullptr));                                                                 \
         break;                                                                                                   \
