@@ -388,7 +388,7 @@ def bench_rtensor(args, cfk):
                                     "achieved": f_units / (tm["units_ms"] * 1e-3) / 1e12, "peak": 78.6,
                                     "unit": "TFLOP/s", "frac": f_units / (tm["units_ms"] * 1e-3) / 1e12 / 78.6,
                                     "flop_units": f_units},
-           "fold": {"kernel": "sumfac_fold_mf_kernel", "flop": f_fold,
+           "fold": {"kernel": "sumfac_fold_tiles_kernel", "flop": f_fold,
                     "tflops": f_fold / (ts["fold_ms"] * 1e-3) / 1e12,
                     "frac": f_fold / (ts["fold_ms"] * 1e-3) / 1e12 / 78.6},
            **res}

@@ -934,74 +934,6 @@ __global__ __launch_bounds__(MF_THREADS, 4) void sumfac_units_mf_kernel(SumfacDe
     }
 }
 
-// Stage B, matrix-core form: for one t_chan tc and one block of 16 (k2, k3) pairs,
-//     R[k1, (k2,k3), tc] += sum_u net1[u][tc][k1] r2l[u][k2] r3s[u][k3] + net2[u][tc][k1] r2s[u][k2] r3l[u][k3]
-// as two products over the unit dimension on v_mfma_f64_16x16x4_f64 (rows k1, columns the pairs,
-// inner 4 units per instruction).  One wave per (tc, pair block); two accumulators per term over
-// alternating unit quads keep consecutive instructions independent.  The pair blocks of one tc
-// read the same unit rows, so they are mapped to one XCD (workgroups go round-robin over the 8
-// XCDs by block index) and share its L2: block b runs on XCD b % 8 and takes tc = 8 g + b % 8,
-// pair block (b / 8) % NBLK with g = b / (8 NBLK).
-constexpr int N_XCD = 8;
-template <int NK>
-__global__ __launch_bounds__(64) void sumfac_fold_mf_kernel(int NL, int L, const int *__restrict__ trip,
-                                                            int64_t n_units, const double *__restrict__ net,
-                                                            const double *__restrict__ rad, double *__restrict__ R) {
-    constexpr int NPAIR = NK * NK, NBLK = (NPAIR + 15) / 16;
-    const int xcd = blockIdx.x % N_XCD, rblk = blockIdx.x / N_XCD;
-    const int blk = rblk % NBLK, tc = (rblk / NBLK) * N_XCD + xcd;
-    if (tc >= NL) return;
-    const int lane = threadIdx.x, i16 = lane & 15, uq = lane >> 4;
-    const int l2 = trip[3 * tc + 1], l3 = trip[3 * tc + 2];
-    const int pair = 16 * blk + i16;
-    const bool pair_ok = pair < NPAIR;
-    const int k2 = pair_ok ? pair / NK : 0, k3 = pair_ok ? pair % NK : 0;
-    const int k1 = i16 < NK ? i16 : NK - 1;
-    const int rstride = 2 * (L + 1) * NK;
-    const int o2l = l2 * NK + k2, o2s = (L + 1) * NK + l2 * NK + k2;
-    const int o3l = l3 * NK + k3, o3s = (L + 1) * NK + l3 * NK + k3;
-    const double arow = i16 < NK ? 1.0 : 0.0;
-    const double bcol = pair_ok ? 1.0 : 0.0;
-    v4d a1[2] = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
-    v4d a2[2] = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
-    const int64_t nstride = (int64_t)NL * 2 * NK;
-    const double *nbase = net + (int64_t)tc * 2 * NK + k1;
-    int64_t u0 = 0;
-    for (; u0 + 8 <= n_units; u0 += 8) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int64_t u = u0 + 4 * h + uq;
-            const double *nu = nbase + u * nstride;
-            const double *ru = rad + u * rstride;
-            const double n1 = nu[0] * arow, n2 = nu[NK] * arow;
-            const double b1 = ru[o2l] * ru[o3s] * bcol, b2 = ru[o2s] * ru[o3l] * bcol;
-            a1[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(n1, b1, a1[h], 0, 0, 0);
-            a2[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(n2, b2, a2[h], 0, 0, 0);
-        }
-    }
-    for (; u0 < n_units; u0 += 4) {
-        const int64_t u = u0 + uq;
-        const bool ok = u < n_units;
-        const int64_t uc = ok ? u : n_units - 1;
-        const double *nu = nbase + uc * nstride;
-        const double *ru = rad + uc * rstride;
-        const double z = ok ? 1.0 : 0.0;
-        const double n1 = nu[0] * arow * z, n2 = nu[NK] * arow * z;
-        const double b1 = ru[o2l] * ru[o3s] * bcol, b2 = ru[o2s] * ru[o3l] * bcol;
-        a1[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(n1, b1, a1[0], 0, 0, 0);
-        a2[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(n2, b2, a2[0], 0, 0, 0);
-    }
-    if (!pair_ok) return;
-#pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-        const int kk = 4 * rr + uq;
-        if (kk < NK) {
-            const int64_t o = kk + (int64_t)NK * (k2 + (int64_t)NK * (k3 + (int64_t)NK * tc));
-            R[o] += (a1[0][rr] + a1[1][rr]) + (a2[0][rr] + a2[1][rr]);
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------------------------
 // Exact-sphere variant (the default where it applies).
 //
@@ -1047,38 +979,61 @@ inline int sphere_threads(int L, int NL) {
     return (n + 63) / 64 * 64;
 }
 
-template <int NK>
+// Slot sum of t_chan tc for both sides: d_side = sum_{s in [s0, s1)} P(s) Y_side[q_s], loads issued
+// 8 slots at a time.
+__device__ inline void slot_sums(const double *__restrict__ Pv, const int *__restrict__ qi, int s0, int s1,
+                                 const double *Y0, const double *Y1, double &d0, double &d1) {
+    d0 = 0.0;
+    d1 = 0.0;
+    for (int j0 = s0; j0 < s1; j0 += 8) {
+        double P[8];
+        int q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool ok = j0 + u < s1;
+            P[u] = ok ? Pv[j0 + u] : 0.0;
+            q[u] = ok ? qi[j0 + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            d0 += P[u] * Y0[q[u]];
+            d1 += P[u] * Y1[q[u]];
+        }
+    }
+}
+
+// MULTI: the patch-1 units (N_t1 inner steps each, t_chan sums accumulated in registers), grid
+// nx * N_u1; otherwise the patch-2 units (one inner step), grid nx * N_y2 * N_t2.  Storage index of
+// a unit: (x - x0) * Ux + r, the reference's unit order.
+template <int NK, bool MULTI>
 __global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(SumfacDev d,
                                                                              const SphereRule *__restrict__ rule,
-                                                                             int x0, int nx,
-                                                                   double *__restrict__ net,
-                                                                   double *__restrict__ rad) {
+                                                                             int x0, double *__restrict__ net,
+                                                                             double *__restrict__ rad) {
     __shared__ double sY[2][(MAXL + 1) * (MAXL + 1)];
     __shared__ double sGk[2][(MAXL + 1) * NK];
     __shared__ double sRadU[2][(MAXL + 1) * MAXK];
-    __shared__ TGeom sG[MAXG];  // geometry of every inner step of the unit
+    __shared__ TGeom sG[MULTI ? MAXG : 1];  // geometry of every inner step of the unit
 
     const int L = d.L, NL = d.NL;
     const int Ux = d.Nu1 + d.Ny2 * d.Nt2;
     const int b = blockIdx.x;
     int xl, r;
-    if (b < nx * d.Nu1) {
+    if (MULTI) {
         xl = b / d.Nu1;
         r = b % d.Nu1;
     } else {
-        const int bb = b - nx * d.Nu1;
-        xl = bb / (d.Ny2 * d.Nt2);
-        r = d.Nu1 + bb % (d.Ny2 * d.Nt2);
+        xl = b / (d.Ny2 * d.Nt2);
+        r = d.Nu1 + b % (d.Ny2 * d.Nt2);
     }
     const int64_t unit = (int64_t)xl * Ux + r;
     const int xi = x0 + xl;
-    const bool patch1 = r < d.Nu1;
     const int tid = threadIdx.x;
     const double x_i = d.x[xi];
 
     double v_large, v_small, Jac_unit;
     int u_idx = 0, y_idx = 0, t2_idx = 0;
-    if (patch1) {
+    if (MULTI) {
         u_idx = r;
         const double u = d.u1[u_idx];
         v_large = sqrt(x_i / 2.0) * (1.0 + u);
@@ -1095,7 +1050,7 @@ __global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(Sum
     }
     // geometry of inner step it (as the matrix-core kernel)
     auto tgeom = [&](int it, TGeom &g) {
-        if (patch1) {
+        if (MULTI) {
             const double u = d.u1[u_idx], t = d.t1[it];
             const double y = u * t;
             const double Jac_t = d.Wt1[it] * (4.0 * u * t);
@@ -1113,14 +1068,16 @@ __global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(Sum
             make_tgeom(g, x_i, v_large, v_small, c_beta, s_beta, u_mag, Jac_t, d.alpha);
         }
     };
-    if (tid < 2 * (L + 1)) {
-        const int side = tid / (L + 1), l = tid % (L + 1);
+    const int n_ev = 2 * (L + 1) * SP_NODES;
+    const int n_inner_t = MULTI ? d.Nt1 : 1;
+    // radial tables at the unit's two speeds, on lanes past the ones forming the geometry
+    if (tid >= 64 && tid < 64 + 2 * (L + 1)) {
+        const int side = (tid - 64) / (L + 1), l = (tid - 64) % (L + 1);
         double tmp[NK];
         radial_all_k<NK>(l, side ? v_small : v_large, d.radnorm, tmp);
 #pragma unroll
         for (int k = 0; k < NK; ++k) sRadU[side][l * MAXK + k] = tmp[k];
     }
-    const int n_inner_t = patch1 ? d.Nt1 : 1;
     for (int it = tid; it < n_inner_t; it += blockDim.x) tgeom(it, sG[it]);
 
     // this thread's t_chan
@@ -1128,13 +1085,13 @@ __global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(Sum
     const int tc = own ? tid : 0;
     const int s0 = d.qi_off[tc], s1 = own ? d.qi_off[tc + 1] : s0;
     const int l1 = d.trip[3 * tc];
-    double acc0[NK], acc1[NK];
+    double acc0[MULTI ? NK : 1], acc1[MULTI ? NK : 1];
 #pragma unroll
-    for (int k = 0; k < NK; ++k) {
+    for (int k = 0; k < (MULTI ? NK : 1); ++k) {
         acc0[k] = 0.0;
         acc1[k] = 0.0;
     }
-    const int n_ev = 2 * (L + 1) * SP_NODES;
+    double dl0 = 0.0, dl1 = 0.0;  // one-step units: the slot sums themselves
     const int ns_all = d.qi_off[NL];
     for (int it = 0; it < n_inner_t; ++it) {
         __syncthreads();  // geometry written; the previous step's sY / sGk consumed
@@ -1191,37 +1148,40 @@ __global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(Sum
         __syncthreads();
         if (own) {
             const double *Pv =
-                patch1 ? d.Pg1 + ((int64_t)u_idx * d.Nt1 + it) * ns_all : d.Pg2 + (int64_t)y_idx * ns_all;
-            double d0 = 0.0, d1 = 0.0;
-            for (int sl = s0; sl < s1; ++sl) {
-                const double P = Pv[sl];
-                const int q = d.qi[sl];
-                d0 += P * sY[0][q];
-                d1 += P * sY[1][q];
-            }
-            const double *G0 = sGk[0] + l1 * NK, *G1 = sGk[1] + l1 * NK;
+                MULTI ? d.Pg1 + ((int64_t)u_idx * d.Nt1 + it) * ns_all : d.Pg2 + (int64_t)y_idx * ns_all;
+            double d0, d1;
+            slot_sums(Pv, d.qi, s0, s1, sY[0], sY[1], d0, d1);
+            if (MULTI) {
+                const double *G0 = sGk[0] + l1 * NK, *G1 = sGk[1] + l1 * NK;
 #pragma unroll
-            for (int k = 0; k < NK; ++k) {
-                acc0[k] += d0 * G0[k];
-                acc1[k] += d1 * G1[k];
+                for (int k = 0; k < (MULTI ? NK : 1); ++k) {
+                    acc0[k] += d0 * G0[k];
+                    acc1[k] += d1 * G1[k];
+                }
+            } else {
+                dl0 = d0;
+                dl1 = d1;
             }
         }
     }
-    __syncthreads();
+    if (MULTI) __syncthreads();  // every step's loss weight formed
     if (own) {
         // loss: the radial factor is fixed per unit, so only loss_weight * P_loss is summed over t
         double ls = 0.0;
-        if (patch1) {
+        if (MULTI) {
             for (int it = 0; it < d.Nt1; ++it)
                 ls += sG[it].loss_weight * d.Pl1[((int64_t)u_idx * d.Nt1 + it) * NL + tc];
         } else {
             ls = sG[0].loss_weight * d.Pl2[(int64_t)y_idx * NL + tc];
         }
         double *o = net + (unit * NL + tc) * 2 * NK;
+        const double *G0 = sGk[0] + l1 * NK, *G1 = sGk[1] + l1 * NK;
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-            o[k] = Jac_unit * (acc0[k] - sRadU[0][l1 * MAXK + k] * ls);
-            o[NK + k] = Jac_unit * (acc1[k] - sRadU[1][l1 * MAXK + k] * ls);
+            const double g0 = MULTI ? acc0[MULTI ? k : 0] : dl0 * G0[k];
+            const double g1 = MULTI ? acc1[MULTI ? k : 0] : dl1 * G1[k];
+            o[k] = Jac_unit * (g0 - sRadU[0][l1 * MAXK + k] * ls);
+            o[NK + k] = Jac_unit * (g1 - sRadU[1][l1 * MAXK + k] * ls);
         }
     }
     double *ro = rad + unit * 2 * (L + 1) * NK;
@@ -1735,10 +1695,15 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     case KK: {                                                                                                   \
         CFK_HIP(hipEventRecord(ev[0], nullptr));                                                                 \
         if (sphere) {                                                                                            \
-            hipLaunchKernelGGL(sumfac_units_sphere_kernel<KK>, dim3((unsigned)n_units),                          \
-                               dim3((unsigned)sphere_threads(L, NL)), 0, nullptr, d,                              \
-                               reinterpret_cast<const SphereRule *>(rule_dev.p), xc, nx, b.net.p,                \
-                               b.rad.p);                                                                         \
+            const SphereRule *rp = reinterpret_cast<const SphereRule *>(rule_dev.p);                             \
+            const dim3 blk((unsigned)sphere_threads(L, NL));                                                     \
+            if (nx * d.Nu1 > 0)                                                                                  \
+                hipLaunchKernelGGL((sumfac_units_sphere_kernel<KK, true>), dim3((unsigned)(nx * d.Nu1)), blk, 0,  \
+                                   nullptr, d, rp, xc, b.net.p, b.rad.p);                                        \
+            CFK_HIP(hipGetLastError());                                                                          \
+            if (nx * d.Ny2 * d.Nt2 > 0)                                                                          \
+                hipLaunchKernelGGL((sumfac_units_sphere_kernel<KK, false>), dim3((unsigned)(nx * d.Ny2 * d.Nt2)), \
+                                   blk, 0, nullptr, d, rp, xc, b.net.p, b.rad.p);                                \
             CFK_HIP(hipGetLastError());                                                                          \
         } else if ((rc = launch_units_mf<KK>(mf_tiles_per_wave(L), mf_dyn, d, xc, nx, n_units, b.net.p,         \
                                              b.rad.p)))                                                          \
