@@ -1202,21 +1202,27 @@ __global__ __launch_bounds__(SP_MAX_THREADS) void sumfac_units_sphere_kernel(Sum
 // no MFMA waits on the one before it), the next step's net operands already in flight.  The units'
 // radial rows at l2 and l3 are staged in LDS 32 units at a time, (r2l, r2s, r3l, r3s) per k in one
 // 32-byte entry.  Each unit range writes its partial R; sumfac_reduce_kernel adds the ranges in
-// order (deterministic).
+// order (deterministic).  Workgroups go to the 8 XCDs round-robin by block index; the block ->
+// (tile, range) table puts the tiles of one class and one range on one XCD, so they share that
+// XCD's L2 copy of the radial rows and of the net rows of a t_chan split across two tiles.
 constexpr int FOLD_SUB = 32;  // units staged per LDS round
+constexpr int N_XCD = 8;
 typedef double double4v __attribute__((ext_vector_type(4)));
 
 template <int NK>
 __global__ __launch_bounds__(64) void sumfac_fold_tiles_kernel(int NL, int L, const int *__restrict__ tile_rows,
-                                                               const int *__restrict__ tile_l23, int n_tiles,
-                                                               int64_t n_units, int64_t units_per_range,
+                                                               const int *__restrict__ tile_l23,
+                                                               const int2 *__restrict__ items, int64_t n_units,
+                                                               int64_t units_per_range,
                                                                const double *__restrict__ net,
                                                                const double *__restrict__ rad,
                                                                double *__restrict__ partial) {
     constexpr int NPAIR = NK * NK, NBLK = (NPAIR + 15) / 16;
     constexpr int64_t NR1 = (int64_t)NK * NK * NK;
     __shared__ double4v sR[FOLD_SUB][NK + 1];  // per unit and k: (r2l, r2s, r3l, r3s); k = NK pads
-    const int tile = blockIdx.x % n_tiles, range = blockIdx.x / n_tiles;
+    const int2 item = items[blockIdx.x];
+    if (item.x < 0) return;  // padding of the per-XCD lists
+    const int tile = item.x, range = item.y;
     const int lane = threadIdx.x, i16 = lane & 15, g = lane >> 4;
     const int l2 = tile_l23[2 * tile], l3 = tile_l23[2 * tile + 1];
     const int row = tile_rows[16 * tile + i16];
@@ -1302,7 +1308,8 @@ __global__ __launch_bounds__(256) void sumfac_reduce_kernel(int64_t nR, int n_ra
 
 // Row tiles of the class-level fold: t_chans grouped by (l2, l3) in first-appearance order, rows
 // (tc, k1) of a class in tiles of 16 (tile_rows: tc * NK + k1, -1 for padding), tile_l23 per tile.
-void fold_tiles(const SumfacTables &T, int NK, std::vector<int> &tile_rows, std::vector<int> &tile_l23) {
+void fold_tiles(const SumfacTables &T, int NK, std::vector<int> &tile_rows, std::vector<int> &tile_l23,
+                std::vector<int> &tile_class) {
     const int NL = T.n_l;
     std::vector<std::pair<int, int>> classes;
     std::vector<std::vector<int>> members;
@@ -1318,6 +1325,7 @@ void fold_tiles(const SumfacTables &T, int NK, std::vector<int> &tile_rows, std:
     }
     tile_rows.clear();
     tile_l23.clear();
+    tile_class.clear();
     for (size_t c = 0; c < classes.size(); ++c) {
         const int n_rows = (int)members[c].size() * NK;
         for (int r0 = 0; r0 < n_rows; r0 += 16) {
@@ -1327,8 +1335,77 @@ void fold_tiles(const SumfacTables &T, int NK, std::vector<int> &tile_rows, std:
             }
             tile_l23.push_back(classes[c].first);
             tile_l23.push_back(classes[c].second);
+            tile_class.push_back((int)c);
         }
     }
+}
+
+// Wave slots of the fold kernel on this device (workgroups per CU at its register use x CUs).
+template <int NK>
+int fold_wave_slots() {
+    int per_cu = 0, dev = 0, n_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sumfac_fold_tiles_kernel<NK>, 64, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * n_cu;
+}
+
+// Unit ranges of the fold: the fewest (>= 1, <= 32) for which n_tiles x ranges waves fill the last
+// round of wave slots to at least 90 % (a partial last round leaves SIMDs idle at the end of the
+// launch), else the best filled.
+int fold_ranges(int NK, int n_tiles) {
+    int slots = 0;
+    switch (NK) {
+#define X(KK)                                \
+    case KK:                                 \
+        slots = fold_wave_slots<KK>();       \
+        break;
+        X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+#undef X
+    }
+    if (slots <= 0 || n_tiles <= 0) return 1;
+    int best = 1;
+    double best_fill = 0.0;
+    for (int r = 1; r <= 32; ++r) {
+        const int64_t waves = (int64_t)n_tiles * r;
+        const int64_t rounds = (waves + slots - 1) / slots;
+        const double fill = (double)waves / (double)(rounds * slots);
+        if (waves >= slots && fill >= 0.9) return r;
+        if (fill > best_fill) {
+            best_fill = fill;
+            best = r;
+        }
+    }
+    return best;
+}
+
+// Block -> (tile, range) for n_ranges unit ranges: the (range, class) groups are spread over the XCDs
+// largest first onto the least loaded one; block b takes entry b / N_XCD of XCD b % N_XCD's list, the
+// lists padded to one length with (-1, -1).
+std::vector<int2> fold_items(const std::vector<int> &tile_class, int n_ranges) {
+    const int n_tiles = (int)tile_class.size();
+    const int n_classes = n_tiles ? tile_class.back() + 1 : 0;
+    std::vector<std::vector<int>> tiles_of(n_classes);
+    for (int t = 0; t < n_tiles; ++t) tiles_of[tile_class[t]].push_back(t);
+    std::vector<int> order(n_classes);
+    for (int c = 0; c < n_classes; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return tiles_of[a].size() > tiles_of[b].size(); });
+    std::vector<std::vector<int2>> lists(N_XCD);
+    for (int r = 0; r < n_ranges; ++r)
+        for (int c : order) {
+            size_t best = 0;
+            for (size_t x = 1; x < lists.size(); ++x)
+                if (lists[x].size() < lists[best].size()) best = x;
+            for (int t : tiles_of[c]) lists[best].push_back(int2{t, r});
+        }
+    size_t len = 0;
+    for (auto &l : lists) len = std::max(len, l.size());
+    std::vector<int2> items(len * N_XCD, int2{-1, -1});
+    for (int x = 0; x < N_XCD; ++x)
+        for (size_t i = 0; i < lists[x].size(); ++i) items[i * N_XCD + x] = lists[x][i];
+    return items;
 }
 
 // Compact P tables on the device (src/GeneralCollisionTensor.m:106-174), from a SumfacTablePlan.
@@ -1659,14 +1736,18 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
     // class-level fold (matrix-core and exact-sphere forms): row tiles, unit ranges for about 4 waves
     // per SIMD, one partial R per range
     DevBuf<int> fold_rows, fold_l23;
+    DevBuf<int2> fold_blk;
     DevBuf<double> fold_part;
-    int n_tiles = 0, n_ranges = 1;
+    int n_ranges = 1, n_fold_blocks = 0;
     if (mf || sphere) {
-        std::vector<int> rows, l23;
-        fold_tiles(T, NK, rows, l23);
-        n_tiles = (int)(l23.size() / 2);
-        n_ranges = std::max(1, std::min(16, (4096 + n_tiles - 1) / n_tiles));
+        std::vector<int> rows, l23, cls;
+        fold_tiles(T, NK, rows, l23, cls);
+        const int n_tiles = (int)cls.size();
+        n_ranges = fold_ranges(NK, n_tiles);
+        const std::vector<int2> items = fold_items(cls, n_ranges);
+        n_fold_blocks = (int)items.size();
         if ((rc = fold_rows.upload(rows.data(), rows.size())) || (rc = fold_l23.upload(l23.data(), l23.size())) ||
+            (rc = fold_blk.upload(items.data(), items.size())) ||
             (rc = fold_part.alloc((size_t)n_ranges * (size_t)nR)))
             return rc;
     }
@@ -1710,9 +1791,9 @@ int run_sumfac(const SumfacTables &T, const SumfacGrids &G, int NK, double alpha
             return rc;                                                                                           \
         CFK_HIP(hipEventRecord(ev[1], nullptr));                                                                 \
         const int64_t per_range = (n_units + n_ranges - 1) / n_ranges;                                          \
-        hipLaunchKernelGGL(sumfac_fold_tiles_kernel<KK>, dim3((unsigned)(n_tiles * n_ranges)), dim3(64), 0,       \
-                           nullptr, NL, L, fold_rows.p, fold_l23.p, n_tiles, n_units, per_range, b.net.p,        \
-                           b.rad.p, fold_part.p);                                                                 \
+        hipLaunchKernelGGL(sumfac_fold_tiles_kernel<KK>, dim3((unsigned)n_fold_blocks), dim3(64), 0, nullptr, NL, \
+                           L, fold_rows.p, fold_l23.p, fold_blk.p, n_units, per_range, b.net.p, b.rad.p,         \
+                           fold_part.p);                                                                          \
         CFK_HIP(hipGetLastError());                                                                              \
         hipLaunchKernelGGL(sumfac_reduce_kernel, dim3((unsigned)((nR + 255) / 256)), dim3(256), 0, nullptr, nR,    \
                            n_ranges, fold_part.p, b.R.p);                                                        \
