@@ -97,6 +97,8 @@ def main():
                          "of the mixed path), e.g. to profile one kernel")
     ap.add_argument("--no-compare", action="store_true",
                     help="rtensor: skip the per-channel kernel run kept beside the matrix-core one")
+    ap.add_argument("--accumulate", action="store_true",
+                    help="default workload: time Q = 0; Q += Q(f) (the MEX semantics) instead of Q = Q(f)")
     ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate", "copy"],
                     help="hs-l8-k10: the metric; uneven: 4096 distributions with L in {4,6,8,12}, K=L+2 "
                          "(batched item kernels in a hipGraph); rtensor: GPU tensor generation, L=12, K=12, 48^7 "
@@ -148,8 +150,13 @@ def main():
     Q = torch.zeros(K * nq, B, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
 
+    # one step = Q(f) for the batch: overwrite mode (Q = Q(f), the operator value a time stepper
+    # wants), or with --accumulate the MEX semantics on a zeroed Q (Q = 0; Q += Q(f))
+    plan.set_accumulate(args.accumulate)
+
     def step(precision):
-        Q.zero_()
+        if args.accumulate:
+            Q.zero_()
         plan.execute(Q, fT, B, precision=precision, layout="batch_fastest", stream=stream)
 
     # FP64 reference result on the same batch
@@ -158,17 +165,22 @@ def main():
     Q64 = Q.clone()
 
     def timed(precision, steps, warmup):
+        """(seconds per step, kernel seconds per launch): the steps timed with nothing else on the
+        stream, then the same number of steps again with the library's HIP-event kernel timer on (its
+        event records add launch overhead, so they stay out of the step time)."""
         for _ in range(warmup):
             step(precision)
-        torch.cuda.synchronize()
-        plan.read_timing()
-        plan.set_timing(True)
         torch.cuda.synchronize()
         ts = time.perf_counter()
         for _ in range(steps):
             step(precision)
         torch.cuda.synchronize()
         te = time.perf_counter()
+        plan.read_timing()
+        plan.set_timing(True)
+        for _ in range(steps):
+            step(precision)
+        torch.cuda.synchronize()
         plan.set_timing(False)
         kms, nl = plan.read_timing()
         return (te - ts) / steps, kms / max(nl, 1) / 1e3
@@ -208,7 +220,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "hard spheres (gamma=1), L=8, K=10 radial modes, batch 4096 perturbed Maxwellians, "
                                "mixed-precision angular-first contraction", "L": L, "K": K, "batch": B,
-                   "gamma": gamma, "N_G": n_g, "slices": S, "sumfac_grid": list(grid)},
+                   "gamma": gamma, "N_G": n_g, "slices": S, "sumfac_grid": list(grid),
+                   "step": "Q = 0; Q += Q(f)" if args.accumulate else "Q = Q(f)"},
         "max_rel_err_vs_fp64": max_rel,
         "invariants_max_abs": inv_abs,
         "fp64_evals_per_s": B / t64,

@@ -78,6 +78,7 @@ SIGNATURES = {
     "cfk_plan_set_tile": ([_vp, _i32, _i32, _i32, _i32], ctypes.c_int),
     "cfk_plan_set_waves": ([_vp, _i32, _i32], ctypes.c_int),
     "cfk_plan_get_waves": ([_vp, _i32, ctypes.POINTER(_i32)], ctypes.c_int),
+    "cfk_plan_set_accumulate": ([_vp, _i32], ctypes.c_int),
     "cfk_plan_set_split": ([_vp, _i32, _i32], ctypes.c_int),
     "cfk_plan_get_split": ([_vp, _i32, ctypes.POINTER(_i32)], ctypes.c_int),
     "cfk_plan_linearize": ([_vp, _dp, _dp, _vp], ctypes.c_int),

@@ -54,7 +54,7 @@ class CollisionPlan:
         self._h = h
 
     def execute(self, Q, f, B=None, variant="angular_first", precision="fp64", layout="natural", stream=None):
-        """Q += Q(f) for B distributions (device buffers of float64)."""
+        """Q += Q(f) for B distributions (device buffers of float64); Q = Q(f) after set_accumulate(False)."""
         if self._h is None:
             raise CfkError(-1, "plan destroyed")
         n = self.N_Q * self.K
@@ -106,6 +106,10 @@ class CollisionPlan:
         w = ctypes.c_int32(0)
         check(lib().cfk_plan_get_waves(self._h, PRECISION[precision], ctypes.byref(w)))
         return int(w.value)
+
+    def set_accumulate(self, accumulate=True):
+        """Output mode of execute: Q += Q(f) (True, the MEX semantics) or Q = Q(f) (False)."""
+        check(lib().cfk_plan_set_accumulate(self._h, 1 if accumulate else 0))
 
     def set_split(self, precision, split=-1):
         """Row split between two waves per batch tile (symmetrized precisions; 0 = off, 1 = on, -1 = the

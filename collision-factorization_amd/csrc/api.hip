@@ -413,14 +413,14 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
                          : launch_af_sym_f32(p, tl, s_f32, s_q64, B, 0, st)))
                 return rc;
             tm.stop();
-            return launch_transpose(s_q64, Q, NK, B, true, st);
+            return launch_transpose(s_q64, Q, NK, B, p->accumulate != 0, st);
         }
         if ((rc = packed ? launch_pack_rows(f, s_f32, p->N_Q, p->K, B, false, st)
                          : launch_to_f32(f, s_f32, NK * B, st)))
             return rc;
         KernelTimer tm(p, st, stg == nullptr);
-        if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, s_f32, Q, B, 1, st)
-                     : launch_af_sym_f32(p, tl, s_f32, Q, B, 1, st)))
+        if ((rc = mx ? launch_af_mx(p, p->sym_Rmx.p, tl, s_f32, Q, B, p->accumulate, st)
+                     : launch_af_sym_f32(p, tl, s_f32, Q, B, p->accumulate, st)))
             return rc;
         tm.stop();
         return CFK_OK;
@@ -428,8 +428,8 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     const bool sym = precision == CFK_PREC_FP64_SYM;
     if (!natural) {
         KernelTimer tm(p, st, stg == nullptr);
-        rc = sym ? launch_af_sym_f64(p, tl, f, Q, B, 1, st)
-                 : launch_sparse_fp64(p, variant, tl, f, Q, B, 1, st);
+        rc = sym ? launch_af_sym_f64(p, tl, f, Q, B, p->accumulate, st)
+                 : launch_sparse_fp64(p, variant, tl, f, Q, B, p->accumulate, st);
         if (rc) return rc;
         tm.stop();
         return CFK_OK;
@@ -448,7 +448,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
         if (rc) return rc;
         tm.stop();
     }
-    return launch_transpose(s_q64, Q, NK, B, true, st);
+    return launch_transpose(s_q64, Q, NK, B, p->accumulate != 0, st);
 }
 
 // One-shot host path shared by the three sparse MEX equivalents.
@@ -695,6 +695,13 @@ int cfk_plan_set_waves(cfk_plan plan, int32_t precision, int32_t wpb) {
     if (wpb != 0 && wpb != 1 && wpb != 2 && wpb != 4)
         return set_error(CFK_EINVAL, "waves per workgroup must be 0, 1, 2 or 4 (got %d)", wpb);
     plan->wpb[precision] = wpb;
+    return CFK_OK;
+}
+
+int cfk_plan_set_accumulate(cfk_plan plan, int32_t accumulate) {
+    if (!plan) return set_error(CFK_EINVAL, "null plan");
+    if (accumulate != 0 && accumulate != 1) return set_error(CFK_EINVAL, "accumulate must be 0 or 1");
+    plan->accumulate = accumulate;
     return CFK_OK;
 }
 

@@ -106,6 +106,7 @@ constexpr int32_t SLICE_PRECISE = 1, SLICE_SYMMETRIC = 2, SLICE_G1_SHIFT = 8;
 // Plan: geometry + R resident on the device.
 struct cfk_plan_s {
     int device = 0;
+    int accumulate = 1;  // execute: Q += Q(f) (1, the MEX semantics) or Q = Q(f) (0)
     int N_Q = 0, K = 0, n_l = 0;
     int64_t N_G = 0, S = 0;
     // host copies (for re-planning and validation)

@@ -248,6 +248,12 @@ int cfk_plan_set_waves(cfk_plan plan, int32_t precision, int32_t wpb);
 /* The waves per workgroup the plan uses for a precision (1, 2 or 4). */
 int cfk_plan_get_waves(cfk_plan plan, int32_t precision, int32_t *wpb_out);
 
+/* Output mode of cfk_plan_execute / cfk_plan_execute_static for this plan: accumulate = 1 (the
+ * default, as the MEX kernels: Q += Q(f)) or 0 (Q = Q(f), for a caller that wants the operator's value
+ * itself, e.g. a time stepper; Q's previous contents are neither read nor needed).  Every variant,
+ * precision and layout.  No reference counterpart. */
+int cfk_plan_set_accumulate(cfk_plan plan, int32_t accumulate);
+
 /* Row split for the symmetrized precisions (CFK_PREC_FP64_SYM, CFK_PREC_MIXED): with split = 1 each
  * batch tile runs as a workgroup of two waves that take the first and second half (by the cost model)
  * of every output row's slices; the second hands its partial sums to the first through LDS, which adds

@@ -150,6 +150,37 @@ def test_config1_layouts_and_accumulation(config1, torch_cuda):
     assert rel_err(a, Qref, nq, K) <= TOL64
 
 
+def test_overwrite_mode_writes_q_without_reading_it(config1, torch_cuda):
+    """cfk_plan_set_accumulate(0): Q = Q(f) for every variant, precision, mixed kernel and layout, with
+    Q's previous contents (NaN here) never read; equal to accumulating into zeros."""
+    torch = torch_cuda
+    c = config1
+    nq, K, B = c["nq"], c["K"], 70
+    f = perturbed_maxwellians(c["L"], K, B, seed=8)
+    fd = torch.from_numpy(f).cuda()
+    fT = fd.reshape(B, K * nq).t().contiguous()
+    plan = c["plan"]
+    runs = [("fp64", v, None) for v in ("angular_first", "radial_first", "naive")]
+    runs += [("fp64_sym", "angular_first", None), ("mixed", "angular_first", 0), ("mixed", "angular_first", 1)]
+    try:
+        for precision, variant, pf in runs:
+            if pf is not None:
+                plan.set_tile("mixed", 0, 0, pf)
+            for layout, fin in (("natural", fd), ("batch_fastest", fT)):
+                print("overwrite: %s %s pf=%s %s" % (precision, variant, pf, layout), flush=True)
+                plan.set_accumulate(True)
+                Qa = torch.zeros(B * K * nq, dtype=torch.float64, device="cuda")
+                plan.execute(Qa, fin, B, variant=variant, precision=precision, layout=layout)
+                plan.set_accumulate(False)
+                Qo = torch.full((B * K * nq,), float("nan"), dtype=torch.float64, device="cuda")
+                plan.execute(Qo, fin, B, variant=variant, precision=precision, layout=layout)
+                torch.cuda.synchronize()
+                assert torch.equal(Qa, Qo), (precision, variant, pf, layout)
+    finally:
+        plan.set_accumulate(True)
+        plan.set_tile("mixed", 0, 0, None)
+
+
 def test_config1_sumfac_on_gpu_matches_golden():
     path = os.path.join(GOLDEN, "sumfac_hs_l8_k10_g48.npz")
     if not os.path.exists(path):
