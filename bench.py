@@ -1,8 +1,8 @@
 """Benchmark: Q(f) evaluations per second, hard spheres, L=8, K=10, batch 4096, on one MI355X.
 
-A step is one pass of the hot path over the whole batch resident in HBM: Q = 0; Q += Q(f) for 4096
+A step is one pass of the hot path over the whole batch resident in HBM: Q = Q(f) for 4096
 distributions through the mixed-precision path (symmetrized FP32 contraction, FP64 accumulation
-and conservation projection).  Its error is measured against the plain FP64 GPU operator on the
+and conservation projection); --accumulate times Q = 0; Q += Q(f), the MEX semantics, instead.  Its error is measured against the plain FP64 GPU operator on the
 same batch (must be <= 1e-6 normwise relative per distribution), and the FP64 path is timed too.
 
 Prints ONE JSON line (see the "metric" key).  --gpus N > 1 is refused: this build runs on one GPU.
@@ -19,15 +19,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "collision-factorization_amd"))
 
 METRIC = "Q(f) evals/sec at HS Lmax=8 batch 4096 vs FP64 GPU path @<=1e-6 rel err"
-# MI355X published peaks: FP64 vector/matrix 78.6 TF/s; FP32 157.3 TF/s (matrix, and vector with packed
-# FP32, which this build does not use); FP32 vector without packed math 78.6 TF/s
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp32_vector_unpacked": 78.6e12}
-# HBM bytes per launch of each mixed kernel at this workload (4096 distributions, batch-fastest), from
-# two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration; the commands and
-# raw figures are in the named profile.  A kernel without an entry here reports traffic as null.
+# MI355X published peaks: FP64 vector/matrix 78.6 TF/s; FP32 matrix and vector 157.3 TF/s.  Measured on the
+# MI355X with register-only loops (wave64 FMAs per SIMD): FP32 FMAs whose operands are all VGPRs issue
+# about every 2.7-3.6 cycles at 2-8 waves per SIMD (two waves' FMAs issue together; the ideal is 2 cycles,
+# the published 157.3 TF/s), FP32 FMAs with an SGPR operand every 4.3-6.2 cycles (never together: at
+# most 78.6 TF/s), FP64 FMAs every 4.3-5 cycles (78.6 TF/s).
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp32_vector_sgpr_operand": 78.6e12}
+# HBM bytes per launch of each mixed kernel at this workload (4096 distributions, batch-fastest, Q = Q(f)),
+# from two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration; the commands
+# and raw figures are in the named profile.  A kernel without an entry here reports traffic as null.
 PROFILED_TRAFFIC = {
-    "valu": (80620 * 1024, "profiles/af_kernel_mixed_hs_l8_k10_b4096.txt"),
-    "mfma": (80640 * 1024, "profiles/af_mx_kernel_mixed_hs_l8_k10_b4096.txt"),
+    "valu": (55392 * 1024, "profiles/af_kernel_mixed_hs_l8_k10_b4096.txt"),
 }
 
 
@@ -39,21 +41,25 @@ def work_model(lab, ic, K):
 
 
 def executed_flops(st, K, matrix_cores):
-    """FLOP per evaluation the mixed kernel executes on the symmetrized tables (plan statistics st):
-    stage 1 (h sums over the kept nonzeros, one outer product per q2 group) and the FP64 radial stage of
-    the precise slices on the VALU; the radial stage of the other slices on the VALU (full blocks K^3,
-    symmetric blocks K^2 (K+1)/2) or on the matrix cores (16x16x4 steps of 4 values of (k2,k3), rows k1
-    padded to 16; executed, not useful, FLOPs)."""
+    """FLOP per evaluation the mixed kernel executes on the symmetrized tables (plan statistics st), by the
+    issue rate that bounds them:
+      dual  -- FP32 FMAs with VGPR operands only: the outer products f2 (x) h of stage 1 (one per q2 group);
+      single -- FMAs with a wave-uniform SGPR operand: the h sums g f3 (Gaunt value), the radial stage on
+                the VALU (R from the scalar path: full blocks K^3, symmetric blocks K^2 (K+1)/2) and the
+                FP64 radial stage of the precise slices;
+      mfma  -- the radial stage on the matrix cores (16x16x4 steps of 4 values of (k2,k3), rows k1 padded
+               to 16; executed, not useful, FLOPs)."""
     plain = st["sym_S"] - st["sym_symmetric"] - st["sym_precise"]
-    valu = 2.0 * (st["sym_N_G"] * K + st["sym_groups"] * K * K + st["sym_precise"] * K ** 3)
+    dual = 2.0 * st["sym_groups"] * K * K
+    single = 2.0 * (st["sym_N_G"] * K + st["sym_precise"] * K ** 3)
     if not matrix_cores:
-        valu += 2.0 * (plain * K ** 3 + st["sym_symmetric"] * K * K * (K + 1) / 2)
-        return valu, 0.0
+        single += 2.0 * (plain * K ** 3 + st["sym_symmetric"] * K * K * (K + 1) / 2)
+        return dual, single, 0.0
     kc = K if K <= 10 else (K + 1) // 2
     steps_full = (K * kc + 3) // 4 * ((K + kc - 1) // kc)
     steps_sym = (K * (K + 1) // 2 + 3) // 4 if kc == K else steps_full
     mfma = (plain * steps_full + st["sym_symmetric"] * steps_sym) * 4 * 16 * 2.0
-    return valu, mfma
+    return dual, single, mfma
 
 
 def cpu_baseline(lab, val, ic, R, nq, K, f, budget_s):
@@ -99,11 +105,13 @@ def main():
                     help="rtensor: skip the per-channel kernel run kept beside the matrix-core one")
     ap.add_argument("--accumulate", action="store_true",
                     help="default workload: time Q = 0; Q += Q(f) (the MEX semantics) instead of Q = Q(f)")
-    ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate", "copy"],
+    ap.add_argument("--workload", default="hs-l8-k10", choices=["hs-l8-k10", "uneven", "rtensor", "surrogate", "copy", "dense"],
                     help="hs-l8-k10: the metric; uneven: 4096 distributions with L in {4,6,8,12}, K=L+2 "
                          "(batched item kernels in a hipGraph); rtensor: GPU tensor generation, L=12, K=12, 48^7 "
                          "grids; surrogate: training on 2e5 mixture pairs and 16384-cell relaxation; copy: "
-                         "a plain 512 MiB device copy, three times (calibration of HBM traffic counters)")
+                         "a plain 512 MiB device copy, three times (calibration of HBM traffic counters); dense: the "
+                         "dense-tensor contraction (dense_tensor_kernel_mex) at L=8, K=10, N=810, C = 4.25 GB on the "
+                         "device, one distribution per call and four per call")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
@@ -122,6 +130,8 @@ def main():
         return bench_surrogate(args, torch, cfk)
     if args.workload == "copy":
         return bench_copy(torch)
+    if args.workload == "dense":
+        return bench_dense(args, torch, cfk, perturbed_maxwellians)
 
     L, K_max, gamma, B = 8, 9, 1.0, args.batch
     K = K_max + 1
@@ -199,11 +209,11 @@ def main():
     matrix_cores = tiles["mixed"][2] == 1
     kind = "mfma" if matrix_cores else "valu"
     st = plan.stats()
-    e_valu, e_mfma = executed_flops(st, K, matrix_cores)
-    # bound on the executed work: VALU FLOPs at the FP32 vector rate without packed math plus MFMA FLOPs
-    # at the FP32 matrix rate, added (on gfx950 FP32 MFMAs and FP32 VALU work of other waves on the same
-    # SIMD do not overlap: measured, 4.65 ms for both vs 4.71 / 4.33 ms for each alone)
-    t_exec = (e_valu / PEAK["fp32_vector_unpacked"] + e_mfma / PEAK["fp32"]) * B
+    e_dual, e_single, e_mfma = executed_flops(st, K, matrix_cores)
+    # bound on the executed work: dual-issue FMAs at 157.3 TF/s, SGPR-operand FMAs at 78.6 TF/s and MFMA
+    # FLOPs at 157.3 TF/s, added (on gfx950 FP32 MFMAs and FP32 VALU work of other waves on the same SIMD
+    # do not overlap: measured, waves of both kinds together take as long as the two alone added)
+    t_exec = (e_dual / PEAK["fp32"] + e_single / PEAK["fp32_vector_sgpr_operand"] + e_mfma / PEAK["fp32"]) * B
     traffic, traffic_src = PROFILED_TRAFFIC.get(kind, (None, None)) if B == 4096 else (None, None)
     out = {
         "metric": METRIC,
@@ -240,10 +250,13 @@ def main():
                                        "time; the kernel executes fewer (executed)",
                      "kernel": "matrix-core radial stage" if matrix_cores else "VALU",
                      "traffic": traffic, "traffic_source": traffic_src,
-                     "executed": {"valu_flop_per_eval": e_valu, "mfma_flop_per_eval": e_mfma,
-                                  "tflops": (e_valu + e_mfma) * B / kmix / 1e12,
-                                  "valu_frac_of_unpacked_fp32": e_valu * B / kmix / PEAK["fp32_vector_unpacked"],
-                                  "bound_ms": t_exec * 1e3, "frac_of_bound": t_exec / kmix},
+                     "executed": {"valu_dual_issue_flop_per_eval": e_dual, "valu_sgpr_operand_flop_per_eval": e_single,
+                                  "mfma_flop_per_eval": e_mfma,
+                                  "tflops": (e_dual + e_single + e_mfma) * B / kmix / 1e12,
+                                  "frac_of_fp32_peak": (e_dual + e_single + e_mfma) * B / kmix / PEAK["fp32"],
+                                  "bound_ms": t_exec * 1e3, "frac_of_bound": t_exec / kmix,
+                                  "bound_basis": "dual-issue VGPR FMAs at 157.3 TF/s + SGPR-operand FMAs at 78.6 TF/s "
+                                                 "+ MFMA at 157.3 TF/s, added"},
                      "fp64_path_frac": F * B / k64 / PEAK["fp64"]},
         "cpu_baseline": None,
     }
@@ -422,6 +435,55 @@ def bench_copy(torch):
     dt = (time.perf_counter() - t0) / 3
     print(json.dumps({"metric": "device copy bandwidth", "value": 2 * n * 8 / dt / 1e9, "unit": "GB/s",
                       "n_gpus": 1, "higher_is_better": True, "config": {"workload": "copy 512 MiB x 3"}}))
+
+
+def bench_dense(args, torch, cfk, perturbed_maxwellians):
+    """Dense-tensor contraction Q += C f f (dense_tensor_kernel_mex) for hard spheres, L=8, K=10: N = 810,
+    C (N^3 doubles, 4.25 GB) assembled on the device from the COO and the sumfac R.  One call streams C
+    once for up to 4 distributions, so the bound is HBM bandwidth: bytes = N^3 * 8 per call."""
+    L, K_max, gamma = 8, 9, 1.0
+    K = K_max + 1
+    nq = (L + 1) ** 2
+    N = nq * K
+    lab, val, ic = cfk.real_gaunt(L)
+    R = cfk.build_rtensor(K_max, L, gamma, (48,) * 7, geometry=(lab, val, ic))
+    C = torch.empty(N ** 3, dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    cfk.dense_assemble(C, lab, val, ic, R, nq, K)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+    res = {}
+    for B in (1, 4):
+        f = torch.from_numpy(perturbed_maxwellians(L, K, B, seed=0).reshape(B, N)).cuda()
+        Q = torch.zeros(B, N, dtype=torch.float64, device="cuda")
+        for _ in range(max(1, args.warmup)):
+            cfk.dense_execute(Q, f, C, N, B, stream=stream)
+        steps = max(3, args.steps // 2)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            cfk.dense_execute(Q, f, C, N, B, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        res[B] = ms
+    bytes_per_call = N ** 3 * 8
+    gbs = bytes_per_call / (res[1] * 1e-3) / 1e9
+    print(json.dumps({
+        "metric": "dense-tensor Q(f) evals/sec (dense_tensor_kernel_mex), HS L=8, K=10, N=810",
+        "value": 1e3 / res[1], "unit": "evals/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic", "config": {"workload": "dense contraction, hard spheres, L=8, K=10, N=810, C=4.25 GB",
+                                         "N": N},
+        "ms_per_call": {"B=1": res[1], "B=4": res[4]}, "evals_per_s_B4": 4e3 / res[4],
+        "assemble_s": t_asm,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
+                     "achieved_basis": "N^3 * 8 bytes of C per call (B=1) over the call's HIP-event time "
+                                       "(stream kernel plus the partial-sum reduction)",
+                     "achieved_B4": bytes_per_call / (res[4] * 1e-3) / 1e9},
+    }))
 
 
 def bench_surrogate(args, torch, cfk):

@@ -12,6 +12,6 @@ from .geometry import (build_rtensor, gauss_gen_laguerre, gauss_legendre, real_g
 from .matfile import load_operator, load_precalc, read_mat_dataset, save_mat, save_operator  # noqa: F401
 from .mex import (MexError, angular_first_collision_kernel_mex, compute_rtensor_sumfac_mex,  # noqa: F401
                   dense_tensor_kernel_mex, naive_collision_kernel_mex, radial_first_collision_kernel_mex)
-from .plan import CollisionPlan, MultiPlan, conservation_project, dense_execute  # noqa: F401
+from .plan import CollisionPlan, MultiPlan, conservation_project, dense_assemble, dense_execute  # noqa: F401
 from .surrogate import (Mlp, Surrogate, make_mlp, maxwellian_mixture_params, project_maxwellians,  # noqa: F401
                         split_params)

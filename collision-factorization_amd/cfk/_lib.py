@@ -89,6 +89,7 @@ SIGNATURES = {
     "cfk_multi_set_graph": ([_vp, _i32], ctypes.c_int),
     "cfk_multi_execute": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_i64), _vp], ctypes.c_int),
     "cfk_dense_execute": ([_vp, _vp, _vp, _i64, _i64, _vp], ctypes.c_int),
+    "cfk_dense_assemble": ([_vp, _dp, _dp, _dp, _i64, _dp, _i64, _i32, _i32, _vp], ctypes.c_int),
     "cfk_basis_evaluate": ([_vp, _vp, _i64, _i32, _i32, _vp], ctypes.c_int),
     "cfk_moments": ([_vp, _vp, _i64, _i32, _i32, _i32, _vp], ctypes.c_int),
     "cfk_moment_weights": ([_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),

@@ -159,6 +159,24 @@ class CollisionPlan:
             pass
 
 
+def dense_assemble(C, labels, vals, ic_map, R, N_Q, K, stream=None):
+    """Write the dense tensor C[(k1 N_Q + q1) + (k2 N_Q + q2) N + (k3 N_Q + q3) N^2] = R_t[k1,k2,k3] g of the
+    COO (host arrays, 1-based) into the device tensor C (N^3 float64 values, N = N_Q K), as
+    assemble_full_tensor (src/GeneralCollisionTensor.m:283-311)."""
+    labels = np.asarray(labels)
+    if labels.ndim != 2 or labels.shape[1] != 3:
+        raise ValueError("labels must be N_G x 3")
+    n_g = labels.shape[0]
+    if np.asarray(vals).size != n_g or np.asarray(ic_map).size != n_g:
+        raise ValueError("vals and ic_map must have N_G = %d entries" % n_g)
+    N = int(N_Q) * int(K)
+    _check_dev_f64(C, N ** 3, "C")
+    Rf = f64(R)
+    check(lib().cfk_dense_assemble(_ptr(C), dptr(f64(labels)), dptr(f64(vals)), dptr(f64(ic_map)), n_g,
+                                   dptr(Rf), Rf.size, int(N_Q), int(K), _stream_ptr(stream)))
+    return C
+
+
 def dense_execute(Q, f, C, N, B=1, stream=None):
     _check_dev_f64(Q, int(B) * int(N), "Q")
     _check_dev_f64(f, int(B) * int(N), "f")

@@ -41,8 +41,10 @@ int check_device() {
 int launch_sparse_fp64(const cfk_plan_s *p, int variant, const Tile &tl, const double *fT, double *QT, int64_t B,
                        int accumulate, hipStream_t st);
 int launch_transpose(const double *in, double *out, int64_t rows, int64_t cols, bool add, hipStream_t st);
-int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t B, double *part, int nsplit,
-                 hipStream_t st);
+int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t B, double *part, hipStream_t st);
+int dense_part_doubles(int64_t N, int64_t *n);
+int launch_dense_assemble(double *C, const int4 *zl, const double *zg, const double *R, int64_t N_G, int N_Q, int K,
+                          hipStream_t st);
 int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStream_t st);
 int default_kc(int K);
 int table_padding();
@@ -509,14 +511,42 @@ int cfk_dense_execute(double *Q_dev, const double *f_dev, const double *C_dev, i
     int rc;
     if ((rc = check_device())) return rc;
     if (N < 1 || B < 0) return set_error(CFK_EINVAL, "N must be >= 1 and B >= 0");
+    if (N > 4096) return set_error(CFK_EINVAL, "N=%lld: dense tensor of N^3 doubles is too large", (long long)N);
     if (B == 0) return CFK_OK;
     if (!Q_dev || !f_dev || !C_dev) return set_error(CFK_EINVAL, "null pointer");
-    const int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(N, 256));
-    DevBuf<double> part;
-    if ((rc = part.alloc((size_t)nsplit * 4 * N))) return rc;
+    int64_t n_part = 0;
+    if ((rc = dense_part_doubles(N, &n_part))) return rc;
+    // the per-workgroup partial sums live for the call only: stream-ordered allocation, so the call
+    // neither waits for the GPU nor for the allocator
     hipStream_t st = (hipStream_t)stream;
-    if ((rc = launch_dense(Q_dev, f_dev, C_dev, N, B, part.p, nsplit, st))) return rc;
-    CFK_HIP(hipStreamSynchronize(st));  // part is freed on return
+    double *part = nullptr;
+    CFK_HIP(hipMallocAsync((void **)&part, (size_t)n_part * sizeof(double), st));
+    rc = launch_dense(Q_dev, f_dev, C_dev, N, B, part, st);
+    const hipError_t fe = hipFreeAsync(part, st);
+    if (rc) return rc;
+    CFK_HIP(fe);
+    return CFK_OK;
+}
+
+int cfk_dense_assemble(double *C_dev, const double *labels, const double *vals, const double *ic_map, int64_t N_G,
+                       const double *R, int64_t R_numel, int32_t N_Q, int32_t K, void *stream) {
+    int rc;
+    if ((rc = validate_coo(labels, vals, ic_map, N_G, R, R_numel, N_Q, K))) return rc;
+    if ((int64_t)N_Q * K > 4096)
+        return set_error(CFK_EINVAL, "N = N_Q*K = %lld: dense tensor of N^3 doubles is too large", (long long)N_Q * K);
+    if (!C_dev) return set_error(CFK_EINVAL, "null C pointer");
+    if ((rc = check_device())) return rc;
+    std::vector<int4> zl((size_t)N_G);
+    for (int64_t z = 0; z < N_G; ++z)
+        zl[z] = make_int4((int)labels[z] - 1, (int)labels[z + N_G] - 1, (int)labels[z + 2 * N_G] - 1, (int)ic_map[z] - 1);
+    DevBuf<int4> dzl;
+    DevBuf<double> dzg, dR;
+    if ((rc = dzl.upload(zl.data(), zl.size()))) return rc;
+    if ((rc = dzg.upload(vals, (size_t)N_G))) return rc;
+    if ((rc = dR.upload(R, (size_t)R_numel))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = launch_dense_assemble(C_dev, dzl.p, dzg.p, dR.p, N_G, N_Q, K, st))) return rc;
+    CFK_HIP(hipStreamSynchronize(st));  // the uploaded tables are freed on return
     return CFK_OK;
 }
 

@@ -702,49 +702,6 @@ __global__ __launch_bounds__(256) void pack_rows_pair_kernel(const double *__res
         }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Dense (dense_tensor_kernel_mex.cpp:14-29): Q[i] += sum_{j,k} C[i + jN + kN^2] f[j] f[k].
-// C is streamed once for up to NB distributions; k is split over gridDim.y and the per-split
-// partial sums are reduced in split order (deterministic).
-// ---------------------------------------------------------------------------------------------
-template <int NB>
-__global__ __launch_bounds__(256) void dense_partial_kernel(const double *__restrict__ C, const double *__restrict__ f,
-                                                             double *__restrict__ part, int64_t N, int nb,
-                                                             int64_t k_per_split) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t k0 = (int64_t)blockIdx.y * k_per_split;
-    const int64_t k1 = min(N, k0 + k_per_split);
-    double acc[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = 0.0;
-    const int64_t ii = i < N ? i : 0;
-    for (int64_t k = k0; k < k1; ++k) {
-        double fk[NB];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) fk[b] = (b < nb) ? f[(int64_t)b * N + k] : 0.0;
-        const double *Ck = C + k * N * N + ii;
-        for (int64_t j = 0; j < N; ++j) {
-            const double c = Ck[j * N];
-#pragma unroll
-            for (int b = 0; b < NB; ++b) acc[b] = fma(c, ((b < nb) ? f[(int64_t)b * N + j] : 0.0) * fk[b], acc[b]);
-        }
-    }
-    if (i < N) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-            if (b < nb) part[((int64_t)blockIdx.y * nb + b) * N + i] = acc[b];
-    }
-}
-
-__global__ __launch_bounds__(256) void dense_reduce_kernel(const double *__restrict__ part, double *__restrict__ Q,
-                                                            int64_t N, int nb, int nsplit) {
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= N * nb) return;
-    double s = 0.0;
-    for (int sp = 0; sp < nsplit; ++sp) s += part[(int64_t)sp * nb * N + idx];
-    Q[idx] += s;
-}
-
 // Zero the five collision invariants of each distribution.
 __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int layout) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -946,21 +903,6 @@ int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st) {
     return CFK_OK;
 }
 
-int launch_dense(double *Q, const double *f, const double *C, int64_t N, int64_t B, double *part, int nsplit,
-                 hipStream_t st) {
-    const int64_t kps = (N + nsplit - 1) / nsplit;
-    for (int64_t b0 = 0; b0 < B; b0 += 4) {
-        const int nb = (int)std::min<int64_t>(4, B - b0);
-        dim3 grid((unsigned)((N + 255) / 256), (unsigned)nsplit);
-        hipLaunchKernelGGL(dense_partial_kernel<4>, grid, dim3(256), 0, st, C, f + b0 * N, part, N, nb, kps);
-        CFK_HIP(hipGetLastError());
-        const int64_t tot = N * nb;
-        hipLaunchKernelGGL(dense_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
-                           Q + b0 * N, N, nb, nsplit);
-        CFK_HIP(hipGetLastError());
-    }
-    return CFK_OK;
-}
 
 int launch_conservation(double *Q, int64_t B, int N_Q, int K, int layout, hipStream_t st) {
     if (B <= 0) return CFK_OK;

@@ -26,8 +26,9 @@
 // K = 10).
 //
 // On gfx950 FP32 MFMAs of one wave and FP32 VALU work of another wave on the same SIMD do not run
-// concurrently (measured with a register-only probe), so the gain over the VALU kernel is the smaller
-// instruction count of the radial stage, not overlap; at K = 10 the two kernels are equally fast,
+// concurrently (measured with register-only loops: both kinds together take the sum of their times), so
+// the gain over the VALU kernel is the cheaper radial stage (VALU FMAs that read R from SGPRs issue one
+// wave at a time), not overlap; at K = 10 the two kernels are equally fast,
 // from K = 12 this one is faster (the 16-row tiles are fuller and the VALU kernel needs half chunks).
 #include <hip/hip_runtime.h>
 

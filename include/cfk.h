@@ -423,8 +423,18 @@ int cfk_moments(double *out_dev, const double *c_dev, int64_t B, int32_t N_Q, in
  * Host only. */
 int cfk_moment_weights(int32_t N_Q, int32_t K, int32_t idx[40], double w[40]);
 
-/* dense variant on device: Q_dev[b] += C f_b f_b for B distributions of length N (layout natural). */
+/* dense variant on device: Q_dev[b] += C f_b f_b for B distributions of length N <= 4096 (layout natural),
+ * C streamed from HBM once per pass of up to 4 distributions (N <= 1024) or one (larger N).  Asynchronous
+ * on `stream` (0: the default stream); the per-call workspace is stream-ordered. */
 int cfk_dense_execute(double *Q_dev, const double *f_dev, const double *C_dev, int64_t N, int64_t B, void *stream);
+
+/* Dense tensor on device from the COO and R, replacing assemble_full_tensor
+ * (src/GeneralCollisionTensor.m:283-311; benchmark_collision_contraction_scaling.m:54-79 builds it inline):
+ * C_dev[(k1 N_Q + q1) + (k2 N_Q + q2) N + (k3 N_Q + q3) N^2] = R[k1,k2,k3,t] g for every nonzero, 0
+ * elsewhere, N = N_Q*K <= 4096 (C_dev holds N^3 doubles).  COO and R are host arrays as for the MEX
+ * entry points (1-based double indices).  Returns after the tensor is written. */
+int cfk_dense_assemble(double *C_dev, const double *labels, const double *vals, const double *ic_map, int64_t N_G,
+                       const double *R, int64_t R_numel, int32_t N_Q, int32_t K, void *stream);
 
 /* Conservation projection: set the five collision invariants (mass, three momenta, energy) of each
  * distribution to exactly zero: q=0,1,2,3 at k=0 and q=0 at k=1 (0-based), cf.
