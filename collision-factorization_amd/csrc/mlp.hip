@@ -34,7 +34,21 @@ constexpr int GT = 256;  // threads per GEMM workgroup (4 waves, 2 x 2 wave tile
 constexpr int BK = 32;   // reduction depth per LDS stage (one MFMA k-step)
 constexpr int LDK = BK + 8;  // padded LDS row (elements): 80 bytes keeps 16-byte rows spread over banks
 
-__device__ inline float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7, far below the BF16 rounding of the activations
+// that follows): erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2), t = 1 / (1 + p x), x >= 0.
+// A dozen instructions instead of the library erff's range reduction: the epilogue of the network's
+// products evaluates it for every activation.
+__device__ inline float erf_as(float x) {
+    const float ax = fabsf(x);
+    const float t = __frcp_rn(fmaf(0.3275911f, ax, 1.0f));
+    float p = fmaf(t, 1.061405429f, -1.453152027f);
+    p = fmaf(t, p, 1.421413741f);
+    p = fmaf(t, p, -0.284496736f);
+    p = fmaf(t, p, 0.254829592f);
+    const float y = 1.0f - t * p * __expf(-ax * ax);
+    return copysignf(y, x);
+}
+__device__ inline float gelu_f(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ inline float gelu_grad_f(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
@@ -117,6 +131,150 @@ __global__ __launch_bounds__(GT) void gemm_nt_kernel(const __bf16 *__restrict__ 
         const float bn = bias ? bias[gn] : 0.0f;
 #pragma unroll
         for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int gm = m0 + wm + 16 * i + 4 * (lane / 16) + rr;
+                if (gm < M) {
+                    const float v = acc[i][j][rr] + bn;
+                    if (C) C[(int64_t)gm * ldc + gn] = v;
+                    if (Cb) Cb[(int64_t)gm * ldcb + gn] = (__bf16)(GELU ? gelu_f(v) : v);
+                }
+            }
+    }
+}
+
+// Large products (the surrogate's inference over thousands of cells): C[M][N] = A[M][K] B[N][K]^T (+ bias)
+// with a 128 x 128 workgroup tile, four waves of 64 x 64, and a ring of PS stages of BK = 32 in LDS that
+// the loads fill directly (global_load_lds, 16 bytes per lane, no registers and no LDS stores on the way),
+// PS - 1 stages ahead of the one being multiplied, so the loads' latency hides behind PS - 1 stages of
+// MFMAs.  A stage holds 128 rows x 64 bytes of A and of B; a wave's load instruction fills 1 KB of it,
+// lane L landing at 16-byte slot L of its kilobyte.  The slot of (row r, 8-element chunk c) is
+// 4 r + (c ^ ((r >> 2) & 3)): the 16 rows one MFMA operand read touches then start in 16 different bank
+// groups (no LDS bank conflicts) although rows are not padded.
+constexpr int PT = 128;  // workgroup tile (both dimensions)
+constexpr int PS = 4;    // LDS stages in the ring
+constexpr int PSTAGE = PT * BK;  // bf16 elements of one operand per stage (8 KB)
+
+__device__ __forceinline__ int pipe_slot(int r, int c) { return 4 * r + (c ^ ((r >> 2) & 3)); }
+
+// The ring's stages are separate LDS arrays and the main loop is unrolled over them, so every load and
+// every operand read names its stage at compile time: the compiler then sees that the reads of a stage
+// do not alias the loads in flight into the others (it still waits for all of them once per round of
+// four stages, at the loop's back edge).
+__shared__ __align__(1024) __bf16 pipe_a0[PSTAGE], pipe_a1[PSTAGE], pipe_a2[PSTAGE], pipe_a3[PSTAGE];
+__shared__ __align__(1024) __bf16 pipe_b0[PSTAGE], pipe_b1[PSTAGE], pipe_b2[PSTAGE], pipe_b3[PSTAGE];
+
+template <int SLOT>
+__device__ __forceinline__ __bf16 *pipe_a() {
+    return SLOT == 0 ? pipe_a0 : SLOT == 1 ? pipe_a1 : SLOT == 2 ? pipe_a2 : pipe_a3;
+}
+template <int SLOT>
+__device__ __forceinline__ __bf16 *pipe_b() {
+    return SLOT == 0 ? pipe_b0 : SLOT == 1 ? pipe_b1 : SLOT == 2 ? pipe_b2 : pipe_b3;
+}
+
+template <bool GELU>
+__global__ __launch_bounds__(GT) void gemm_nt_pipe_kernel(const __bf16 *__restrict__ A, const __bf16 *__restrict__ B,
+                                                          float *__restrict__ C, __bf16 *__restrict__ Cb,
+                                                          const float *__restrict__ bias, int M, int N, int K,
+                                                          int64_t lda, int64_t ldb, int64_t ldc, int64_t ldcb) {
+    static_assert(PS == 4, "the unrolled loop below is written for four stages");
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    // Tile of this workgroup.  Workgroups go to the 8 XCDs round-robin in launch order; when the row tiles
+    // divide among them, XCD x takes the row tiles t = x (mod 8), each with all column tiles in a row, so
+    // an XCD's L2 holds its A rows and all of B instead of every XCD streaming all of A.
+    const int NT = (N + PT - 1) / PT, MT = (M + PT - 1) / PT;
+    int mt, nt;
+    if (MT % 8 == 0) {
+        const int idx = blockIdx.x / 8;
+        nt = idx % NT;
+        mt = (idx / NT) * 8 + blockIdx.x % 8;
+    } else {
+        nt = blockIdx.x % NT;
+        mt = blockIdx.x / NT;
+    }
+    const int m0 = mt * PT, n0 = nt * PT;
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+    // Loader: this wave fills kilobytes 2*wave and 2*wave+1 of each operand's stage (rows 32*wave .. +32);
+    // lane L of kilobyte j holds slot 64 j + L = (row, chunk) with row = 16 j + L / 4 and chunk from the swizzle.
+    const __bf16 *ga[2], *gb[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int kb = 2 * wave + j, r = 16 * kb + lane / 4, c = (lane % 4) ^ ((r >> 2) & 3);
+        const int gm = min(m0 + r, M - 1), gn = min(n0 + r, N - 1);  // rows past the end load a valid row
+        ga[j] = A + (int64_t)gm * lda + 8 * c;                       // (their products are never stored)
+        gb[j] = B + (int64_t)gn * ldb + 8 * c;
+    }
+    auto issue = [&](__bf16 *as, __bf16 *bs, int k0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(ga[j] + k0),
+                                             (__attribute__((address_space(3))) void *)(as + (2 * wave + j) * 512), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(gb[j] + k0),
+                                             (__attribute__((address_space(3))) void *)(bs + (2 * wave + j) * 512), 16, 0, 0);
+        }
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // operand reads: lane l takes row (16-row block base + l % 16), chunk l / 16; the swizzle term depends
+    // on l % 16 only, because block bases are multiples of 16
+    const int r16 = lane % 16, cq = lane / 16;
+    const int rd = 4 * r16 + (cq ^ ((r16 >> 2) & 3));  // slot within a 16-row block
+    auto mult = [&](const __bf16 *as, const __bf16 *bs) {
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(as + ((wm + 16 * i) * 4 + rd) * 8);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(bs + ((wn + 16 * j) * 4 + rd) * 8);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    };
+    // Stage s has landed once at most the loads of the younger in-flight stages are outstanding (4 loads
+    // per stage and thread; loads retire in order); the barrier then makes every wave's part visible, and
+    // the slot refilled after it was last read in stage s - 1, which every wave has finished.
+    auto wait_stage = [&](int s, int nk) {
+        const int younger = min(nk - 1 - s, PS - 2);
+        if (younger >= 2)
+            __builtin_amdgcn_s_waitcnt(0x0F70 | 8);
+        else if (younger == 1)
+            __builtin_amdgcn_s_waitcnt(0x0F70 | 4);
+        else
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+        __builtin_amdgcn_s_barrier();
+    };
+    const int nk = K / BK;
+    if (0 < nk) issue(pipe_a<0>(), pipe_b<0>(), 0);
+    if (1 < nk) issue(pipe_a<1>(), pipe_b<1>(), BK);
+    if (2 < nk) issue(pipe_a<2>(), pipe_b<2>(), 2 * BK);
+#define CFK_PIPE_STEP(U, REFILL)                                                    \
+    {                                                                               \
+        const int s = s0 + U;                                                       \
+        if (s < nk) {                                                               \
+            wait_stage(s, nk);                                                      \
+            if (s + 3 < nk) issue(pipe_a<REFILL>(), pipe_b<REFILL>(), (s + 3) * BK); \
+            mult(pipe_a<U>(), pipe_b<U>());                                         \
+        }                                                                           \
+    }
+    for (int s0 = 0; s0 < nk; s0 += 4) {
+        CFK_PIPE_STEP(0, 3)
+        CFK_PIPE_STEP(1, 0)
+        CFK_PIPE_STEP(2, 1)
+        CFK_PIPE_STEP(3, 2)
+    }
+#undef CFK_PIPE_STEP
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int gn = n0 + wn + 16 * j + lane % 16;
+        if (gn >= N) continue;
+        const float bn = bias ? bias[gn] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int rr = 0; rr < 4; ++rr) {
                 const int gm = m0 + wm + 16 * i + 4 * (lane / 16) + rr;
@@ -242,19 +400,25 @@ int gemm_nt(const __bf16 *A, const __bf16 *B, float *C, __bf16 *Cb, const float 
     if (M <= 0 || N <= 0) return CFK_OK;
     if (K % BK || lda % 8 || ldb % 8) return set_error(CFK_EINVAL, "gemm_nt: K=%d lda=%lld ldb=%lld not aligned", K,
                                                        (long long)lda, (long long)ldb);
-    // 128 x 128 tiles when they fill the GPU twice over, 64 x 64 otherwise
-    const int64_t big = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
-    const bool b128 = big >= 512;
-    const int TM = b128 ? 128 : 64, TN = TM;
+    // the pipelined 128 x 128 kernel when its tiles fill the GPU twice over, 64 x 64 tiles otherwise
+    const int64_t big = (int64_t)((M + PT - 1) / PT) * ((N + PT - 1) / PT);
+    if (big >= 512) {
+        const dim3 grid((unsigned)big);
+        if (gelu)
+            hipLaunchKernelGGL((gemm_nt_pipe_kernel<true>), grid, dim3(GT), 0, st, A, B, C, Cb, bias, M, N, K, lda, ldb,
+                               ldc, ldcb);
+        else
+            hipLaunchKernelGGL((gemm_nt_pipe_kernel<false>), grid, dim3(GT), 0, st, A, B, C, Cb, bias, M, N, K, lda, ldb,
+                               ldc, ldcb);
+        CFK_HIP(hipGetLastError());
+        return CFK_OK;
+    }
+    const int TM = 64, TN = 64;
     if ((M + TM - 1) / TM > 65535) return set_error(CFK_EINVAL, "gemm_nt: M=%d too large", M);
     const dim3 grid((unsigned)((N + TN - 1) / TN), (unsigned)((M + TM - 1) / TM));
-#define CFK_G(BM_, G_) \
-    hipLaunchKernelGGL((gemm_nt_kernel<BM_, BM_, G_>), grid, dim3(GT), 0, st, A, B, C, Cb, bias, M, N, K, lda, ldb, ldc, ldcb)
-    if (b128) {
-        if (gelu) CFK_G(128, true); else CFK_G(128, false);
-    } else {
-        if (gelu) CFK_G(64, true); else CFK_G(64, false);
-    }
+#define CFK_G(BM_, BN_, G_) \
+    hipLaunchKernelGGL((gemm_nt_kernel<BM_, BN_, G_>), grid, dim3(GT), 0, st, A, B, C, Cb, bias, M, N, K, lda, ldb, ldc, ldcb)
+    if (gelu) CFK_G(64, 64, true); else CFK_G(64, 64, false);
 #undef CFK_G
     CFK_HIP(hipGetLastError());
     return CFK_OK;

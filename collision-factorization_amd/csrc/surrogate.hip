@@ -66,19 +66,22 @@ __global__ void norm_transpose_kernel(const double *__restrict__ in, const float
     }
 }
 
-// Per cell: n = f[0], s = max(|f/n - e1|, S_MIN).  A workgroup covers 64 cells with 4 threads per
-// cell striding over the features (coalesced over cells), reduced through LDS.
+// Per cell: n = f[0], s = max(|f/n - e1|, S_MIN).  A workgroup covers 64 cells with DN threads per
+// cell striding over the features (coalesced over cells, several loads in flight per thread), reduced
+// through LDS in a fixed order.
 constexpr double S_MIN = 1e-6;
-__global__ void dev_norm_kernel(const double *__restrict__ f, int NK, int64_t B, double *__restrict__ nvec,
-                                double *__restrict__ svec) {
-    __shared__ double red[4][64];
+constexpr int DN = 16;
+__global__ __launch_bounds__(64 * DN) void dev_norm_kernel(const double *__restrict__ f, int NK, int64_t B,
+                                                           double *__restrict__ nvec, double *__restrict__ svec) {
+    __shared__ double red[DN][64];
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     const int64_t b = (int64_t)blockIdx.x * 64 + tx;
     double s2 = 0.0, n = 1.0;
     if (b < B) {
         n = f[b];
         const double in = 1.0 / n;
-        for (int i = 1 + ty; i < NK; i += 4) {
+#pragma unroll 4
+        for (int i = 1 + ty; i < NK; i += DN) {
             const double d = f[(int64_t)i * B + b] * in;
             s2 = fma(d, d, s2);
         }
@@ -86,31 +89,40 @@ __global__ void dev_norm_kernel(const double *__restrict__ f, int NK, int64_t B,
     red[ty][tx] = s2;
     __syncthreads();
     if (ty == 0 && b < B) {
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < DN; ++r) t += red[r][tx];
         nvec[b] = n;
-        svec[b] = fmax(sqrt(red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx]), S_MIN);
+        svec[b] = fmax(sqrt(t), S_MIN);
     }
 }
 
 // Q[k1][q][b] = n^2 (sum_k J_l[k1][k] d[k][q][b] + s_b^2 (Y[k1][q][b] s[..] + mu[..])), invariants zeroed.
-// use_net = 0 keeps the linear part only.
-__global__ void post_kernel(const double *__restrict__ f, const double *__restrict__ nvec,
-                            const double *__restrict__ svec, const float *__restrict__ Y,
-                            const float *__restrict__ mu, const float *__restrict__ s, const double *__restrict__ JL,
-                            double *__restrict__ Q, int K, int N_Q, int64_t B, int use_net) {
-    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
-    if (t >= (int64_t)N_Q * B) return;
-    const int q = (int)(t / B);
-    const int64_t b = t % B;
+// use_net = 0 keeps the linear part only.  One workgroup row per q (blockIdx.y), so l and the K x K block
+// J_l are wave-uniform (scalar loads); the radial size K is a template parameter so d and the products
+// stay in registers.
+template <int K>
+__global__ __launch_bounds__(ET) void post_kernel(const double *__restrict__ f, const double *__restrict__ nvec,
+                                                  const double *__restrict__ svec, const float *__restrict__ Y,
+                                                  const float *__restrict__ mu, const float *__restrict__ s,
+                                                  const double *__restrict__ JL, double *__restrict__ Q, int N_Q,
+                                                  int64_t B, int use_net) {
+    const int q = blockIdx.y;
+    const int64_t b = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (b >= B) return;
     int l = 0;
     while ((l + 1) * (l + 1) <= q) ++l;
     const double n = nvec[b], rn = 1.0 / n;
     const double s2 = use_net ? svec[b] * svec[b] : 0.0;
-    const int64_t ks = (int64_t)N_Q * B;
-    double d[16];
+    const int64_t ks = (int64_t)N_Q * B, t = (int64_t)q * B + b;
+    double d[K];
+#pragma unroll
     for (int k = 0; k < K; ++k) d[k] = f[k * ks + t] * rn - ((k == 0 && q == 0) ? 1.0 : 0.0);
     const double *J = JL + (int64_t)l * K * K;
+#pragma unroll
     for (int k1 = 0; k1 < K; ++k1) {
         double v = 0.0;
+#pragma unroll
         for (int k = 0; k < K; ++k) v = fma(J[k1 * K + k], d[k], v);
         if (use_net) {
             const int64_t i = (int64_t)k1 * N_Q + q;
@@ -118,6 +130,22 @@ __global__ void post_kernel(const double *__restrict__ f, const double *__restri
         }
         Q[k1 * ks + t] = invariant_slot(q, k1) ? 0.0 : n * n * v;
     }
+}
+
+int launch_post(int K, const double *f, const double *nvec, const double *svec, const float *Y, const float *mu,
+                const float *s, const double *JL, double *Q, int N_Q, int64_t B, int use_net, hipStream_t st) {
+    if (B <= 0) return CFK_OK;
+    const dim3 grid((unsigned)((B + ET - 1) / ET), (unsigned)N_Q);
+    switch (K) {
+#define CFK_POST(KK) \
+    case KK: hipLaunchKernelGGL(post_kernel<KK>, grid, dim3(ET), 0, st, f, nvec, svec, Y, mu, s, JL, Q, N_Q, B, use_net); break;
+        CFK_POST(1) CFK_POST(2) CFK_POST(3) CFK_POST(4) CFK_POST(5) CFK_POST(6) CFK_POST(7) CFK_POST(8)
+        CFK_POST(9) CFK_POST(10) CFK_POST(11) CFK_POST(12) CFK_POST(13) CFK_POST(14) CFK_POST(15) CFK_POST(16)
+#undef CFK_POST
+        default: return set_error(CFK_EUNSUP, "surrogate: K=%d unsupported (1..16)", K);
+    }
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
 }
 
 // Training pairs: d = f/n - e1 and r = Q/n^2 - J d (both FP64, [NK][n] like f and Q)
@@ -277,7 +305,7 @@ struct cfk_surrogate_s {
     int apply(double *Q, const double *f, int64_t B, bool use_net, hipStream_t st) {
         int rc;
         if ((rc = ensure(B))) return rc;
-        hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((B + 63) / 64)), dim3(ET), 0, st, f, NK, B, nvec.p, svec.p);
+        hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((B + 63) / 64)), dim3(64 * DN), 0, st, f, NK, B, nvec.p, svec.p);
         CFK_HIP(hipGetLastError());
         if (use_net) {
             if ((rc = mlp.ensure_batch(B))) return rc;
@@ -288,8 +316,9 @@ struct cfk_surrogate_s {
             if ((rc = mlp.forward(B, false, st))) return rc;
             if ((rc = transpose_f32(mlp.Y.p, mlp.p_out, Yt.p, B, B, NK, B, st))) return rc;
         }
-        hipLaunchKernelGGL(post_kernel, dim3(nb((int64_t)N_Q * B)), dim3(ET), 0, st, f, nvec.p, svec.p,
-                           use_net ? Yt.p : nullptr, mu_y.p, s_y.p, JL.p, Q, K, N_Q, B, use_net ? 1 : 0);
+        if ((rc = launch_post(K, f, nvec.p, svec.p, use_net ? Yt.p : nullptr, mu_y.p, s_y.p, JL.p, Q, N_Q, B,
+                              use_net ? 1 : 0, st)))
+            return rc;
         CFK_HIP(hipGetLastError());
         return CFK_OK;
     }
@@ -413,7 +442,7 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
         return rc;
     hipLaunchKernelGGL(target_kernel, dim3(nb((int64_t)s->N_Q * n)), dim3(ET), 0, st, f_dev, Q_dev, s->JL.p, d.p, r.p, s->K,
                        s->N_Q, n);
-    hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((n + 63) / 64)), dim3(ET), 0, st, f_dev, NK, n, nv.p, sv.p);
+    hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64 * DN), 0, st, f_dev, NK, n, nv.p, sv.p);
     CFK_HIP(hipGetLastError());
     std::vector<double> hm(NK), hs(NK);
     auto stats = [&](const double *X, int power, bool global, std::vector<float> &mu_o, std::vector<float> &s_o,
