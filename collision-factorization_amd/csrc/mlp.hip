@@ -158,9 +158,7 @@ constexpr int PSTAGE = PT * BK;  // bf16 elements of one operand per stage (8 KB
 __device__ __forceinline__ int pipe_slot(int r, int c) { return 4 * r + (c ^ ((r >> 2) & 3)); }
 
 // The ring's stages are separate LDS arrays and the main loop is unrolled over them, so every load and
-// every operand read names its stage at compile time: the compiler then sees that the reads of a stage
-// do not alias the loads in flight into the others (it still waits for all of them once per round of
-// four stages, at the loop's back edge).
+// every operand read names its stage at compile time.
 __shared__ __align__(1024) __bf16 pipe_a0[PSTAGE], pipe_a1[PSTAGE], pipe_a2[PSTAGE], pipe_a3[PSTAGE];
 __shared__ __align__(1024) __bf16 pipe_b0[PSTAGE], pipe_b1[PSTAGE], pipe_b2[PSTAGE], pipe_b3[PSTAGE];
 
@@ -205,13 +203,20 @@ __global__ __launch_bounds__(GT) void gemm_nt_pipe_kernel(const __bf16 *__restri
         ga[j] = A + (int64_t)gm * lda + 8 * c;                       // (their products are never stored)
         gb[j] = B + (int64_t)gn * ldb + 8 * c;
     }
+    // One 16-byte-per-lane load straight into LDS: M0 holds the wave's LDS destination, lane L lands at
+    // M0 + 16 L.  Written as assembly (M0 bound as an input operand) because the compiler, unable to tell
+    // the ring's stages apart, would otherwise wait for every load in flight before each operand read; the
+    // waits are explicit below instead.  The instruction has no destination register: its address
+    // registers are read at issue.
+    auto dma = [&](const __bf16 *src, __bf16 *dst) {
+        const uint32_t lds = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) void *)dst;
+        asm volatile("global_load_lds_dwordx4 %0, off" ::"v"(src), "{m0}"(__builtin_amdgcn_readfirstlane(lds)) : "memory");
+    };
     auto issue = [&](__bf16 *as, __bf16 *bs, int k0) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(ga[j] + k0),
-                                             (__attribute__((address_space(3))) void *)(as + (2 * wave + j) * 512), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(gb[j] + k0),
-                                             (__attribute__((address_space(3))) void *)(bs + (2 * wave + j) * 512), 16, 0, 0);
+            dma(ga[j] + k0, as + (2 * wave + j) * 512);
+            dma(gb[j] + k0, bs + (2 * wave + j) * 512);
         }
     };
     f32x4 acc[4][4];
@@ -223,23 +228,28 @@ __global__ __launch_bounds__(GT) void gemm_nt_pipe_kernel(const __bf16 *__restri
     // on l % 16 only, because block bases are multiples of 16
     const int r16 = lane % 16, cq = lane / 16;
     const int rd = 4 * r16 + (cq ^ ((r16 >> 2) & 3));  // slot within a 16-row block
-    auto mult = [&](const __bf16 *as, const __bf16 *bs) {
-        bf16x8 af[4], bfr[4];
+    // Operand fragments of two stages in registers: the reads of stage s + 1 are issued before the MFMAs
+    // of stage s, which do not wait for them, so the LDS latency hides behind the matrix work.
+    bf16x8 fa[2][4], fb[2][4];
+    auto frags = [&](const __bf16 *as, const __bf16 *bs, bf16x8 (&a)[4], bf16x8 (&b)[4]) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8 *>(as + ((wm + 16 * i) * 4 + rd) * 8);
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(as + ((wm + 16 * i) * 4 + rd) * 8);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const bf16x8 *>(bs + ((wn + 16 * j) * 4 + rd) * 8);
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8 *>(bs + ((wn + 16 * j) * 4 + rd) * 8);
+    };
+    auto mult = [&](const bf16x8 (&a)[4], const bf16x8 (&b)[4]) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     };
-    // Stage s has landed once at most the loads of the younger in-flight stages are outstanding (4 loads
-    // per stage and thread; loads retire in order); the barrier then makes every wave's part visible, and
-    // the slot refilled after it was last read in stage s - 1, which every wave has finished.
-    auto wait_stage = [&](int s, int nk) {
-        const int younger = min(nk - 1 - s, PS - 2);
+    // Stage t has landed once at most the loads of the younger stages in flight (at most `ymax` of them)
+    // are outstanding (4 loads per stage and thread; loads retire in order); the barrier then makes every
+    // wave's part visible.
+    const int nk = K / BK;
+    auto wait_stage = [&](int t, int ymax) {
+        const int younger = min(nk - 1 - t, ymax);
         if (younger >= 2)
             __builtin_amdgcn_s_waitcnt(0x0F70 | 8);
         else if (younger == 1)
@@ -247,25 +257,33 @@ __global__ __launch_bounds__(GT) void gemm_nt_pipe_kernel(const __bf16 *__restri
         else
             __builtin_amdgcn_s_waitcnt(0x0F70);
         __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");  // no operand read moves above the wait and the barrier
     };
-    const int nk = K / BK;
-    if (0 < nk) issue(pipe_a<0>(), pipe_b<0>(), 0);
+    if (nk <= 0) return;
+    issue(pipe_a<0>(), pipe_b<0>(), 0);
     if (1 < nk) issue(pipe_a<1>(), pipe_b<1>(), BK);
     if (2 < nk) issue(pipe_a<2>(), pipe_b<2>(), 2 * BK);
-#define CFK_PIPE_STEP(U, REFILL)                                                    \
+    wait_stage(0, 2);
+    frags(pipe_a<0>(), pipe_b<0>(), fa[0], fb[0]);
+    // Step s: stage s + 1 is waited for, stage s + 3 is requested into the slot of stage s - 1 (read in step
+    // s - 2, before the barrier of step s - 1), the fragments of s + 1 are read, and stage s is multiplied.
+    // The last step waits for nothing in flight and reads a stale slot into fragments it never uses: no
+    // branch around the waits and reads, so the compiler can track which reads each MFMA depends on.
+#define CFK_PIPE_STEP(U, NEXT, REFILL)                                              \
     {                                                                               \
         const int s = s0 + U;                                                       \
         if (s < nk) {                                                               \
-            wait_stage(s, nk);                                                      \
+            wait_stage(s + 1, 1);                                                   \
             if (s + 3 < nk) issue(pipe_a<REFILL>(), pipe_b<REFILL>(), (s + 3) * BK); \
-            mult(pipe_a<U>(), pipe_b<U>());                                         \
+            frags(pipe_a<NEXT>(), pipe_b<NEXT>(), fa[(U + 1) % 2], fb[(U + 1) % 2]); \
+            mult(fa[U % 2], fb[U % 2]);                                             \
         }                                                                           \
     }
     for (int s0 = 0; s0 < nk; s0 += 4) {
-        CFK_PIPE_STEP(0, 3)
-        CFK_PIPE_STEP(1, 0)
-        CFK_PIPE_STEP(2, 1)
-        CFK_PIPE_STEP(3, 2)
+        CFK_PIPE_STEP(0, 1, 3)
+        CFK_PIPE_STEP(1, 2, 0)
+        CFK_PIPE_STEP(2, 3, 1)
+        CFK_PIPE_STEP(3, 0, 2)
     }
 #undef CFK_PIPE_STEP
 #pragma unroll
