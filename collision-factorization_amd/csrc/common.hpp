@@ -78,6 +78,7 @@ struct MultiGroupDev {
     const int32_t *row_mid;   // row split points (matrix-core kernel with the split)
     int64_t B;
     int N_Q, K;
+    int accumulate;  // the group's plan's output mode: Q += Q(f) (1) or Q = Q(f) (0)
 };
 constexpr int MULTI_MAX_GROUPS = 8;
 // Largest batch per call: the contraction kernels read f through a buffer resource, which spans

@@ -523,8 +523,8 @@ __global__ __launch_bounds__(WAVE) void multi_af_kernel(const MultiGroupDev *__r
     if (it >= i1) return;
     const int4 w = items[it];
     const MultiGroupDev &G = groups[w.x];
-    af_item<float, K, KC, true, true, 1, false, KC == K>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.R32, G.B, G.N_Q, 1,
-                                                w.y, (int64_t)w.z);
+    af_item<float, K, KC, true, true, 1, false, KC == K>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.R32, G.B, G.N_Q,
+                                                G.accumulate, w.y, (int64_t)w.z);
 }
 
 bool multi_supports_k(int K) { return K == 6 || K == 8 || K == 10 || K == 12 || K == 14; }

@@ -464,7 +464,7 @@ __global__ __launch_bounds__(2 * WAVE) void multi_mx_kernel(const MultiGroupDev 
     const int4 w = items[it];
     const MultiGroupDev &G = groups[w.x];
     const int half = SPLIT ? __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE) : -1;
-    af_mx_item<K, mx_kc(K), SPLIT>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.Rmx, G.B, G.N_Q, 1, w.y,
+    af_mx_item<K, mx_kc(K), SPLIT>(G.fT, G.Q, G.slices, G.row_ptr, G.zq, G.zg, G.R, G.Rmx, G.B, G.N_Q, G.accumulate, w.y,
                                    (int64_t)w.z, G.row_mid, half, mx_lds);
 }
 

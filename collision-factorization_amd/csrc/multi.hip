@@ -38,6 +38,7 @@ struct cfk_multi_s {
     std::vector<const double *> key_f;
     std::vector<double *> key_Q;
     std::vector<int64_t> key_B;
+    std::vector<int> key_acc;  // each plan's output mode (cfk_plan_set_accumulate) at capture
     DevBuf<int4> items;
     DevBuf<MultiGroupDev> groups;
     bool use_graph = true;
@@ -138,7 +139,7 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
         if ((rc = m->f32.back()->alloc((size_t)std::max<int64_t>(n, 1)))) return rc;
         gd[g] = MultiGroupDev{m->f32.back()->p, Q[g], p->sym_slices.p, p->sym_row_ptr.p, p->sym_zq.p,
                               p->sym_zg32.p, p->sym_R.p, p->sym_R32.p, mx ? p->sym_Rmx.p : nullptr,
-                              p->sym_row_mid.p, B[g], p->N_Q, p->K};
+                              p->sym_row_mid.p, B[g], p->N_Q, p->K, p->accumulate};
     }
     if ((rc = m->groups.upload(gd.data(), gd.size()))) return rc;
     const int nc = (int)cls.size();
@@ -152,6 +153,8 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
     m->key_f.assign(f, f + G);
     m->key_Q.assign(Q, Q + G);
     m->key_B.assign(B, B + G);
+    m->key_acc.resize(G);
+    for (int g = 0; g < G; ++g) m->key_acc[g] = m->plans[g]->accumulate;
     m->prepared = true;
     return CFK_OK;
 }
@@ -246,7 +249,9 @@ int cfk_multi_execute(cfk_multi m, double *const *Q_dev, const double *const *f_
     if (dev != m->device) return set_error(CFK_EINVAL, "multi plan lives on device %d, current is %d", m->device, dev);
     const bool same = m->prepared && std::equal(m->key_f.begin(), m->key_f.end(), f_dev) &&
                       std::equal(m->key_Q.begin(), m->key_Q.end(), Q_dev) &&
-                      std::equal(m->key_B.begin(), m->key_B.end(), B);
+                      std::equal(m->key_B.begin(), m->key_B.end(), B) &&
+                      std::equal(m->key_acc.begin(), m->key_acc.end(), m->plans.begin(),
+                                 [](int a, const cfk_plan_s *p) { return a == p->accumulate; });
     int rc;
     if (!same && (rc = multi_prepare(m, Q_dev, f_dev, B))) return rc;
     if (!m->use_graph) return multi_enqueue(m, (hipStream_t)stream);

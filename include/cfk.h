@@ -291,12 +291,14 @@ int cfk_multi_create(const cfk_plan *plans, int32_t n_groups, cfk_multi *out);
 int cfk_multi_destroy(cfk_multi multi);
 
 /* Q_dev[g] += Q(f_dev[g]) for B[g] distributions of group g (batch-fastest layout [K][N_Q][B],
- * FP64 in and out, mixed precision inside, invariants projected).  The work is split into
+ * FP64 in and out, mixed precision inside, invariants projected); Q_dev[g] = Q(f_dev[g]) without
+ * reading Q_dev[g] for a group whose plan is in overwrite mode (cfk_plan_set_accumulate(plan, 0)).  The work is split into
  * (group, row, batch tile) items, ordered heaviest first within each radial size K; one kernel per
  * K runs its items (one workgroup per item, dispatched in that order), the kernels of different K
  * on parallel graph branches.  The conversions and the kernels are captured into a hipGraph on the
  * first call for a given set of pointers and sizes; later calls with the same set replay the graph
- * on `stream`.  Each B[g] obeys cfk_plan_execute's per-call limit. */
+ * on `stream` (a change of a plan's output mode captures anew).  Each B[g] obeys cfk_plan_execute's
+ * per-call limit. */
 int cfk_multi_execute(cfk_multi multi, double *const *Q_dev, const double *const *f_dev, const int64_t *B,
                       void *stream);
 

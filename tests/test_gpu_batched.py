@@ -175,6 +175,38 @@ def test_multi_full_config2_mix(groups, torch_cuda):
     mp.close()
 
 
+def test_multi_overwrite_mode(groups, torch_cuda):
+    """With every plan in overwrite mode the multi plan writes Q = Q(f) without reading Q (Q starts as
+    NaN) and gives the rows it gives when accumulating into zeros; switching the mode back re-captures."""
+    torch = torch_cuda
+    keys = list(CONFIG2)
+    fs, Bs = [], [37, 64, 5, 70]
+    for i, k in enumerate(keys):
+        fs.append(_batch(torch, groups[k], Bs[i], 40 + i)[1])
+    mp = cfk.MultiPlan([groups[k]["plan"] for k in keys])
+    try:
+        Qa = [torch.zeros_like(f) for f in fs]
+        mp.execute(Qa, fs, Bs)
+        for k in keys:
+            groups[k]["plan"].set_accumulate(False)
+        Qo = [torch.full_like(f, float("nan")) for f in fs]
+        mp.execute(Qo, fs, Bs)
+        mp.execute(Qo, fs, Bs)   # a replay overwrites again
+        torch.cuda.synchronize()
+        for i, k in enumerate(keys):
+            assert np.array_equal(_rows(Qa[i]), _rows(Qo[i])), k
+    finally:
+        for k in keys:
+            groups[k]["plan"].set_accumulate(True)
+    Qb = [torch.zeros_like(f) for f in fs]
+    mp.execute(Qb, fs, Bs)
+    mp.execute(Qb, fs, Bs)      # accumulating again: twice the result
+    torch.cuda.synchronize()
+    for i, k in enumerate(keys):
+        assert np.allclose(_rows(Qb[i]), 2 * _rows(Qa[i]), rtol=0, atol=1e-12 * np.abs(_rows(Qa[i])).max()), k
+    mp.close()
+
+
 def test_multi_rejects_unsupported_groups(torch_cuda):
     lab, val, ic = cfk.real_gaunt(2)
     ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
