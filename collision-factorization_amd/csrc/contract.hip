@@ -745,6 +745,10 @@ void launch_af_v(const SparseTables<T> &tb, int N_Q, int wpb, bool split, const 
 
 // k3 chunk candidates: the full width and half of it.
 constexpr int kc_half(int K) { return (K + 1) / 2; }
+// The partial chunk of a tile: half the width, for FP64 from K = 12 a third (at K = 14 half a chunk of
+// FP64 Phi alone takes 196 VGPRs and leaves one wave per SIMD).
+template <typename T>
+constexpr int kc_part(int K) { return (std::is_same<T, double>::value && K >= 12) ? (K + 2) / 3 : kc_half(K); }
 // Default chunk: full width while the Phi accumulators stay within ~100 values.
 constexpr int kc_default(int K) { return (K * K <= 100) ? K : kc_half(K); }
 
@@ -770,12 +774,12 @@ void launch_af_k(const SparseTables<T> &tb, int N_Q, const Tile &tl, const T *fT
     } while (0)
         if constexpr (GROUPED) {
             (void)pf;
-            if (full) CFK_AF_NB(K, false); else CFK_AF_NB(kc_half(K), false);
+            if (full) CFK_AF_NB(K, false); else CFK_AF_NB(kc_part<T>(K), false);
         } else {
             if (full) {
                 if (pf) CFK_AF_NB(K, true); else CFK_AF_NB(K, false);
             } else {
-                if (pf) CFK_AF_NB(kc_half(K), true); else CFK_AF_NB(kc_half(K), false);
+                if (pf) CFK_AF_NB(kc_part<T>(K), true); else CFK_AF_NB(kc_part<T>(K), false);
             }
         }
 #undef CFK_AF_NB

@@ -34,6 +34,7 @@ struct cfk_multi_s {
     hipEvent_t ev[MULTI_MAX_GROUPS + 1] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t launched = nullptr;  // recorded after each graph launch
     // the pointer/size set the current graph was captured for
     std::vector<const double *> key_f;
     std::vector<double *> key_Q;
@@ -53,13 +54,18 @@ struct cfk_multi_s {
     int n_items = 0;
     ~cfk_multi_s() {
         reset();
+        if (launched) (void)hipEventDestroy(launched);
         if (cap) (void)hipStreamDestroy(cap);
         for (auto &b : branch)
             if (b) (void)hipStreamDestroy(b);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
     }
+    // Destroys the captured graph.  A launch of it may still be running (execute returns as soon as
+    // the graph is enqueued): wait for the last launch first, since destroying an executable graph
+    // in flight crashes the runtime.
     void reset() {
+        if (exec && launched) (void)hipEventSynchronize(launched);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
@@ -257,6 +263,8 @@ int cfk_multi_execute(cfk_multi m, double *const *Q_dev, const double *const *f_
     if (!m->use_graph) return multi_enqueue(m, (hipStream_t)stream);
     if (!m->exec && (rc = multi_capture(m))) return rc;
     CFK_HIP(hipGraphLaunch(m->exec, (hipStream_t)stream));
+    if (!m->launched) CFK_HIP(hipEventCreateWithFlags(&m->launched, hipEventDisableTiming));
+    CFK_HIP(hipEventRecord(m->launched, (hipStream_t)stream));
     return CFK_OK;
 }
 
