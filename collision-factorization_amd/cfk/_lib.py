@@ -105,6 +105,7 @@ SIGNATURES = {
     "cfk_mlp_adam_step": ([_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp], ctypes.c_int),
     "cfk_project_maxwellians": ([_vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _vp], ctypes.c_int),
     "cfk_surrogate_create": ([_vp, _i32, _i32, ctypes.c_uint64, ctypes.POINTER(_vp)], ctypes.c_int),
+    "cfk_surrogate_create_ex": ([_vp, _i32, _i32, ctypes.c_uint64, _i32, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_surrogate_destroy": ([_vp], ctypes.c_int),
     "cfk_surrogate_mlp": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
     "cfk_surrogate_linear_blocks": ([_vp, _dp], ctypes.c_int),

@@ -143,18 +143,23 @@ def split_params(p, n_in, width, depth, n_out):
 class Surrogate:
     """Q~(f) = n^2 (J d + N(d)) with the exact linear part and a trained network (cfk_surrogate_*)."""
 
-    def __init__(self, plan, width=1024, depth=3, seed=0):
+    def __init__(self, plan, width=1024, depth=3, seed=0, anchored=False):
+        """anchored=True: decomposition about each cell's own Maxwellian (CFK_SURROGATE_ANCHORED in
+        include/cfk.h), the network seeing 2 K N_Q + 1 inputs."""
         self.plan = plan   # the plan must outlive the surrogate
         self.K, self.N_Q = plan.K, plan.N_Q
         self.L = int(round(math.sqrt(plan.N_Q))) - 1
         self.NK = self.K * self.N_Q
         self.width, self.depth = width, depth
+        self.anchored = bool(anchored)
+        self.n_in = 2 * self.NK + 1 if self.anchored else self.NK
         h = _vp()
-        check(lib().cfk_surrogate_create(plan._h, int(width), int(depth), ctypes.c_uint64(seed), ctypes.byref(h)))
+        check(lib().cfk_surrogate_create_ex(plan._h, int(width), int(depth), ctypes.c_uint64(seed),
+                                            1 if self.anchored else 0, ctypes.byref(h)))
         self._h = h
         m = _vp()
         check(lib().cfk_surrogate_mlp(self._h, ctypes.byref(m)))
-        self.mlp = Mlp(handle=m, owner=self).shape_from(self.NK, width, depth, self.NK)
+        self.mlp = Mlp(handle=m, owner=self).shape_from(self.n_in, width, depth, self.NK)
 
     def linear_blocks(self):
         out = np.empty((self.L + 1, self.K, self.K))
@@ -162,7 +167,7 @@ class Surrogate:
         return out
 
     def normalization(self):
-        arrs = [np.empty(self.NK, dtype=np.float32) for _ in range(4)]
+        arrs = [np.empty(n, dtype=np.float32) for n in (self.n_in, self.n_in, self.NK, self.NK)]
         fp = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for a in arrs]
         check(lib().cfk_surrogate_get_normalization(self._h, *fp))
         return dict(zip(("mu_x", "inv_s_x", "mu_y", "s_y"), arrs))

@@ -14,6 +14,8 @@
 //   Q~(f) = n^2 ( J d + s^2 N(d / s) ),
 // then the FP64 conservation projection sets the five invariant coefficients of Q~ to zero.
 // Network inputs and targets are standardized per feature with statistics of the training set.
+// A second decomposition, about each cell's own Maxwellian, is described with its kernels below
+// (CFK_SURROGATE_ANCHORED).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -148,6 +150,159 @@ int launch_post(int K, const double *f, const double *nvec, const double *svec, 
     return CFK_OK;
 }
 
+// ---- Decomposition about each cell's own Maxwellian (CFK_SURROGATE_ANCHORED) ----------------------
+// Mass, momentum and energy of a cell are collision invariants, so its Maxwellian M (same n, u, T) is
+// fixed along a relaxation.  With c_M the basis coefficients of M and g = (f - c_M) / n:
+//   Q(f) = Q(c_M) + n^2 ( J g + R ),   R = 2 B(d_M, g) + Q(g),   d_M = c_M / n - e1,
+// B the symmetric bilinear form of Q.  Q(c_M) is computed exactly once per cell (it is not zero for the
+// truncated basis); the network sees (d_M, g / |g|, log |g|) and predicts R / |g|, which stays finite as
+// the cell approaches equilibrium (R is linear in g there), so the surrogate's error shrinks with |g|
+// instead of staying at a fixed fraction of the unit-Maxwellian remainder.
+
+// Maxwellian parameters (n, u_x, u_y, u_z, T, T, T) of each cell from its moments [B][CFK_N_MOMENTS]
+__global__ void maxwellian_params_kernel(const double *__restrict__ mom, double *__restrict__ par, int64_t B) {
+    const int64_t b = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (b >= B) return;
+    const double *o = mom + b * CFK_N_MOMENTS;
+    double *q = par + b * 7;
+    q[0] = o[0];
+    q[1] = o[1];
+    q[2] = o[2];
+    q[3] = o[3];
+    q[4] = q[5] = q[6] = o[4];
+}
+
+// Per cell: n = f[0], g = (f - c_M) / n (written), |g| (floored at G_MIN); layout as dev_norm_kernel
+constexpr double G_MIN = 1e-12;
+__global__ __launch_bounds__(64 * DN) void gap_kernel(const double *__restrict__ f, const double *__restrict__ cM, int NK,
+                                                      int64_t B, double *__restrict__ gap, double *__restrict__ nvec,
+                                                      double *__restrict__ gnorm) {
+    __shared__ double red[DN][64];
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    const int64_t b = (int64_t)blockIdx.x * 64 + tx;
+    double s2 = 0.0, n = 1.0;
+    if (b < B) {
+        n = f[b];
+        const double in = 1.0 / n;
+#pragma unroll 4
+        for (int i = ty; i < NK; i += DN) {
+            const int64_t x = (int64_t)i * B + b;
+            const double g = (f[x] - cM[x]) * in;
+            gap[x] = g;
+            s2 = fma(g, g, s2);
+        }
+    }
+    red[ty][tx] = s2;
+    __syncthreads();
+    if (ty == 0 && b < B) {
+        double t = 0.0;
+#pragma unroll
+        for (int r = 0; r < DN; ++r) t += red[r][tx];
+        nvec[b] = n;
+        gnorm[b] = fmax(sqrt(t), G_MIN);
+    }
+}
+
+// The last input feature: out[b][col] = (log |g|_b - mu) * inv_s (zero padding after it stays zero)
+template <typename TO>
+__global__ void log_feature_kernel(const double *__restrict__ gnorm, TO *__restrict__ out, int64_t ld, int col,
+                                   float mu, float inv_s, int64_t B) {
+    const int64_t b = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (b < B) out[b * ld + col] = (TO)(((float)log(gnorm[b]) - mu) * inv_s);
+}
+
+// Training targets: y = ((Q - Q(c_M)) / n^2 - J g) / |g|, FP64 [NK][B]
+__global__ void anchored_target_kernel(const double *__restrict__ gap, const double *__restrict__ nvec,
+                                       const double *__restrict__ gnorm, const double *__restrict__ Qx,
+                                       const double *__restrict__ QM, const double *__restrict__ JL,
+                                       double *__restrict__ yout, int K, int N_Q, int64_t B) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)N_Q * B) return;
+    const int q = (int)(t / B);
+    const int64_t b = t % B;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= q) ++l;
+    const double n = nvec[b], ig = 1.0 / gnorm[b];
+    const int64_t ks = (int64_t)N_Q * B;
+    const double *J = JL + (int64_t)l * K * K;
+    for (int k1 = 0; k1 < K; ++k1) {
+        double v = 0.0;
+        for (int k = 0; k < K; ++k) v = fma(J[k1 * K + k], gap[k * ks + t], v);
+        const int64_t x = k1 * ks + t;
+        yout[x] = ((Qx[x] - QM[x]) / (n * n) - v) * ig;
+    }
+}
+
+// Q = Q(c_M) + n^2 (J g + |g| (Y s + mu)), invariants zeroed; use_net = 0 keeps Q(c_M) + n^2 J g.
+template <int K>
+__global__ __launch_bounds__(ET) void anchored_post_kernel(const double *__restrict__ gap, const double *__restrict__ nvec,
+                                                           const double *__restrict__ gnorm, const float *__restrict__ Y,
+                                                           const float *__restrict__ mu, const float *__restrict__ s,
+                                                           const double *__restrict__ JL, const double *__restrict__ QM,
+                                                           double *__restrict__ Q, int N_Q, int64_t B, int use_net) {
+    const int q = blockIdx.y;
+    const int64_t b = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (b >= B) return;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= q) ++l;
+    const double n = nvec[b], gs = use_net ? gnorm[b] : 0.0;
+    const int64_t ks = (int64_t)N_Q * B, t = (int64_t)q * B + b;
+    double g[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) g[k] = gap[k * ks + t];
+    const double *J = JL + (int64_t)l * K * K;
+#pragma unroll
+    for (int k1 = 0; k1 < K; ++k1) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) v = fma(J[k1 * K + k], g[k], v);
+        const int64_t x = k1 * ks + t;
+        if (use_net) {
+            const int64_t i = (int64_t)k1 * N_Q + q;
+            v += gs * ((double)Y[x] * (double)s[i] + (double)mu[i]);
+        }
+        Q[x] = invariant_slot(q, k1) ? 0.0 : QM[x] + n * n * v;
+    }
+}
+
+int launch_anchored_post(int K, const double *gap, const double *nvec, const double *gnorm, const float *Y,
+                         const float *mu, const float *s, const double *JL, const double *QM, double *Q, int N_Q,
+                         int64_t B, int use_net, hipStream_t st) {
+    if (B <= 0) return CFK_OK;
+    const dim3 grid((unsigned)((B + ET - 1) / ET), (unsigned)N_Q);
+    switch (K) {
+#define CFK_APOST(KK)                                                                                                \
+    case KK:                                                                                                         \
+        hipLaunchKernelGGL(anchored_post_kernel<KK>, grid, dim3(ET), 0, st, gap, nvec, gnorm, Y, mu, s, JL, QM, Q, N_Q, \
+                           B, use_net);                                                                              \
+        break;
+        CFK_APOST(1) CFK_APOST(2) CFK_APOST(3) CFK_APOST(4) CFK_APOST(5) CFK_APOST(6) CFK_APOST(7) CFK_APOST(8)
+        CFK_APOST(9) CFK_APOST(10) CFK_APOST(11) CFK_APOST(12) CFK_APOST(13) CFK_APOST(14) CFK_APOST(15) CFK_APOST(16)
+#undef CFK_APOST
+        default: return set_error(CFK_EUNSUP, "surrogate: K=%d unsupported (1..16)", K);
+    }
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+// d_M = c_M / c_M[0] - e1, FP64 [NK][B] (training statistics of the first input block)
+__global__ void rel_kernel(const double *__restrict__ cM, double *__restrict__ dM, int NK, int64_t B) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)NK * B) return;
+    const int64_t b = t % B;
+    dM[t] = cM[t] / cM[b] - (t < B ? 1.0 : 0.0);
+}
+
+// dst[i][j] = src[i][idx[j]] (rows x m), src row length n
+__global__ void gather_cols_kernel(const double *__restrict__ src, const int32_t *__restrict__ idx, double *__restrict__ dst,
+                                   int rows, int64_t n, int64_t m) {
+    const int64_t t = (int64_t)blockIdx.x * ET + threadIdx.x;
+    if (t >= (int64_t)rows * m) return;
+    const int i = (int)(t / m);
+    const int64_t j = t % m;
+    dst[t] = src[(int64_t)i * n + idx[j]];
+}
+
 // Training pairs: d = f/n - e1 and r = Q/n^2 - J d (both FP64, [NK][n] like f and Q)
 __global__ void target_kernel(const double *__restrict__ f, const double *__restrict__ Qx, const double *__restrict__ JL,
                               double *__restrict__ dout, double *__restrict__ rout, int K, int N_Q, int64_t B) {
@@ -266,6 +421,8 @@ using namespace cfk;
 struct cfk_surrogate_s {
     cfk_plan_s *plan = nullptr;
     int L = 0, K = 0, N_Q = 0, NK = 0;
+    bool anchored = false;  // decomposition about each cell's own Maxwellian (CFK_SURROGATE_ANCHORED)
+    int n_in = 0;           // network inputs: NK, or 2 NK + 1 anchored (d_M, g / |g|, log |g|)
     cfk_mlp_s mlp;
     std::vector<double> JL_host;        // [(L+1)][K][K]
     DevBuf<double> JL;
@@ -275,6 +432,8 @@ struct cfk_surrogate_s {
     // scratch
     DevBuf<double> nvec, svec, Qx, cs, Qs, acc;
     DevBuf<float> Yt;  // network output, feature-major [NK][B]
+    // anchored decomposition: g, |g|, each cell's c_M and Q(c_M), moments and Maxwellian parameters
+    DevBuf<double> gap, aM, aQ, mom, par;
     int64_t cap = 0;
 
     int ensure(int64_t B) {
@@ -283,15 +442,53 @@ struct cfk_surrogate_s {
         if ((rc = nvec.alloc(B)) || (rc = svec.alloc(B)) || (rc = Qx.alloc((size_t)NK * B)) ||
             (rc = Yt.alloc((size_t)NK * B)))
             return rc;
+        if (anchored && ((rc = gap.alloc((size_t)NK * B)) || (rc = aM.alloc((size_t)NK * B)) ||
+                         (rc = aQ.alloc((size_t)NK * B)) || (rc = mom.alloc((size_t)CFK_N_MOMENTS * B)) ||
+                         (rc = par.alloc((size_t)7 * B))))
+            return rc;
         cap = B;
         return CFK_OK;
     }
     int upload_norm() {
         int rc;
-        if ((rc = mu_x.upload(h_mu_x.data(), NK)) || (rc = is_x.upload(h_is_x.data(), NK)) ||
+        if ((rc = mu_x.upload(h_mu_x.data(), n_in)) || (rc = is_x.upload(h_is_x.data(), n_in)) ||
             (rc = mu_y.upload(h_mu_y.data(), NK)) || (rc = s_y.upload(h_s_y.data(), NK)))
             return rc;
         return CFK_OK;
+    }
+    // Each cell's Maxwellian (same n, u, T; the tutorials' projection grid) and its exact Q, into cM and
+    // QM; mom and par hold B x CFK_N_MOMENTS and B x 7 values.
+    int anchor(double *cM, double *QM, const double *f, int64_t B, double *momb, double *parb, hipStream_t st) {
+        int rc;
+        if ((rc = cfk_moments(momb, f, B, N_Q, K, CFK_LAYOUT_BATCH_FASTEST, st))) return rc;
+        hipLaunchKernelGGL(maxwellian_params_kernel, dim3(nb(B)), dim3(ET), 0, st, momb, parb, B);
+        CFK_HIP(hipGetLastError());
+        if ((rc = cfk_project_maxwellians(cM, parb, B, 1, L, K, 60, 30, 30, st))) return rc;
+        return exact(QM, cM, B, st);
+    }
+    // Q = Q~(f) with the anchors (cM, QM) of f's cells
+    int apply_anchored(double *Q, const double *f, const double *cM, const double *QM, int64_t B, bool use_net,
+                       hipStream_t st) {
+        int rc;
+        if ((rc = ensure(B))) return rc;
+        hipLaunchKernelGGL(gap_kernel, dim3((unsigned)((B + 63) / 64)), dim3(64 * DN), 0, st, f, cM, NK, B, gap.p, nvec.p,
+                           svec.p);
+        CFK_HIP(hipGetLastError());
+        if (use_net) {
+            if ((rc = mlp.ensure_batch(B))) return rc;
+            const dim3 grid((unsigned)((B + 31) / 32), (unsigned)((NK + 31) / 32));
+            hipLaunchKernelGGL(norm_transpose_kernel<__bf16>, grid, dim3(ET), 0, st, cM, mu_x.p, is_x.p, mlp.X.p,
+                               (int64_t)mlp.p_in, NK, B, 1, (const double *)nullptr, 0);
+            hipLaunchKernelGGL(norm_transpose_kernel<__bf16>, grid, dim3(ET), 0, st, gap.p, mu_x.p + NK, is_x.p + NK,
+                               mlp.X.p + NK, (int64_t)mlp.p_in, NK, B, 0, svec.p, 1);
+            hipLaunchKernelGGL(log_feature_kernel<__bf16>, dim3(nb(B)), dim3(ET), 0, st, svec.p, mlp.X.p, (int64_t)mlp.p_in,
+                               2 * NK, h_mu_x[2 * NK], h_is_x[2 * NK], B);
+            CFK_HIP(hipGetLastError());
+            if ((rc = mlp.forward(B, false, st))) return rc;
+            if ((rc = transpose_f32(mlp.Y.p, mlp.p_out, Yt.p, B, B, NK, B, st))) return rc;
+        }
+        return launch_anchored_post(K, gap.p, nvec.p, svec.p, use_net ? Yt.p : nullptr, mu_y.p, s_y.p, JL.p, QM, Q, N_Q, B,
+                                    use_net ? 1 : 0, st);
     }
     // exact operator (FP64, symmetrized tables) with the conservation projection: Q = Q(f)
     int exact(double *Q, const double *f, int64_t B, hipStream_t st) {
@@ -305,6 +502,10 @@ struct cfk_surrogate_s {
     int apply(double *Q, const double *f, int64_t B, bool use_net, hipStream_t st) {
         int rc;
         if ((rc = ensure(B))) return rc;
+        if (anchored) {
+            if ((rc = anchor(aM.p, aQ.p, f, B, mom.p, par.p, st))) return rc;
+            return apply_anchored(Q, f, aM.p, aQ.p, B, use_net, st);
+        }
         hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((B + 63) / 64)), dim3(64 * DN), 0, st, f, NK, B, nvec.p, svec.p);
         CFK_HIP(hipGetLastError());
         if (use_net) {
@@ -320,6 +521,48 @@ struct cfk_surrogate_s {
                               use_net ? 1 : 0, st)))
             return rc;
         CFK_HIP(hipGetLastError());
+        return CFK_OK;
+    }
+    // Epochs of shuffled minibatches of the normalized set (X [n][p_in] BF16, T [n][p_out] FP32), Adam with
+    // a cosine learning-rate schedule from lr to lr_final
+    int train(const __bf16 *Xall, const float *Tall, int64_t n, int epochs, int batch, float lr, float lr_final,
+              uint64_t seed, double *loss_hist, hipStream_t st) {
+        int rc;
+        const int p_in = mlp.p_in, p_out = mlp.p_out;
+        if ((rc = mlp.ensure_batch(batch))) return rc;
+        std::vector<int32_t> perm(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::mt19937_64 rng(seed);
+        DevBuf<int32_t> dperm;
+        if ((rc = dperm.alloc(n))) return rc;
+        const int64_t steps_per_epoch = n / batch;  // the remainder of each shuffle is dropped
+        const int64_t total = steps_per_epoch * epochs;
+        int64_t step = 0;
+        for (int ep = 0; ep < epochs; ++ep) {
+            std::shuffle(perm.begin(), perm.end(), rng);
+            CFK_HIP(hipMemcpyAsync(dperm.p, perm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+            if ((rc = fill_f64(mlp.loss_acc.p, 0.0, 1, st))) return rc;
+            for (int64_t bi = 0; bi < steps_per_epoch; ++bi, ++step) {
+                const int64_t cols = batch;
+                hipLaunchKernelGGL(gather_rows_kernel<__bf16>, dim3(nb(cols * p_in)), dim3(ET), 0, st, Xall, dperm.p,
+                                   bi * batch, mlp.X.p, cols, p_in);
+                hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(nb(cols * p_out)), dim3(ET), 0, st, Tall, dperm.p,
+                                   bi * batch, mlp.T.p, cols, p_out);
+                CFK_HIP(hipGetLastError());
+                if ((rc = mlp.forward(cols, true, st)) || (rc = mlp.backward(cols, st))) return rc;
+                const double frac = total > 1 ? (double)step / (double)(total - 1) : 1.0;
+                const float lr_t = (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(M_PI * frac)));
+                if ((rc = mlp.adam(lr_t, 0.9f, 0.999f, 1e-8f, st))) return rc;
+            }
+            if (loss_hist) {
+                double h = 0.0;
+                CFK_HIP(hipMemcpyAsync(&h, mlp.loss_acc.p, sizeof(double), hipMemcpyDeviceToHost, st));
+                CFK_HIP(hipStreamSynchronize(st));
+                loss_hist[ep] = h / ((double)NK * (double)batch * (double)std::max<int64_t>(steps_per_epoch, 1));
+            }
+        }
+        CFK_HIP(hipStreamSynchronize(st));
+        trained = true;
         return CFK_OK;
     }
 };
@@ -374,7 +617,13 @@ int build_linear(cfk_surrogate_s *s) {
 extern "C" {
 
 int cfk_surrogate_create(cfk_plan plan, int32_t width, int32_t depth, uint64_t seed, cfk_surrogate *out) {
+    return cfk_surrogate_create_ex(plan, width, depth, seed, 0, out);
+}
+
+int cfk_surrogate_create_ex(cfk_plan plan, int32_t width, int32_t depth, uint64_t seed, int32_t flags,
+                            cfk_surrogate *out) {
     if (!out || !plan) return set_error(CFK_EINVAL, "null argument");
+    if (flags & ~CFK_SURROGATE_ANCHORED) return set_error(CFK_EINVAL, "unknown surrogate flags %d", flags);
     *out = nullptr;
     if (plan->K > 16) return set_error(CFK_EUNSUP, "surrogate supports K <= 16 (K=%d)", plan->K);
     if (!plan->sym_ok) return set_error(CFK_EUNSUP, "plan without symmetrized tables: %s", plan->sym_why.c_str());
@@ -387,10 +636,12 @@ int cfk_surrogate_create(cfk_plan plan, int32_t width, int32_t depth, uint64_t s
     s->NK = plan->N_Q * plan->K;
     s->L = (int)std::lround(std::sqrt((double)plan->N_Q)) - 1;
     if ((s->L + 1) * (s->L + 1) != s->N_Q) return set_error(CFK_EINVAL, "N_Q=%d is not a square", s->N_Q);
-    if ((rc = s->mlp.create(s->NK, width, depth, s->NK, seed))) return rc;
+    s->anchored = (flags & CFK_SURROGATE_ANCHORED) != 0;
+    s->n_in = s->anchored ? 2 * s->NK + 1 : s->NK;
+    if ((rc = s->mlp.create(s->n_in, width, depth, s->NK, seed))) return rc;
     if ((rc = build_linear(s.get()))) return rc;
-    s->h_mu_x.assign(s->NK, 0.0f);
-    s->h_is_x.assign(s->NK, 1.0f);
+    s->h_mu_x.assign(s->n_in, 0.0f);
+    s->h_is_x.assign(s->n_in, 1.0f);
     s->h_mu_y.assign(s->NK, 0.0f);
     s->h_s_y.assign(s->NK, 1.0f);
     if ((rc = s->upload_norm())) return rc;
@@ -435,19 +686,16 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
     const int NK = s->NK;
     const int64_t tot = (int64_t)NK * n;
     int rc;
-    // 1. d and r in FP64, per-cell scale s = |d|, per-feature statistics of d / s and r / s^2
-    DevBuf<double> d, r, mean, sd, nv, sv;
-    if ((rc = d.alloc(tot)) || (rc = r.alloc(tot)) || (rc = mean.alloc(NK)) || (rc = sd.alloc(NK)) || (rc = nv.alloc(n)) ||
-        (rc = sv.alloc(n)))
-        return rc;
-    hipLaunchKernelGGL(target_kernel, dim3(nb((int64_t)s->N_Q * n)), dim3(ET), 0, st, f_dev, Q_dev, s->JL.p, d.p, r.p, s->K,
-                       s->N_Q, n);
-    hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64 * DN), 0, st, f_dev, NK, n, nv.p, sv.p);
-    CFK_HIP(hipGetLastError());
+    const int p_in = s->mlp.p_in, p_out = s->mlp.p_out;
+    DevBuf<__bf16> Xall;  // sample-major [n][p_in], zero padded
+    DevBuf<float> Tall;   // sample-major [n][p_out]
+    DevBuf<double> mean, sd;
+    if ((rc = mean.alloc(NK)) || (rc = sd.alloc(NK))) return rc;
     std::vector<double> hm(NK), hs(NK);
-    auto stats = [&](const double *X, int power, bool global, std::vector<float> &mu_o, std::vector<float> &s_o,
+    // per-row mean and standard deviation of X[i][b] / sv[b]^power over the m samples -> (mu, scale)
+    auto stats = [&](const double *X, int64_t m, const double *svp, int power, bool global, float *mu_o, float *s_o,
                      bool inverse) -> int {
-        hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)NK), dim3(ET), 0, st, X, mean.p, sd.p, n, sv.p, power);
+        hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)NK), dim3(ET), 0, st, X, mean.p, sd.p, m, svp, power);
         CFK_HIP(hipGetLastError());
         CFK_HIP(hipMemcpyAsync(hm.data(), mean.p, NK * sizeof(double), hipMemcpyDeviceToHost, st));
         CFK_HIP(hipMemcpyAsync(hs.data(), sd.p, NK * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -465,8 +713,95 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
         }
         return CFK_OK;
     };
-    if ((rc = stats(d.p, 1, (flags & CFK_FIT_GLOBAL_INPUT_SCALE) != 0, s->h_mu_x, s->h_is_x, true)) ||
-        (rc = stats(r.p, 2, (flags & CFK_FIT_GLOBAL_TARGET_SCALE) != 0, s->h_mu_y, s->h_s_y, false)))
+    if (s->anchored) {
+        // 1. each sample's anchor (c_M, Q(c_M)), g and |g|; samples that are their own Maxwellian (|g| ~ 0:
+        // one-component mixtures) carry no direction g / |g| and are left out
+        DevBuf<double> cM, QM, gp, nv, gn, momb, parb;
+        if ((rc = cM.alloc(tot)) || (rc = QM.alloc(tot)) || (rc = gp.alloc(tot)) || (rc = nv.alloc(n)) ||
+            (rc = gn.alloc(n)) || (rc = momb.alloc((size_t)CFK_N_MOMENTS * n)) || (rc = parb.alloc((size_t)7 * n)))
+            return rc;
+        if ((rc = s->anchor(cM.p, QM.p, f_dev, n, momb.p, parb.p, st))) return rc;
+        momb.release();
+        parb.release();
+        hipLaunchKernelGGL(gap_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64 * DN), 0, st, f_dev, cM.p, NK, n, gp.p, nv.p,
+                           gn.p);
+        CFK_HIP(hipGetLastError());
+        std::vector<double> hg(n);
+        CFK_HIP(hipMemcpyAsync(hg.data(), gn.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        CFK_HIP(hipStreamSynchronize(st));
+        std::vector<int32_t> keep;
+        for (int64_t j = 0; j < n; ++j)
+            if (hg[j] > 1e-9) keep.push_back((int32_t)j);
+        const int64_t m = (int64_t)keep.size();
+        if (m < std::max<int64_t>(2, batch)) return set_error(CFK_EINVAL, "only %lld samples differ from their Maxwellian", (long long)m);
+        DevBuf<int32_t> dk;
+        DevBuf<double> cMk, QMk, gpk, Qk, nvk, gnk;
+        const int64_t totk = (int64_t)NK * m;
+        if ((rc = dk.upload(keep.data(), keep.size())) || (rc = cMk.alloc(totk)) || (rc = QMk.alloc(totk)) ||
+            (rc = gpk.alloc(totk)) || (rc = Qk.alloc(totk)) || (rc = nvk.alloc(m)) || (rc = gnk.alloc(m)))
+            return rc;
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nb(totk)), dim3(ET), 0, st, cM.p, dk.p, cMk.p, NK, n, m);
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nb(totk)), dim3(ET), 0, st, QM.p, dk.p, QMk.p, NK, n, m);
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nb(totk)), dim3(ET), 0, st, gp.p, dk.p, gpk.p, NK, n, m);
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nb(totk)), dim3(ET), 0, st, Q_dev, dk.p, Qk.p, NK, n, m);
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nb(m)), dim3(ET), 0, st, nv.p, dk.p, nvk.p, 1, n, m);
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(nb(m)), dim3(ET), 0, st, gn.p, dk.p, gnk.p, 1, n, m);
+        CFK_HIP(hipGetLastError());
+        CFK_HIP(hipStreamSynchronize(st));
+        cM.release();
+        QM.release();
+        gp.release();
+        // 2. targets y = ((Q - Q(c_M)) / n^2 - J g) / |g| and the statistics of the three input blocks
+        DevBuf<double> y, dM;
+        if ((rc = y.alloc(totk)) || (rc = dM.alloc(totk))) return rc;
+        hipLaunchKernelGGL(anchored_target_kernel, dim3(nb((int64_t)s->N_Q * m)), dim3(ET), 0, st, gpk.p, nvk.p, gnk.p,
+                           Qk.p, QMk.p, s->JL.p, y.p, s->K, s->N_Q, m);
+        hipLaunchKernelGGL(rel_kernel, dim3(nb(totk)), dim3(ET), 0, st, cMk.p, dM.p, NK, m);
+        CFK_HIP(hipGetLastError());
+        const bool gin = (flags & CFK_FIT_GLOBAL_INPUT_SCALE) != 0;
+        if ((rc = stats(dM.p, m, nullptr, 0, gin, s->h_mu_x.data(), s->h_is_x.data(), true)) ||
+            (rc = stats(gpk.p, m, gnk.p, 1, gin, s->h_mu_x.data() + NK, s->h_is_x.data() + NK, true)) ||
+            (rc = stats(y.p, m, nullptr, 0, (flags & CFK_FIT_GLOBAL_TARGET_SCALE) != 0, s->h_mu_y.data(), s->h_s_y.data(),
+                        false)))
+            return rc;
+        double lm = 0.0, lv = 0.0;
+        for (int32_t j : keep) lm += std::log(hg[j]);
+        lm /= (double)m;
+        for (int32_t j : keep) lv += (std::log(hg[j]) - lm) * (std::log(hg[j]) - lm);
+        const double lsd = std::sqrt(lv / (double)m);
+        s->h_mu_x[2 * NK] = (float)lm;
+        s->h_is_x[2 * NK] = (float)(lsd > 0.0 ? 1.0 / lsd : 1.0);
+        std::vector<float> inv_sy(NK);
+        for (int i = 0; i < NK; ++i) inv_sy[i] = 1.0f / s->h_s_y[i];
+        if ((rc = s->upload_norm())) return rc;
+        DevBuf<float> isy;
+        if ((rc = isy.upload(inv_sy.data(), NK))) return rc;
+        // 3. normalized training arrays
+        if ((rc = Xall.alloc((size_t)m * p_in)) || (rc = Tall.alloc((size_t)m * p_out)) ||
+            (rc = fill_bf16(Xall.p, m * p_in, st)) || (rc = fill_f32(Tall.p, 0.0f, m * p_out, st)))
+            return rc;
+        const dim3 tgrid((unsigned)((m + 31) / 32), (unsigned)((NK + 31) / 32));
+        hipLaunchKernelGGL(norm_transpose_kernel<__bf16>, tgrid, dim3(ET), 0, st, cMk.p, s->mu_x.p, s->is_x.p, Xall.p,
+                           (int64_t)p_in, NK, m, 1, (const double *)nullptr, 0);
+        hipLaunchKernelGGL(norm_transpose_kernel<__bf16>, tgrid, dim3(ET), 0, st, gpk.p, s->mu_x.p + NK, s->is_x.p + NK,
+                           Xall.p + NK, (int64_t)p_in, NK, m, 0, gnk.p, 1);
+        hipLaunchKernelGGL(log_feature_kernel<__bf16>, dim3(nb(m)), dim3(ET), 0, st, gnk.p, Xall.p, (int64_t)p_in, 2 * NK,
+                           s->h_mu_x[2 * NK], s->h_is_x[2 * NK], m);
+        hipLaunchKernelGGL(norm_transpose_kernel<float>, tgrid, dim3(ET), 0, st, y.p, s->mu_y.p, isy.p, Tall.p,
+                           (int64_t)p_out, NK, m, 0, (const double *)nullptr, 0);
+        CFK_HIP(hipGetLastError());
+        CFK_HIP(hipStreamSynchronize(st));
+        return s->train(Xall.p, Tall.p, m, epochs, batch, lr, lr_final, seed, loss_hist, st);
+    }
+    // 1. d and r in FP64, per-cell scale s = |d|, per-feature statistics of d / s and r / s^2
+    DevBuf<double> d, r, nv, sv;
+    if ((rc = d.alloc(tot)) || (rc = r.alloc(tot)) || (rc = nv.alloc(n)) || (rc = sv.alloc(n))) return rc;
+    hipLaunchKernelGGL(target_kernel, dim3(nb((int64_t)s->N_Q * n)), dim3(ET), 0, st, f_dev, Q_dev, s->JL.p, d.p, r.p, s->K,
+                       s->N_Q, n);
+    hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64 * DN), 0, st, f_dev, NK, n, nv.p, sv.p);
+    CFK_HIP(hipGetLastError());
+    if ((rc = stats(d.p, n, sv.p, 1, (flags & CFK_FIT_GLOBAL_INPUT_SCALE) != 0, s->h_mu_x.data(), s->h_is_x.data(), true)) ||
+        (rc = stats(r.p, n, sv.p, 2, (flags & CFK_FIT_GLOBAL_TARGET_SCALE) != 0, s->h_mu_y.data(), s->h_s_y.data(), false)))
         return rc;
     // target normalization uses 1/s on the device: keep both forms
     std::vector<float> inv_sy(NK);
@@ -475,9 +810,6 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
     DevBuf<float> isy;
     if ((rc = isy.upload(inv_sy.data(), NK))) return rc;
     // 2. normalized training arrays
-    DevBuf<__bf16> Xall;  // sample-major [n][p_in], zero padded
-    DevBuf<float> Tall;   // sample-major [n][p_out]
-    const int p_in = s->mlp.p_in, p_out = s->mlp.p_out;
     if ((rc = Xall.alloc((size_t)n * p_in)) || (rc = Tall.alloc((size_t)n * p_out)) ||
         (rc = fill_bf16(Xall.p, n * p_in, st)) || (rc = fill_f32(Tall.p, 0.0f, n * p_out, st)))
         return rc;
@@ -490,42 +822,7 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
     CFK_HIP(hipStreamSynchronize(st));
     d.release();
     r.release();
-    // 3. epochs of shuffled minibatches, Adam with a cosine learning-rate schedule
-    if ((rc = s->mlp.ensure_batch(batch))) return rc;
-    std::vector<int32_t> perm(n);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::mt19937_64 rng(seed);
-    DevBuf<int32_t> dperm;
-    if ((rc = dperm.alloc(n))) return rc;
-    const int64_t steps_per_epoch = n / batch;  // the remainder of each shuffle is dropped
-    const int64_t total = steps_per_epoch * epochs;
-    int64_t step = 0;
-    for (int ep = 0; ep < epochs; ++ep) {
-        std::shuffle(perm.begin(), perm.end(), rng);
-        CFK_HIP(hipMemcpyAsync(dperm.p, perm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        if ((rc = fill_f64(s->mlp.loss_acc.p, 0.0, 1, st))) return rc;
-        for (int64_t bi = 0; bi < steps_per_epoch; ++bi, ++step) {
-            const int64_t cols = batch;
-            hipLaunchKernelGGL(gather_rows_kernel<__bf16>, dim3(nb(cols * p_in)), dim3(ET), 0, st, Xall.p, dperm.p,
-                               bi * batch, s->mlp.X.p, cols, p_in);
-            hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(nb(cols * p_out)), dim3(ET), 0, st, Tall.p, dperm.p,
-                               bi * batch, s->mlp.T.p, cols, p_out);
-            CFK_HIP(hipGetLastError());
-            if ((rc = s->mlp.forward(cols, true, st)) || (rc = s->mlp.backward(cols, st))) return rc;
-            const double frac = total > 1 ? (double)step / (double)(total - 1) : 1.0;
-            const float lr_t = (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(M_PI * frac)));
-            if ((rc = s->mlp.adam(lr_t, 0.9f, 0.999f, 1e-8f, st))) return rc;
-        }
-        if (loss_hist) {
-            double h = 0.0;
-            CFK_HIP(hipMemcpyAsync(&h, s->mlp.loss_acc.p, sizeof(double), hipMemcpyDeviceToHost, st));
-            CFK_HIP(hipStreamSynchronize(st));
-            loss_hist[ep] = h / ((double)NK * (double)batch * (double)std::max<int64_t>(steps_per_epoch, 1));
-        }
-    }
-    CFK_HIP(hipStreamSynchronize(st));
-    s->trained = true;
-    return CFK_OK;
+    return s->train(Xall.p, Tall.p, n, epochs, batch, lr, lr_final, seed, loss_hist, st);
 }
 
 int cfk_surrogate_apply(cfk_surrogate s, double *Q_dev, const double *f_dev, int64_t B, int32_t use_network,
@@ -552,6 +849,13 @@ int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, in
         if ((rc = s->cs.ensure((size_t)NK * nsmax)) || (rc = s->Qs.ensure((size_t)NK * nsmax)) || (rc = s->acc.ensure(2)))
             return rc;
     }
+    // anchored decomposition: each cell's Maxwellian and its exact Q once (the cells' invariants do not change)
+    if (s->anchored && mode != CFK_RELAX_EXACT && (rc = s->anchor(s->aM.p, s->aQ.p, c_dev, B, s->mom.p, s->par.p, st)))
+        return rc;
+    auto approx = [&](bool net) {
+        return s->anchored ? s->apply_anchored(s->Qx.p, c_dev, s->aM.p, s->aQ.p, B, net, st)
+                           : s->apply(s->Qx.p, c_dev, B, net, st);
+    };
     cfk_relax_stats out{};
     int skip = 0, backoff = 1;  // after a failed check the surrogate rests for `backoff` steps (doubling, <= 64)
     for (int step = 0; step < n_steps; ++step) {
@@ -562,10 +866,10 @@ int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, in
             ++out.steps_exact;
             if (skip > 0) --skip;
         } else if (mode == CFK_RELAX_LINEAR) {
-            if ((rc = s->apply(s->Qx.p, c_dev, B, false, st))) return rc;
+            if ((rc = approx(false))) return rc;
             ++out.steps_network;
         } else {
-            if ((rc = s->apply(s->Qx.p, c_dev, B, true, st))) return rc;
+            if ((rc = approx(true))) return rc;
             // residual check on every sample_stride-th cell, offset rotating with the step
             const int64_t off = step % sample_stride;
             const int64_t ns = (B - off + sample_stride - 1) / sample_stride;

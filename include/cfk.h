@@ -354,12 +354,24 @@ typedef struct cfk_surrogate_s *cfk_surrogate;
 /* Builds J from the plan's FP64 operator (2 K N_Q evaluations) and an untrained network
  * K N_Q -> width x depth -> K N_Q.  The plan must outlive the surrogate. */
 int cfk_surrogate_create(cfk_plan plan, int32_t width, int32_t depth, uint64_t seed, cfk_surrogate *out);
+/* As cfk_surrogate_create; flags CFK_SURROGATE_ANCHORED selects the decomposition about each cell's own
+ * Maxwellian c_M (same n, u, T; projected on the grid of cfk_project_maxwellians with 60 x 30 x 30
+ * nodes): with g = (f - c_M) / n and d_M = c_M / n - e1,
+ *   Q~(f) = Q(c_M) + n^2 (J g + |g| N(d_M, g / |g|, log |g|)),
+ * Q(c_M) exact (FP64).  The network then has 2 K N_Q + 1 inputs.  Its error shrinks as a cell approaches
+ * its Maxwellian, where the unit-Maxwellian form keeps a fixed fraction of the remainder Q(d).  Each
+ * apply computes the anchors (c_M, Q(c_M)) of its cells; a relaxation computes them once (mass,
+ * momentum and energy are invariants). */
+#define CFK_SURROGATE_ANCHORED 1
+int cfk_surrogate_create_ex(cfk_plan plan, int32_t width, int32_t depth, uint64_t seed, int32_t flags,
+                            cfk_surrogate *out);
 int cfk_surrogate_destroy(cfk_surrogate s);
 /* The surrogate's network (owned by the surrogate) for parameter I/O. */
 int cfk_surrogate_mlp(cfk_surrogate s, cfk_mlp *out);
 /* Host copy of the linear blocks J_l[k1][k], (L+1) K K values. */
 int cfk_surrogate_linear_blocks(cfk_surrogate s, double *JL_host);
-/* Host copies of the per-feature input mean and inverse std and target mean and std (K N_Q each). */
+/* Host copies of the per-feature input mean and inverse std (one per network input: K N_Q, or
+ * 2 K N_Q + 1 anchored) and target mean and std (K N_Q each). */
 int cfk_surrogate_get_normalization(cfk_surrogate s, float *mu_x, float *inv_s_x, float *mu_y, float *s_y);
 /* Trains on n pairs (f_dev, Q_dev = Q(f)): per-feature standardization from the set, `epochs`
  * passes of shuffled minibatches of `batch` pairs (seeded), Adam with a cosine learning-rate decay
@@ -372,13 +384,13 @@ int cfk_surrogate_get_normalization(cfk_surrogate s, float *mu_x, float *inv_s_x
 int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev, int64_t n, int32_t epochs,
                       int32_t batch, float lr, float lr_final, uint64_t seed, int32_t flags, double *loss_hist,
                       void *stream);
-/* Q_dev = Q~(f_dev) (overwrites).  use_network = 0 evaluates n^2 J d only. */
+/* Q_dev = Q~(f_dev) (overwrites).  use_network = 0 evaluates n^2 J d only (anchored: Q(c_M) + n^2 J g). */
 int cfk_surrogate_apply(cfk_surrogate s, double *Q_dev, const double *f_dev, int64_t B, int32_t use_network,
                         void *stream);
 
 #define CFK_RELAX_EXACT 0      /* exact FP64 operator every step */
 #define CFK_RELAX_SURROGATE 1  /* surrogate, sampled residual check, exact fallback */
-#define CFK_RELAX_LINEAR 2     /* n^2 J d only (no network, no check) */
+#define CFK_RELAX_LINEAR 2     /* the linear part only (no network, no check) */
 
 typedef struct cfk_relax_stats {
     int64_t steps_network;  /* steps advanced with the surrogate (or the linear part) */

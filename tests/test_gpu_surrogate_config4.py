@@ -16,6 +16,12 @@ What the network gains over the method without it (each asserted):
   than 25%;
 * conservation: mass, momentum and energy coefficients do not move (the projection zeroes them in
   every Q used).
+
+The decomposition about each cell's own Maxwellian (anchored) is trained on mixtures plus states along
+exact relaxations and asserted to keep the network in use near equilibrium: with the check at 10% it
+advances most steps on the network, more than the unit-Maxwellian form at the same tolerance, with the
+final state within 3% of the exact trajectory; its linear part alone (Q(c_M) + n^2 J g) stays within
+15% where the unit-Maxwellian linear part is off by more than 25%.
 """
 import time
 
@@ -58,7 +64,41 @@ def setup(torch_cuda):
     print("config4: training", flush=True)
     hist = sur.fit(f, Q, epochs=40, batch=1024, lr=1e-3, lr_final=1e-5, seed=3)
     del f, Q
-    return dict(torch=torch, plan=plan, sur=sur, hist=hist, fv=fv, Qv=Qv)
+    return dict(torch=torch, plan=plan, sur=sur, hist=hist, fv=fv, Qv=Qv, pairs=pairs)
+
+
+@pytest.fixture(scope="module")
+def anchored(setup):
+    """The anchored surrogate, trained on 1e5 mixtures plus 21 states (every 15 steps) along exact
+    relaxations of 12000 other mixtures."""
+    torch, plan = setup["torch"], setup["plan"]
+    sur = cfk.Surrogate(plan, width=1024, depth=3, seed=0, anchored=True)
+    dt = 0.005 / abs(sur.linear_blocks()[2][0][0])
+    fmix, Qmix = setup["pairs"](100000, 1)
+    c = cfk.project_maxwellians(cfk.maxwellian_mixture_params(12000, seed=7), L, K)
+    states = [c.clone()]
+    for s in range(300):
+        Qc = torch.zeros_like(c)
+        plan.execute(Qc, c, c.shape[1], precision="fp64_sym", layout="batch_fastest")
+        cfk.conservation_project(Qc, c.shape[1], NQ, K, layout="batch_fastest")
+        c = c + dt * Qc
+        if (s + 1) % 15 == 0:
+            states.append(c.clone())
+    ftr = torch.cat(states, dim=1).contiguous()
+    Qtr = torch.zeros_like(ftr)
+    for s0 in range(0, ftr.shape[1], 65536):
+        e = min(ftr.shape[1], s0 + 65536)
+        fs = ftr[:, s0:e].contiguous()
+        Qs = torch.zeros_like(fs)
+        plan.execute(Qs, fs, e - s0, precision="fp64_sym", layout="batch_fastest")
+        Qtr[:, s0:e] = Qs
+    cfk.conservation_project(Qtr, ftr.shape[1], NQ, K, layout="batch_fastest")
+    f = torch.cat([fmix, ftr], dim=1).contiguous()
+    Q = torch.cat([Qmix, Qtr], dim=1).contiguous()
+    print("config4 anchored: training on %d pairs" % f.shape[1], flush=True)
+    hist = sur.fit(f, Q, epochs=40, batch=1024, lr=1e-3, lr_final=1e-5, seed=3)
+    del f, Q, fmix, Qmix, ftr, Qtr
+    return dict(sur=sur, hist=hist, dt=dt)
 
 
 def _rel(a, b):
@@ -132,3 +172,40 @@ def test_relaxation_with_fallback(setup):
     inv = [0, 1, 2, 3, NQ]
     for c in (ce, cs, cl):
         assert float((c[inv] - c0[inv]).abs().max()) <= 1e-13
+
+
+def test_anchored_surrogate_near_equilibrium(setup, anchored):
+    torch, sur_u = setup["torch"], setup["sur"]
+    sur, dt = anchored["sur"], anchored["dt"]
+    h = anchored["hist"]
+    assert np.all(np.isfinite(h)) and h[-1] < 0.05 * h[0]
+    e_net = _rel(sur.apply(setup["fv"]), setup["Qv"])
+    e_lin = _rel(sur.apply(setup["fv"], use_network=False), setup["Qv"])
+    print("anchored held-out relative error: network %.3e, linear part only %.3e" % (e_net, e_lin), flush=True)
+    assert e_net < 0.05 and e_net < 0.2 * e_lin
+    B, steps, tol = 16384, 200, 0.1
+    c0 = cfk.project_maxwellians(cfk.maxwellian_mixture_params(B, seed=5), L, K)
+    runs = {}
+    for name, s, mode in (("exact", sur, "exact"), ("anchored", sur, "surrogate"), ("unit", sur_u, "surrogate"),
+                          ("anchored_linear", sur, "linear")):
+        c = c0.clone()
+        print("relaxation", name, flush=True)
+        st, res = s.relax(c, dt, steps, mode=mode, tol=tol, sample_stride=100)
+        runs[name] = (c, st, res)
+    ce, st_e, _ = runs["exact"]
+    change = float(torch.linalg.norm(ce - c0))
+    err = {k: float(torch.linalg.norm(v[0] - ce)) / change for k, v in runs.items()}
+    st_a, st_u = runs["anchored"][1], runs["unit"][1]
+    print("tol %g: anchored %d network steps, %d exact evals (exact stepping %d), final error %.3e; unit %d network "
+          "steps, final error %.3e; anchored linear part %.3e" % (tol, st_a["steps_network"], st_a["exact_evals"],
+                                                               st_e["exact_evals"], err["anchored"],
+                                                               st_u["steps_network"], err["unit"],
+                                                               err["anchored_linear"]), flush=True)
+    assert st_a["steps_network"] >= 120
+    assert st_a["steps_network"] > st_u["steps_network"]
+    assert st_a["exact_evals"] <= 0.35 * st_e["exact_evals"]
+    assert err["anchored"] < 0.03
+    assert err["anchored_linear"] < 0.15
+    inv = [0, 1, 2, 3, NQ]
+    for name in ("anchored", "anchored_linear"):
+        assert float((runs[name][0][inv] - c0[inv]).abs().max()) <= 1e-13

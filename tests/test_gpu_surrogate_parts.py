@@ -3,7 +3,10 @@
 * the linear blocks J_l of cfk_surrogate_create vs the dense linearization of the oracle
   angular-first contraction (restated reference MEX) at 1e-12, on the reference's L=4 fixture;
 * the linear-only surrogate n^2 J d vs numpy with those blocks (1e-12) and vs the exact operator
-  for a small perturbation (relative difference of the order of the perturbation).
+  for a small perturbation (relative difference of the order of the perturbation);
+* the anchored linear-only surrogate Q(c_M) + n^2 J g vs numpy with those blocks, c_M the projection of
+  each cell's Maxwellian (moments from cfk.moments) and Q(c_M) from the exact operator (1e-12), and vs
+  the exact operator near the Maxwellian (relative difference of the order of the deviation).
 """
 import math
 
@@ -91,3 +94,50 @@ def test_linear_surrogate(torch_cuda):
     Qx = Qx.cpu().numpy()
     Qx[[0, 1, 2, 3, nq]] = 0.0
     assert np.linalg.norm(Q - Qx) <= 1e-3 * np.linalg.norm(Qx)
+
+
+def test_anchored_linear_surrogate(torch_cuda):
+    torch = torch_cuda
+    L, K = 4, 5
+    nq = (L + 1) ** 2
+    (ls, vs, ics, R), plan = _fixture_plan(L, K)
+    sur = cfk.Surrogate(plan, width=64, depth=1, seed=0, anchored=True)
+    JL = sur.linear_blocks()
+    # single Maxwellians close to the unit one, so that the dropped 2 B(d_M, g) term stays small
+    params = cfk.maxwellian_mixture_params(29, seed=9, max_components=1, drift_max=0.05, temp_range=(0.95, 1.05))
+    cM = cfk.project_maxwellians(params, L, K)
+    rng = np.random.default_rng(6)
+    pert = np.zeros((K * nq, 29))
+    pert[4:] = 1e-4 * rng.standard_normal((K * nq - 4, 29))   # keep mass and momentum of the Maxwellian
+    pert[nq] = 0.0                                            # and its energy coefficient
+    fd = cM + torch.from_numpy(pert).cuda()
+    Q = sur.apply(fd, use_network=False).cpu().numpy()
+    # reference: the anchors from the moments of f (unchanged by the perturbation), Q(c_M) exact
+    mom = cfk.moments(fd, 29, nq, K, layout="batch_fastest").cpu().numpy()
+    par = np.zeros((29, 1, 7))
+    par[:, 0, 0], par[:, 0, 1:4], par[:, 0, 4:7] = mom[:, 0], mom[:, 1:4], mom[:, 4:5]
+    cMr = cfk.project_maxwellians(par, L, K)
+    QM = torch.zeros_like(cMr)
+    plan.execute(QM, cMr, 29, precision="fp64_sym", layout="batch_fastest")
+    QM = QM.cpu().numpy()
+    # the anchor carries the cell's mass, momentum and temperature
+    momM = cfk.moments(cMr, 29, nq, K, layout="batch_fastest").cpu().numpy()
+    assert np.max(np.abs(momM[:, :5] - mom[:, :5])) <= 1e-8
+    f, cMh = fd.cpu().numpy(), cMr.cpu().numpy()
+    n = f[0]
+    g = (f - cMh) / n
+    ref = QM.copy()
+    for q in range(nq):
+        l = int(math.isqrt(q))
+        idx = q + nq * np.arange(K)
+        ref[idx] += n ** 2 * (JL[l] @ g[idx])
+    ref[[0, 1, 2, 3, nq]] = 0.0
+    assert np.max(np.abs(Q - ref)) <= 1e-12 * np.max(np.abs(ref))
+    # against the exact operator: the dropped remainder 2 B(d_M, g) + Q(g) is small against the
+    # first-order change Q(f) - Q(c_M) for |d_M| ~ 0.05
+    Qx = torch.zeros_like(fd)
+    plan.execute(Qx, fd, 29, precision="fp64", layout="batch_fastest")
+    Qx = Qx.cpu().numpy()
+    Qx[[0, 1, 2, 3, nq]] = 0.0
+    QM[[0, 1, 2, 3, nq]] = 0.0
+    assert np.linalg.norm(Q - Qx) <= 0.5 * np.linalg.norm(Qx - QM)
