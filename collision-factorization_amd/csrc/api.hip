@@ -55,7 +55,7 @@ int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, doub
                       hipStream_t st);
 int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t cols, hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
-int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st);
+int launch_pack_rows(const double *in, float *out, int N_Q, int K, int KC, int64_t B, bool natural, hipStream_t st);
 bool af_f32_packed(int K, const Tile &tl);
 bool mx_supports_k(int K);
 int mx_chunk(int K);
@@ -167,6 +167,12 @@ static int build_sym_tables(const double *labels, const double *vals, const doub
         std::vector<int32_t> fill(row_ptr.begin(), row_ptr.end() - 1);
         for (const Slice &s : tq) sl[fill[s.q1]++] = s;
     }
+    // within a row the slices whose partner block is l = 0 (one per row) come last: the matrix-core
+    // kernel runs their FP64 radial stage after the row's other slices, so its FP64 sums are not held
+    // across the row
+    for (int r = 0; r < N_Q; ++r)
+        std::stable_partition(sl.begin() + row_ptr[r], sl.begin() + row_ptr[r + 1],
+                              [&](const Slice &x) { return trip[x.t][1] != 0; });
     std::vector<double> work(N_Q, 0.0);
     for (const Slice &s : tq) work[s.q1] += (double)(s.z1 - s.z0) * K * K + (double)K3;
     std::iota(order.begin(), order.end(), 0);
@@ -337,7 +343,8 @@ Tile plan_tile(const cfk_plan_s *p, int precision) {
 }
 
 // Device staging an execute needs, in elements: the FP32 copy of f (mixed precision: packed rows
-// [N_Q][B][KP] for kernels with a full k3 chunk, else [K][N_Q][B]) and, for the natural layout, the
+// [N_Q][B][RL] for the VALU kernel with a full k3 chunk and for the matrix-core kernel (chunk-packed),
+// else [K][N_Q][B]) and, for the natural layout, the
 // batch-fastest FP64 copies of f (FP64 precisions) and of Q.
 StagingNeed plan_staging_need(const cfk_plan_s *p, int precision, int layout, int64_t B) {
     StagingNeed n{0, 0, 0};
@@ -345,8 +352,9 @@ StagingNeed plan_staging_need(const cfk_plan_s *p, int precision, int layout, in
     const bool natural = layout == CFK_LAYOUT_NATURAL;
     if (precision == CFK_PREC_MIXED) {
         const Tile tl = plan_tile(p, precision);
-        const bool packed = tl.pf == 1 ? mx_chunk(p->K) == p->K : af_f32_packed(p->K, tl);
-        n.f32 = packed ? (int64_t)p->N_Q * ((p->K + 3) / 4 * 4) * B : NK * B;
+        const bool packed = tl.pf == 1 || af_f32_packed(p->K, tl);
+        const int kc = tl.pf == 1 ? mx_chunk(p->K) : p->K;
+        n.f32 = packed ? (int64_t)p->N_Q * rl_of(p->K, kc) * B : NK * B;
         if (natural) n.q64 = NK * B;
     } else if (natural) {
         n.f64 = NK * B;
@@ -396,8 +404,10 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
             if (stg) return set_error(CFK_EUNSUP, "matrix-core operand tables missing");
             if ((rc = ensure_mx_tables(p))) return rc;
         }
-        // kernels with a full k3 chunk read packed rows [N_Q][B][KP]; the others [K][N_Q][B]
-        const bool packed = mx ? mx_chunk(p->K) == p->K : af_f32_packed(p->K, tl);
+        // the matrix-core kernel and the VALU kernel with a full k3 chunk read packed rows [N_Q][B][RL]
+        // (chunk-packed for the matrix-core kernel's k3 chunks); the others [K][N_Q][B]
+        const bool packed = mx || af_f32_packed(p->K, tl);
+        const int pkc = mx ? mx_chunk(p->K) : p->K;
         if (!stg) {
             if ((rc = p->fT32.ensure((size_t)need.f32))) return rc;
             s_f32 = p->fT32.p;
@@ -407,7 +417,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
             }
         }
         if (natural) {
-            if ((rc = packed ? launch_pack_rows(f, s_f32, p->N_Q, p->K, B, true, st)
+            if ((rc = packed ? launch_pack_rows(f, s_f32, p->N_Q, p->K, pkc, B, true, st)
                              : launch_transpose_f32(f, s_f32, B, NK, st)))
                 return rc;
             KernelTimer tm(p, st, stg == nullptr);
@@ -417,7 +427,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
             tm.stop();
             return launch_transpose(s_q64, Q, NK, B, p->accumulate != 0, st);
         }
-        if ((rc = packed ? launch_pack_rows(f, s_f32, p->N_Q, p->K, B, false, st)
+        if ((rc = packed ? launch_pack_rows(f, s_f32, p->N_Q, p->K, pkc, B, false, st)
                          : launch_to_f32(f, s_f32, NK * B, st)))
             return rc;
         KernelTimer tm(p, st, stg == nullptr);

@@ -80,13 +80,21 @@ struct MultiGroupDev {
     int N_Q, K;
     int accumulate;  // the group's plan's output mode: Q += Q(f) (1) or Q = Q(f) (0)
 };
+// Packed FP32 rows of f.  A kernel with k3 chunks of width KC (NCH chunks) reads chunk c of row (q, b)
+// -- the coefficients k = c KC .. c KC + KC - 1, zero padded to KCP = KC rounded up to 4 -- at float
+// offset ((q NCH + c) B + b) KCP, so one chunk is KCP / 4 16-byte loads and a wave's 64 lanes read 64 KCP
+// contiguous floats.  With KC = K this is [N_Q][B][KP].  RL = NCH KCP floats per (q, b) in all.
+constexpr int kcp_of(int KC) { return (KC + 3) / 4 * 4; }
+constexpr int rl_of(int K, int KC) { return (K + KC - 1) / KC * kcp_of(KC); }
+int mx_chunk(int K);  // k3 chunk width of the matrix-core kernel
+
 constexpr int MULTI_MAX_GROUPS = 8;
 // Largest batch per call: the contraction kernels read f through a buffer resource, which spans
-// at most 4 GiB (8-byte coefficients counted), and the mixed kernel's packed FP32 rows (K rounded up
-// to 4 per row) take a signed 32-bit row offset.
+// at most 4 GiB (8-byte coefficients counted), and the mixed kernels' packed FP32 rows (K rounded up
+// to 4 per row; 16 slots for the chunk-packed rows from K = 11) take a signed 32-bit row offset.
 inline int64_t max_batch(int N_Q, int K) {
     const int64_t a = (((int64_t)1 << 32) - 1) / (8 * (int64_t)N_Q * K);
-    const int64_t b = (((int64_t)1 << 31) - 1) / (4 * (int64_t)N_Q * ((K + 3) / 4 * 4));
+    const int64_t b = (((int64_t)1 << 31) - 1) / (4 * (int64_t)N_Q * (K <= 10 ? (K + 3) / 4 * 4 : 16));
     return a < b ? a : b;
 }
 

@@ -208,8 +208,9 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                                     if constexpr (PK) {
                                         T f3[K];
                                         load_row(qv[e].y, i, f3);
+                                        const T gg = in_vgpr(gv[e]);
 #pragma unroll
-                                        for (int j = 0; j < KC; ++j) h[i][j] = fma(gv[e], f3[j], h[i][j]);
+                                        for (int j = 0; j < KC; ++j) h[i][j] = fma(gg, f3[j], h[i][j]);
                                     } else {
 #pragma unroll
                                         for (int j = 0; j < KC; ++j)
@@ -282,8 +283,9 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                                 if constexpr (PK) {
                                     T f3[K];
                                     load_row(qq.y, i, f3);
+                                    const T gg = in_vgpr(g);
 #pragma unroll
-                                    for (int j = 0; j < KC; ++j) h[i][j] = fma(g, f3[j], h[i][j]);
+                                    for (int j = 0; j < KC; ++j) h[i][j] = fma(gg, f3[j], h[i][j]);
                                 } else {
 #pragma unroll
                                     for (int j = 0; j < KC; ++j)
@@ -645,12 +647,14 @@ __global__ __launch_bounds__(256) void to_f32_kernel(const double *__restrict__ 
     if (i < n) out[i] = (float)in[i];
 }
 
-// FP64 f -> packed FP32 rows [N_Q][B][KP] (zero padded), one thread per (q, distribution) row.
-// NATURAL: input [B][K][N_Q] (B MATLAB N_Q x K arrays), thread index q fastest (coalesced reads);
-// otherwise batch-fastest [K][N_Q][B], thread index b fastest.
+// FP64 f -> packed FP32 rows: for k3 chunks of width KC (NCH chunks, KCP = KC rounded up to 4), chunk c
+// of row (q, b) is KCP floats at ((q NCH + c) B + b) KCP (coefficients c KC .. c KC + KC - 1, zero
+// padding), so a wave's load of one chunk of one row covers a contiguous 64 KCP floats.  One thread
+// per (q, distribution) row.  NATURAL: input [B][K][N_Q] (B MATLAB N_Q x K arrays), thread index q
+// fastest (coalesced reads); otherwise batch-fastest [K][N_Q][B], thread index b fastest.
 template <bool NATURAL>
 __global__ __launch_bounds__(256) void pack_rows_kernel(const double *__restrict__ in, float *__restrict__ out,
-                                                        int N_Q, int K, int64_t B) {
+                                                        int N_Q, int K, int KC, int64_t B) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)N_Q * B) return;
     int64_t q, b;
@@ -661,48 +665,51 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(const double *__restrict
         q = i / B;
         b = i - q * B;
     }
-    const int KP = kp_of(K);
-    float4 *o = (float4 *)(out + (q * B + b) * KP);
-    for (int c = 0; c < KP / 4; ++c) {
-        float v[4];
-        for (int r = 0; r < 4; ++r) {
-            const int k = 4 * c + r;
-            v[r] = k < K ? (float)(NATURAL ? in[(b * K + k) * N_Q + q] : in[((int64_t)k * N_Q + q) * B + b]) : 0.0f;
+    const int KCP = kcp_of(KC), NCH = (K + KC - 1) / KC;
+    for (int c = 0; c < NCH; ++c) {
+        float4 *o = (float4 *)(out + ((q * NCH + c) * B + b) * KCP);
+        for (int v4 = 0; v4 < KCP / 4; ++v4) {
+            float v[4];
+            for (int r = 0; r < 4; ++r) {
+                const int j = 4 * v4 + r, k = c * KC + j;
+                v[r] = (j < KC && k < K) ? (float)(NATURAL ? in[(b * K + k) * N_Q + q] : in[((int64_t)k * N_Q + q) * B + b])
+                                         : 0.0f;
+            }
+            o[v4] = make_float4(v[0], v[1], v[2], v[3]);
         }
-        o[c] = make_float4(v[0], v[1], v[2], v[3]);
     }
 }
 
 // Batch-fastest input with B even (and 16-byte aligned): one thread per (q, pair of distributions),
-// 16-byte loads of the two distributions' coefficient k, the two rows written as 2 KP / 4 float4.
+// one 16-byte load of the two distributions' coefficient k, each chunk of the two rows written as
+// 2 KCP / 4 float4.
 __global__ __launch_bounds__(256) void pack_rows_pair_kernel(const double *__restrict__ in, float *__restrict__ out,
-                                                             int N_Q, int K, int64_t B) {
+                                                             int N_Q, int K, int KC, int64_t B) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t half = B / 2;
     if (i >= (int64_t)N_Q * half) return;
     const int64_t q = i / half, b = 2 * (i - q * half);
-    const int KP = kp_of(K);
-    double2 v[16];
+    const int KCP = kcp_of(KC), NCH = (K + KC - 1) / KC;
+    for (int c = 0; c < NCH; ++c) {
+        float4 *o = reinterpret_cast<float4 *>(out + ((q * NCH + c) * B + b) * KCP);
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
-        if (k < K) v[k] = *reinterpret_cast<const double2 *>(in + ((int64_t)k * N_Q + q) * B + b);
-    float4 *o = reinterpret_cast<float4 *>(out + (q * B + b) * KP);
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (4 * c >= KP) break;
-            float w[4];
+        for (int v4 = 0; v4 < 4; ++v4) {
+            if (4 * v4 >= KCP) break;
+            float w0[4], w1[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int k = 4 * c + e;
-                w[e] = k < K ? (float)(r ? v[k].y : v[k].x) : 0.0f;
+                const int j = 4 * v4 + e, k = c * KC + j;
+                double2 v = make_double2(0.0, 0.0);
+                if (j < KC && k < K) v = *reinterpret_cast<const double2 *>(in + ((int64_t)k * N_Q + q) * B + b);
+                w0[e] = (float)v.x;
+                w1[e] = (float)v.y;
             }
-            o[r * (KP / 4) + c] = make_float4(w[0], w[1], w[2], w[3]);
+            o[v4] = make_float4(w0[0], w0[1], w0[2], w0[3]);
+            o[KCP / 4 + v4] = make_float4(w1[0], w1[1], w1[2], w1[3]);
         }
+    }
 }
 
-// Zero the five collision invariants of each distribution.
 __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int layout) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
@@ -878,17 +885,18 @@ int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t col
     return CFK_OK;
 }
 
-int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st) {
+int launch_pack_rows(const double *in, float *out, int N_Q, int K, int KC, int64_t B, bool natural, hipStream_t st) {
     const int64_t n = (int64_t)N_Q * B;
     if (n <= 0) return CFK_OK;
+    if (KC < 1 || KC > K || rl_of(K, KC) > 16) return set_error(CFK_EUNSUP, "packed rows: K=%d KC=%d", K, KC);
     const dim3 grid((unsigned)((n + 255) / 256));
     if (natural)
-        hipLaunchKernelGGL(pack_rows_kernel<true>, grid, dim3(256), 0, st, in, out, N_Q, K, B);
+        hipLaunchKernelGGL(pack_rows_kernel<true>, grid, dim3(256), 0, st, in, out, N_Q, K, KC, B);
     else if (B % 2 == 0 && K <= 16 && reinterpret_cast<uintptr_t>(in) % 16 == 0)
         hipLaunchKernelGGL(pack_rows_pair_kernel, dim3((unsigned)((n / 2 + 255) / 256)), dim3(256), 0, st, in, out,
-                           N_Q, K, B);
+                           N_Q, K, KC, B);
     else
-        hipLaunchKernelGGL(pack_rows_kernel<false>, grid, dim3(256), 0, st, in, out, N_Q, K, B);
+        hipLaunchKernelGGL(pack_rows_kernel<false>, grid, dim3(256), 0, st, in, out, N_Q, K, KC, B);
     CFK_HIP(hipGetLastError());
     return CFK_OK;
 }

@@ -55,12 +55,14 @@ struct MxShape {
     static constexpr int NCH = (K + KC - 1) / KC;
 };
 
-// LDS region of one wave, in doubles: the precise slices' FP64 sums [K][64], then the A operands of the
-// current chunk [NG][64] float4 (fetched straight into LDS at the chunk start).  With a row split the
-// second wave's region, free by then, carries its [16][64] partial sums to the first.
+// LDS region of one wave, in doubles: the A operands of the current chunk [NG][64] float4 (fetched
+// straight into LDS at the chunk start) while the matrix-core slices run, then the FP64 sums [K][64] of
+// the trailing precise slices.  With a row split the second wave's region, free by then, carries its
+// [16][64] partial sums to the first.
 template <int K, int KC, bool SPLIT>
 constexpr int mx_region_doubles() {
-    constexpr int r = K * WAVE + MxShape<K, KC>::NG * WAVE * 2;
+    constexpr int a = MxShape<K, KC>::NG * WAVE * 2, q = K * WAVE;
+    constexpr int r = a > q ? a : q;
     return SPLIT && r < 16 * WAVE ? 16 * WAVE : r;
 }
 
@@ -124,8 +126,9 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
                                            const int32_t *__restrict__ row_mid, int half, double *lds) {
     using S = MxShape<K, KC>;
     constexpr int K2 = K * K, K3 = K2 * K;
-    constexpr bool PK = KC == K;  // full chunk: packed FP32 rows [N_Q][B][KP]
-    constexpr int KP = kp_of(K);
+    constexpr bool FULL = KC == K;  // one k3 chunk
+    // f as packed FP32 rows: chunk ch of row (q, b) at float ((q NCH + ch) B + b) KCP
+    constexpr int KCP = kcp_of(KC), NCH = S::NCH;
     const int lane = threadIdx.x % WAVE;
     const gptr<double> QT = (gptr<double>)QT_in;
     const cptr<Slice> slices_c = (cptr<Slice>)slices;
@@ -135,18 +138,35 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
     const cptr<double> R_c = (cptr<double>)R;
     const int64_t b = tile * WAVE + lane;
     const bool act = b < B;
-    const uint32_t lo = act ? (uint32_t)b * (uint32_t)(PK ? KP * 4 : 4) : 0u;
-    const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B] when not packed
-    const frsrc_t frs = make_frsrc(fT_in, (uint32_t)((PK ? KP : K) * kstride * 4));
-    auto load_row = [&](int q, float (&dst)[K]) {
-        const int soff = q * (int)B * (KP * 4);
+    const uint32_t lo = act ? (uint32_t)b * (uint32_t)(KCP * 4) : 0u;
+    const frsrc_t frs = make_frsrc(fT_in, (uint32_t)((int64_t)N_Q * B * NCH * KCP * 4));
+    // the KC coefficients of chunk ch of row q (the f3 of an entry; zero past K)
+    auto load_chunk = [&](int q, int ch, float (&dst)[KC]) {
+        const int soff = (q * NCH + ch) * (int)B * (KCP * 4);
 #pragma unroll
-        for (int c = 0; c < KP / 4; ++c) {
+        for (int c = 0; c < KCP / 4; ++c) {
             const float4 v = fload4(frs, lo + 16u * c, soff);
-            if (4 * c + 0 < K) dst[4 * c + 0] = v.x;
-            if (4 * c + 1 < K) dst[4 * c + 1] = v.y;
-            if (4 * c + 2 < K) dst[4 * c + 2] = v.z;
-            if (4 * c + 3 < K) dst[4 * c + 3] = v.w;
+            const float ve[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (4 * c + e < KC) dst[4 * c + e] = ve[e];
+        }
+    };
+    // all K coefficients of row q (the f2 of a q2 group)
+    auto load_row = [&](int q, float (&dst)[K]) {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int soff = (q * NCH + ch) * (int)B * (KCP * 4);
+#pragma unroll
+            for (int c = 0; c < KCP / 4; ++c) {
+                const float4 v = fload4(frs, lo + 16u * c, soff);
+                const float ve[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int j = 4 * c + e, k = ch * KC + j;
+                    if (j < KC && k < K) dst[k] = ve[e];
+                }
+            }
         }
     };
     int s0 = row_ptr_c[row], s1 = row_ptr_c[row + 1];
@@ -160,26 +180,110 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
     mx_f32x4 acc[4];
 #pragma unroll
     for (int T = 0; T < 4; ++T) acc[T] = mx_f32x4{0.f, 0.f, 0.f, 0.f};
-    // FP64 sums of the precise slices (lane = distribution) live in this wave's LDS area [K][64]: each lane
-    // reads and writes only its own column, and keeping them out of registers across the row keeps the
-    // kernel at 3 waves per SIMD.
     constexpr int RD = mx_region_doubles<K, KC, SPLIT>();
-    double *const qpl = lds + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE) * RD;
-    mx_f32x4 *const abuf = reinterpret_cast<mx_f32x4 *>(qpl + K * WAVE);  // [NG][64]
-#pragma unroll
-    for (int k = 0; k < K; ++k) qpl[k * WAVE + lane] = 0.0;
+    mx_f32x4 *const abuf = reinterpret_cast<mx_f32x4 *>(lds + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE) * RD);  // [NG][64]
     constexpr TriIndex<K> tri{};
-    for (int s = s0; s < s1; ++s) {
+    // ---- stage 1 of chunk c0 of slice sl: Phi[k2][j] (k3 = c0 + j) over the slice's q2 groups, FP32.
+    // A group accumulates h = sum g f3 and is closed (phi += f2 (x) h) by the next group's first entry or
+    // the end of the slice.  With a full chunk the first group is peeled off and its close writes phi by
+    // a multiply (0 + x*y rounds as x*y); with k3 chunks phi is zero-filled (the peel's second copy of
+    // the entry loop costs registers there).
+    auto build_phi = [&](const Slice &sl, int c0, float (&phi)[K][KC]) {
+        const int ch = c0 / KC;
+        float f2[K], h[KC];
+        int zmain = sl.z0;
+#pragma unroll
+        for (int j = 0; j < KC; ++j) h[j] = 0.f;
+        if constexpr (FULL) {
+            const int zg2 = sl.z0 + (sl.q1 >> SLICE_G1_SHIFT);
+            load_row(~zq_c[sl.z0].x, f2);
+            for (int zb = sl.z0; zb < zg2; zb += ZB) {
+                int2 qv[ZB];
+                float gv[ZB];
+#pragma unroll
+                for (int e = 0; e < ZB; ++e) {
+                    qv[e] = zq_c[zb + e];
+                    gv[e] = zg_c[zb + e];
+                }
+#pragma unroll
+                for (int e = 0; e < ZB; ++e) {
+                    if (zb + e < zg2) {
+                        float f3[KC];
+                        load_chunk(qv[e].y, ch, f3);
+                        const float gg = in_vgpr(gv[e]);
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) h[j] = fmaf(gg, f3[j], h[j]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = f2[k2] * h[j];
+                h[j] = 0.f;
+            }
+            zmain = zg2;
+        } else {
+#pragma unroll
+            for (int k2 = 0; k2 < K; ++k2)
+#pragma unroll
+                for (int j = 0; j < KC; ++j) phi[k2][j] = 0.f;
+        }
+        bool have = false;
+        for (int zb = zmain; zb < sl.z1; zb += ZB) {
+            int2 qv[ZB];
+            float gv[ZB];
+#pragma unroll
+            for (int e = 0; e < ZB; ++e) {
+                qv[e] = zq_c[zb + e];
+                gv[e] = zg_c[zb + e];
+            }
+#pragma unroll
+            for (int e = 0; e < ZB; ++e) {
+                if (zb + e < sl.z1) {
+                    const int2 qq = qv[e];
+                    if (qq.x < 0) {
+                        if (have) {
+#pragma unroll
+                            for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                                for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fmaf(f2[k2], h[j], phi[k2][j]);
+                                h[j] = 0.f;
+                            }
+                        }
+                        have = true;
+                        load_row(~qq.x, f2);
+                    }
+                    float f3[KC];
+                    load_chunk(qq.y, ch, f3);
+                    const float gg = in_vgpr(gv[e]);
+#pragma unroll
+                    for (int j = 0; j < KC; ++j) h[j] = fmaf(gg, f3[j], h[j]);
+                }
+            }
+        }
+        if (have) {
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+#pragma unroll
+                for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fmaf(f2[k2], h[j], phi[k2][j]);
+        }
+    };
+    // The host orders each row's precise slices (partner block l = 0, FP64 radial stage) after its other
+    // slices: this wave's range [s0, s1) ends with its precise slices [sp, s1), which run after the
+    // matrix-core loop with their FP64 sums in registers only then.
+    int sp = s1;
+    while (sp > s0 && (slices_c[sp - 1].q1 & SLICE_PRECISE) != 0) --sp;
+    for (int s = s0; s < sp; ++s) {
         const Slice sl = slices_c[s];
-        const bool precise = (sl.q1 & SLICE_PRECISE) != 0;
-        const bool symm = PK && !precise && (sl.q1 & SLICE_SYMMETRIC) != 0;
+        const bool symm = FULL && (sl.q1 & SLICE_SYMMETRIC) != 0;
 #pragma unroll 1
         for (int c0 = 0; c0 < K; c0 += KC) {
             // The chunk's A operands go straight from memory into this wave's LDS area (one 16-byte
             // element per lane and 4-step group, landing at lane order) while stage 1 runs; no
             // registers are held for them until stage 2 reads them back.
             const int64_t gbase = ((int64_t)sl.t * S::NCH + c0 / KC) * S::NG;  // first 4-step group of the chunk
-            if (!precise) {
+            {
                 const int ng = symm ? (S::NSS + 3) / 4 : S::NG;
 #pragma unroll
                 for (int g4 = 0; g4 < S::NG; ++g4)
@@ -188,139 +292,13 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
                             (const __attribute__((address_space(1))) void *)(Rmx + ((gbase + g4) * WAVE + lane) * 4),
                             (__attribute__((address_space(3))) void *)(abuf + g4 * WAVE), 16, 0, 0);
             }
-            // ---- stage 1: Phi[k2][j] (k3 = c0 + j) over the slice's q2 groups, FP32.  A group
-            // accumulates h = sum g f3 and is closed (phi += f2 (x) h) by the next group's first entry
-            // or the end of the slice.  With a full chunk the first group is peeled off and its close
-            // writes phi by a multiply (0 + x*y rounds as x*y).
             float phi[K][KC];
-            {
-                float f2[K], h[KC];
-                int zmain = sl.z0;
-                if constexpr (PK) {
-                    const int zg2 = sl.z0 + (sl.q1 >> SLICE_G1_SHIFT);
-                    load_row(~zq_c[sl.z0].x, f2);
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) h[j] = 0.f;
-                    for (int zb = sl.z0; zb < zg2; zb += ZB) {
-                        int2 qv[ZB];
-                        float gv[ZB];
-#pragma unroll
-                        for (int e = 0; e < ZB; ++e) {
-                            qv[e] = zq_c[zb + e];
-                            gv[e] = zg_c[zb + e];
-                        }
-#pragma unroll
-                        for (int e = 0; e < ZB; ++e) {
-                            if (zb + e < zg2) {
-                                float f3[K];
-                                load_row(qv[e].y, f3);
-#pragma unroll
-                                for (int j = 0; j < KC; ++j) h[j] = fmaf(gv[e], f3[j], h[j]);
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) {
-#pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = f2[k2] * h[j];
-                        h[j] = 0.f;
-                    }
-                    zmain = zg2;
-                } else {
-#pragma unroll
-                    for (int k = 0; k < K; ++k)
-#pragma unroll
-                        for (int j = 0; j < KC; ++j) phi[k][j] = 0.f;
-#pragma unroll
-                    for (int j = 0; j < KC; ++j) h[j] = 0.f;
-                }
-                bool have = false;
-                for (int zb = zmain; zb < sl.z1; zb += ZB) {
-                    int2 qv[ZB];
-                    float gv[ZB];
-#pragma unroll
-                    for (int e = 0; e < ZB; ++e) {
-                        qv[e] = zq_c[zb + e];
-                        gv[e] = zg_c[zb + e];
-                    }
-#pragma unroll
-                    for (int e = 0; e < ZB; ++e) {
-                        if (zb + e < sl.z1) {
-                            const int2 qq = qv[e];
-                            if (qq.x < 0) {
-                                if (have) {
-#pragma unroll
-                                    for (int j = 0; j < KC; ++j) {
-#pragma unroll
-                                        for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fmaf(f2[k2], h[j], phi[k2][j]);
-                                        h[j] = 0.f;
-                                    }
-                                }
-                                have = true;
-                                if constexpr (PK) {
-                                    load_row(~qq.x, f2);
-                                } else {
-                                    const int64_t u2 = (int64_t)(~qq.x) * B;
-#pragma unroll
-                                    for (int k2 = 0; k2 < K; ++k2) f2[k2] = fload<float>(frs, lo, u2 + k2 * kstride);
-                                }
-                            }
-                            if constexpr (PK) {
-                                float f3[K];
-                                load_row(qq.y, f3);
-#pragma unroll
-                                for (int j = 0; j < KC; ++j) h[j] = fmaf(gv[e], f3[j], h[j]);
-                            } else {
-                                const int64_t u3 = (int64_t)qq.y * B;
-#pragma unroll
-                                for (int j = 0; j < KC; ++j)
-                                    if (c0 + j < K) h[j] = fmaf(gv[e], fload<float>(frs, lo, u3 + (c0 + j) * kstride), h[j]);
-                            }
-                        }
-                    }
-                }
-                if (have) {
-#pragma unroll
-                    for (int j = 0; j < KC; ++j)
-#pragma unroll
-                        for (int k2 = 0; k2 < K; ++k2) phi[k2][j] = fmaf(f2[k2], h[j], phi[k2][j]);
-                }
-            }
-            if (precise) {
-                // ---- stage 2 in FP64 on the VALU: one k3-plane of R_t per k3, scalar batches
-                const cptr<double> Rt = R_c + (int64_t)sl.t * K3;
-                constexpr int RB = (K2 + 1) / 2 < 25 ? (K2 + 1) / 2 : 25;
-                double qp[K];
-#pragma unroll
-                for (int k = 0; k < K; ++k) qp[k] = qpl[k * WAVE + lane];
-#pragma unroll
-                for (int j = 0; j < KC; ++j) {
-                    if (c0 + j < K) {
-                        const cptr<double> plane = Rt + (c0 + j) * K2;
-#pragma unroll
-                        for (int e0 = 0; e0 < K2; e0 += RB) {
-                            double rr[RB];
-#pragma unroll
-                            for (int e = 0; e < RB; ++e)
-                                if (e0 + e < K2) rr[e] = plane[e0 + e];
-#pragma unroll
-                            for (int e = 0; e < RB; ++e)
-                                if (e0 + e < K2) {
-                                    const int k2 = (e0 + e) / K, k1 = (e0 + e) % K;
-                                    qp[k1] = fma(rr[e], (double)phi[k2][j], qp[k1]);
-                                }
-                        }
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < K; ++k) qpl[k * WAVE + lane] = qp[k];
-                continue;
-            }
+            build_phi(sl, c0, phi);
             // ---- stage 2 on the matrix cores.  The LDS fills of the A operands are vector-memory
             // operations: wait for them (the f loads of stage 1 have all been consumed by now).
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             if (symm) {
-                if constexpr (PK) {
+                if constexpr (FULL) {
 #pragma unroll
                     for (int k3 = 1; k3 < K; ++k3)
 #pragma unroll
@@ -370,12 +348,55 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
             }
         }
     }
+    // Their FP64 sums (lane = distribution) stay in this wave's LDS region, free of A operands by now,
+    // across the stage-1 passes; each lane reads and writes only its own column.
+    double *const qpl = lds + __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE) * RD;  // [K][64]
+    if (sp < s1) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // the last chunk's LDS fills have landed (vmcnt(0))
+#pragma unroll
+        for (int k = 0; k < K; ++k) qpl[k * WAVE + lane] = 0.0;
+    }
+    for (int s = sp; s < s1; ++s) {
+        const Slice sl = slices_c[s];
+        const cptr<double> Rt = R_c + (int64_t)sl.t * K3;
+#pragma unroll 1
+        for (int c0 = 0; c0 < K; c0 += KC) {
+            float phi[K][KC];
+            build_phi(sl, c0, phi);
+            // ---- stage 2 in FP64 on the VALU: one k3-plane of R_t per k3, scalar batches
+            constexpr int RB = (K2 + 1) / 2 < 25 ? (K2 + 1) / 2 : 25;
+            double qp[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) qp[k] = qpl[k * WAVE + lane];
+#pragma unroll
+            for (int j = 0; j < KC; ++j) {
+                if (c0 + j < K) {
+                    const cptr<double> plane = Rt + (c0 + j) * K2;
+#pragma unroll
+                    for (int e0 = 0; e0 < K2; e0 += RB) {
+                        double rr[RB];
+#pragma unroll
+                        for (int e = 0; e < RB; ++e)
+                            if (e0 + e < K2) rr[e] = plane[e0 + e];
+#pragma unroll
+                        for (int e = 0; e < RB; ++e)
+                            if (e0 + e < K2) {
+                                const int k2 = (e0 + e) / K, k1 = (e0 + e) % K;
+                                qp[k1] = fma(rr[e], (double)phi[k2][j], qp[k1]);
+                            }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) qpl[k * WAVE + lane] = qp[k];
+        }
+    }
     // ---- combine.  The precise part qp (lane = distribution 16T+n, register k1) moves into the matrix
     // layout (lane (g,n), block T, register r, k1 = 4g + r) by the same 4x4 transpose, once per r:
     // registers qp[4G + r], G = 0..3, become pv[T][r] = qp_{16T+n}[4g + r].
     double qp[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) qp[k] = qpl[k * WAVE + lane];
+    for (int k = 0; k < K; ++k) qp[k] = sp < s1 ? qpl[k * WAVE + lane] : 0.0;
     double pv[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -424,7 +445,7 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
 // blockDim.x / 64 waves per workgroup.  Without a row split each wave is one batch tile of the same
 // output row; with it, consecutive wave pairs are one tile, each wave half of the row's slices.
 template <int K, int KC, bool SPLIT>
-__global__ __launch_bounds__(4 * WAVE) void af_mx_kernel(const float *__restrict__ fT, double *__restrict__ QT,
+__global__ __launch_bounds__(4 * WAVE) __attribute__((amdgpu_waves_per_eu(3))) void af_mx_kernel(const float *__restrict__ fT, double *__restrict__ QT,
                                                        const Slice *__restrict__ slices,
                                                        const int32_t *__restrict__ row_ptr,
                                                        const int32_t *__restrict__ row_order,
@@ -456,7 +477,7 @@ constexpr int mx_kc(int K) { return K <= 10 ? K : (K + 1) / 2; }
 // wave, or of two waves taking the halves of the row's slices (SPLIT), in the order the host sorted
 // the items (heaviest first; the dispatcher hands out workgroups in index order).
 template <int K, bool SPLIT>
-__global__ __launch_bounds__(2 * WAVE) void multi_mx_kernel(const MultiGroupDev *__restrict__ groups,
+__global__ __launch_bounds__(2 * WAVE) __attribute__((amdgpu_waves_per_eu(3))) void multi_mx_kernel(const MultiGroupDev *__restrict__ groups,
                                                            const int4 *__restrict__ items, int i0, int i1) {
     extern __shared__ double mx_lds[];
     const int it = i0 + (int)blockIdx.x;

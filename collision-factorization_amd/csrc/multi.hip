@@ -18,7 +18,7 @@ int launch_multi(const MultiGroupDev *groups, const int4 *items, int K, int KC, 
 int launch_multi_mx(const MultiGroupDev *groups, const int4 *items, int K, bool split, int i0, int i1,
                     hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
-int launch_pack_rows(const double *in, float *out, int N_Q, int K, int64_t B, bool natural, hipStream_t st);
+int launch_pack_rows(const double *in, float *out, int N_Q, int K, int KC, int64_t B, bool natural, hipStream_t st);
 bool multi_supports_k(int K);
 Tile plan_tile(const cfk_plan_s *p, int precision);
 int ensure_mx_tables(cfk_plan_s *p);
@@ -50,7 +50,8 @@ struct cfk_multi_s {
     };
     std::vector<Launch> launches;  // one item kernel per K-class
     std::vector<std::unique_ptr<DevBuf<float>>> f32;
-    std::vector<char> packed;  // per group: its item kernel (full k3 chunk) reads packed FP32 rows
+    std::vector<char> packed;  // per group: its item kernel reads packed FP32 rows
+    std::vector<int> pack_kc;  // per group: the k3 chunk width of its packed rows
     int n_items = 0;
     ~cfk_multi_s() {
         reset();
@@ -133,14 +134,17 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
     if ((rc = m->items.upload(items.data(), items.size()))) return rc;
     m->f32.clear();
     m->packed.assign(G, 0);
+    m->pack_kc.assign(G, 0);
     std::vector<MultiGroupDev> gd(G);
     for (int g = 0; g < G; ++g) {
         cfk_plan_s *p = m->plans[g];
-        // (from the group's own tile, as its class was formed: cls has been re-sorted since); kernels
-        // with a full k3 chunk read packed FP32 rows
+        // (from the group's own tile, as its class was formed: cls has been re-sorted since); the
+        // matrix-core kernel and VALU kernels with a full k3 chunk read packed FP32 rows (chunk-packed
+        // for the matrix-core kernel's chunks)
         const bool mx = tiles[g].pf == 1;
-        m->packed[g] = B[g] > 0 && tiles[g].kc >= p->K;
-        const int64_t n = (int64_t)p->N_Q * (m->packed[g] ? (p->K + 3) / 4 * 4 : p->K) * B[g];
+        m->packed[g] = B[g] > 0 && (mx || tiles[g].kc >= p->K);
+        m->pack_kc[g] = mx ? mx_chunk(p->K) : p->K;
+        const int64_t n = (int64_t)p->N_Q * (m->packed[g] ? rl_of(p->K, m->pack_kc[g]) : p->K) * B[g];
         m->f32.emplace_back(new DevBuf<float>());
         if ((rc = m->f32.back()->alloc((size_t)std::max<int64_t>(n, 1)))) return rc;
         gd[g] = MultiGroupDev{m->f32.back()->p, Q[g], p->sym_slices.p, p->sym_row_ptr.p, p->sym_zq.p,
@@ -173,7 +177,8 @@ static int multi_enqueue(cfk_multi_s *m, hipStream_t st) {
     for (int g = 0; g < G && rc == CFK_OK; ++g) {
         const cfk_plan_s *p = m->plans[g];
         if (m->key_B[g] > 0)
-            rc = m->packed[g] ? launch_pack_rows(m->key_f[g], m->f32[g]->p, p->N_Q, p->K, m->key_B[g], false, st)
+            rc = m->packed[g] ? launch_pack_rows(m->key_f[g], m->f32[g]->p, p->N_Q, p->K, m->pack_kc[g], m->key_B[g],
+                                                 false, st)
                               : launch_to_f32(m->key_f[g], m->f32[g]->p, (int64_t)p->N_Q * p->K * m->key_B[g], st);
     }
     const int nc = (int)m->launches.size();
