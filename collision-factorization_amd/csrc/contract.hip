@@ -681,28 +681,30 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(const double *__restrict
 }
 
 // Batch-fastest input with B even (and 16-byte aligned): one thread per (q, pair of distributions),
-// one 16-byte load of the two distributions' coefficient k, each chunk of the two rows written as
-// 2 KCP / 4 float4.
+// the K 16-byte loads of the two distributions' coefficients issued together, then each chunk of the
+// two rows written as 2 KCP / 4 float4.
+template <int K, int KC>
 __global__ __launch_bounds__(256) void pack_rows_pair_kernel(const double *__restrict__ in, float *__restrict__ out,
-                                                             int N_Q, int K, int KC, int64_t B) {
+                                                             int N_Q, int64_t B) {
+    constexpr int KCP = kcp_of(KC), NCH = (K + KC - 1) / KC;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t half = B / 2;
     if (i >= (int64_t)N_Q * half) return;
     const int64_t q = i / half, b = 2 * (i - q * half);
-    const int KCP = kcp_of(KC), NCH = (K + KC - 1) / KC;
+    double2 v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = *reinterpret_cast<const double2 *>(in + ((int64_t)k * N_Q + q) * B + b);
+#pragma unroll
     for (int c = 0; c < NCH; ++c) {
         float4 *o = reinterpret_cast<float4 *>(out + ((q * NCH + c) * B + b) * KCP);
 #pragma unroll
-        for (int v4 = 0; v4 < 4; ++v4) {
-            if (4 * v4 >= KCP) break;
+        for (int v4 = 0; v4 < KCP / 4; ++v4) {
             float w0[4], w1[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int j = 4 * v4 + e, k = c * KC + j;
-                double2 v = make_double2(0.0, 0.0);
-                if (j < KC && k < K) v = *reinterpret_cast<const double2 *>(in + ((int64_t)k * N_Q + q) * B + b);
-                w0[e] = (float)v.x;
-                w1[e] = (float)v.y;
+                w0[e] = (j < KC && k < K) ? (float)v[j < KC && k < K ? k : 0].x : 0.0f;
+                w1[e] = (j < KC && k < K) ? (float)v[j < KC && k < K ? k : 0].y : 0.0f;
             }
             o[v4] = make_float4(w0[0], w0[1], w0[2], w0[3]);
             o[KCP / 4 + v4] = make_float4(w1[0], w1[1], w1[2], w1[3]);
@@ -892,10 +894,21 @@ int launch_pack_rows(const double *in, float *out, int N_Q, int K, int KC, int64
     const dim3 grid((unsigned)((n + 255) / 256));
     if (natural)
         hipLaunchKernelGGL(pack_rows_kernel<true>, grid, dim3(256), 0, st, in, out, N_Q, K, KC, B);
-    else if (B % 2 == 0 && K <= 16 && reinterpret_cast<uintptr_t>(in) % 16 == 0)
-        hipLaunchKernelGGL(pack_rows_pair_kernel, dim3((unsigned)((n / 2 + 255) / 256)), dim3(256), 0, st, in, out,
-                           N_Q, K, KC, B);
-    else
+    else if (B % 2 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0 && (KC == K || KC == (K + 1) / 2)) {
+        const dim3 g2((unsigned)((n / 2 + 255) / 256));
+        switch (K * 2 + (KC == K ? 0 : 1)) {
+#define CFK_PAIR(KK)                                                                                         \
+    case 2 * KK:                                                                                            \
+        hipLaunchKernelGGL((pack_rows_pair_kernel<KK, KK>), g2, dim3(256), 0, st, in, out, N_Q, B);         \
+        break;                                                                                              \
+    case 2 * KK + 1:                                                                                        \
+        hipLaunchKernelGGL((pack_rows_pair_kernel<KK, (KK + 1) / 2>), g2, dim3(256), 0, st, in, out, N_Q, B); \
+        break;
+            CFK_PAIR(1) CFK_PAIR(2) CFK_PAIR(3) CFK_PAIR(4) CFK_PAIR(5) CFK_PAIR(6) CFK_PAIR(7) CFK_PAIR(8)
+            CFK_PAIR(9) CFK_PAIR(10) CFK_PAIR(11) CFK_PAIR(12) CFK_PAIR(13) CFK_PAIR(14) CFK_PAIR(15) CFK_PAIR(16)
+#undef CFK_PAIR
+        }
+    } else
         hipLaunchKernelGGL(pack_rows_kernel<false>, grid, dim3(256), 0, st, in, out, N_Q, K, KC, B);
     CFK_HIP(hipGetLastError());
     return CFK_OK;
