@@ -90,11 +90,13 @@ int mx_chunk(int K);  // k3 chunk width of the matrix-core kernel
 
 constexpr int MULTI_MAX_GROUPS = 8;
 // Largest batch per call: the contraction kernels read f through a buffer resource, which spans
-// at most 4 GiB (8-byte coefficients counted), and the mixed kernels' packed FP32 rows (K rounded up
-// to 4 per row; 16 slots for the chunk-packed rows from K = 11) take a signed 32-bit row offset.
+// at most 4 GiB (8-byte coefficients counted), and the mixed kernels' packed FP32 rows (RL floats per
+// (q, distribution): K rounded up to 4, or the matrix-core kernel's chunk-packed rows) take a signed
+// 32-bit offset.
 inline int64_t max_batch(int N_Q, int K) {
     const int64_t a = (((int64_t)1 << 32) - 1) / (8 * (int64_t)N_Q * K);
-    const int64_t b = (((int64_t)1 << 31) - 1) / (4 * (int64_t)N_Q * (K <= 10 ? (K + 3) / 4 * 4 : 16));
+    const int rl = rl_of(K, mx_chunk(K)) > (K + 3) / 4 * 4 ? rl_of(K, mx_chunk(K)) : (K + 3) / 4 * 4;
+    const int64_t b = (((int64_t)1 << 31) - 1) / (4 * (int64_t)N_Q * rl);
     return a < b ? a : b;
 }
 
