@@ -603,6 +603,7 @@ def bench_surrogate(args, torch, cfk):
         return (time.perf_counter() - t) / n * 1e3
 
     t_sur = timed(lambda: sur.apply(c0, out=Qb))
+    t_sur_a = timed(lambda: sur_a.apply(c0, out=Qb))   # anchors (moments, c_M, exact Q(c_M)) included
     t_f64 = timed(lambda: plan.execute(Qb, c0, B, precision="fp64_sym", layout="batch_fastest"))
     t_mix = timed(lambda: plan.execute(Qb, c0, B, precision="mixed", layout="batch_fastest"))
     gemm_flop = 2.0 * B * (810 * 1024 + 2 * 1024 * 1024 + 1024 * 810)
@@ -615,7 +616,8 @@ def bench_surrogate(args, torch, cfk):
            "train_data_s": t_data, "fit_s": t_fit, "loss_first_last": [float(hist[0]), float(hist[-1])],
            "heldout_rel_err": {"network": err_net, "linear_part_only": err_lin},
            "relax": out_modes,
-           "eval_ms_16384": {"surrogate": t_sur, "exact_fp64": t_f64, "mixed": t_mix},
+           "eval_ms_16384": {"surrogate": t_sur, "anchored_surrogate_with_anchors": t_sur_a, "exact_fp64": t_f64,
+                             "mixed": t_mix},
            "network_tflops": gemm_flop / (t_sur * 1e-3) / 1e12,
            "anchored": {"train_pairs": n_pairs_a, "train_data_s": t_data_a, "fit_s": t_fit_a,
                         "loss_first_last": [float(hist_a[0]), float(hist_a[-1])],

@@ -114,6 +114,7 @@ SIGNATURES = {
                            _vp],
                           ctypes.c_int),
     "cfk_surrogate_apply": ([_vp, _vp, _vp, _i64, _i32, _vp], ctypes.c_int),
+    "cfk_surrogate_anchors": ([_vp, _vp, _vp, _vp, _i64, _vp], ctypes.c_int),
     "cfk_surrogate_relax": ([_vp, _vp, _i64, ctypes.c_double, _i32, _i32, ctypes.c_double, _i32, _dp,
                              ctypes.POINTER(RelaxStats), _vp], ctypes.c_int),
 }

@@ -199,6 +199,17 @@ class Surrogate:
                                         _stream_ptr(stream)))
         return out
 
+    def anchors(self, f, stream=None):
+        """(c_M, Q(c_M)) [K*N_Q, B] float64: each cell's isotropic Maxwellian on the basis (the cell's n, u,
+        T) and its exact operator value."""
+        import torch
+        if f.dtype != torch.float64 or not f.is_cuda or f.dim() != 2:
+            raise ValueError("f must be a 2-D float64 cuda tensor [K*N_Q, B]")
+        f = f.contiguous()
+        cM, QM = torch.empty_like(f), torch.empty_like(f)
+        check(lib().cfk_surrogate_anchors(self._h, _ptr(cM), _ptr(QM), _ptr(f), f.shape[1], _stream_ptr(stream)))
+        return cM, QM
+
     def relax(self, c, dt, n_steps, mode="surrogate", tol=1e-2, sample_stride=100, stream=None):
         """Forward-Euler relaxation of c [K*N_Q, B] in place; returns (stats dict, residual per step)."""
         if not c.is_contiguous():

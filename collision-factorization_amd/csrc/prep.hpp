@@ -40,6 +40,9 @@ double legendre_plm(int l, int m, double x);
 void sh_norm(int L, double *out);
 // RadialNorm[k + N_K*l] (src/GeneralCollisionTensor.m:82-90)
 void radial_norm(int N_K, int L, double *out);
+// Radial nodes r_a = sqrt(x_a) (n_rad generalized Gauss-Laguerre nodes, alpha = 1/2, in x = |v|^2) and
+// rad[(k (L+1) + l) n_rad + a] = R_{k,l}(r_a) w_a / 2, the radial factor of the tutorials' projection.
+void projection_radial_table(int L, int K, int n_rad, std::vector<double> &r, std::vector<double> &rad);
 
 // The 1-D rules of the sumfac (src/GeneralCollisionTensor.m:66-79)
 struct SumfacGrids {

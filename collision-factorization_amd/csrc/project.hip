@@ -136,7 +136,7 @@ __global__ __launch_bounds__(PT) void project_kernel(ProjDev d, const double *__
 }
 
 // L_k^{(alpha)}(x) by the three-term recursion (src/SpectralBasis.m:143-162)
-double gen_laguerre(int k, double alpha, double x) {
+double gen_laguerre_k(int k, double alpha, double x) {
     if (k == 0) return 1.0;
     double p2 = 1.0, p1 = 1.0 + alpha - x;
     for (int i = 1; i < k; ++i) {
@@ -148,6 +148,21 @@ double gen_laguerre(int k, double alpha, double x) {
 }
 
 }  // namespace
+
+void projection_radial_table(int L, int K, int n_rad, std::vector<double> &r, std::vector<double> &rad) {
+    std::vector<double> x(n_rad), w(n_rad), rn((size_t)K * (L + 1));
+    gauss_gen_laguerre(n_rad, 0.5, x.data(), w.data());
+    radial_norm(K, L, rn.data());
+    r.resize(n_rad);
+    for (int a = 0; a < n_rad; ++a) r[a] = std::sqrt(x[a]);
+    rad.assign((size_t)K * (L + 1) * n_rad, 0.0);
+    for (int k = 0; k < K; ++k)
+        for (int l = 0; l <= L; ++l)
+            for (int a = 0; a < n_rad; ++a)
+                rad[((size_t)k * (L + 1) + l) * n_rad + a] =
+                    rn[k + (size_t)K * l] * gen_laguerre_k(k, l + 0.5, x[a]) * std::pow(r[a], l) * 0.5 * w[a];
+}
+
 }  // namespace cfk
 
 using namespace cfk;
@@ -168,25 +183,18 @@ extern "C" int cfk_project_maxwellians(double *c_dev, const double *params_dev, 
     if ((rc = check_device())) return rc;
     if (B == 0) return CFK_OK;
     // host tables
-    std::vector<double> x(n_rad), w(n_rad), r(n_rad), mu(n_pol), wp(n_pol), cphi(n_azi), sphi(n_azi);
-    gauss_gen_laguerre(n_rad, 0.5, x.data(), w.data());
+    std::vector<double> r, rad, mu(n_pol), wp(n_pol), cphi(n_azi), sphi(n_azi);
+    projection_radial_table(L, K, n_rad, r, rad);
     gauss_legendre(n_pol, -1.0, 1.0, mu.data(), wp.data());
-    for (int a = 0; a < n_rad; ++a) r[a] = std::sqrt(x[a]);
     const double wphi = 2.0 * M_PI / n_azi;
     for (int j = 0; j < n_azi; ++j) {
         const double ph = 2.0 * M_PI * j / n_azi;
         cphi[j] = std::cos(ph);
         sphi[j] = std::sin(ph);
     }
-    std::vector<double> rn((size_t)K * (L + 1)), shn(N_Q);
-    radial_norm(K, L, rn.data());
+    std::vector<double> shn(N_Q);
     sh_norm(L, shn.data());
-    std::vector<double> rad((size_t)K * (L + 1) * n_rad), leg((size_t)N_Q * n_pol), trig((size_t)M2 * n_azi);
-    for (int k = 0; k < K; ++k)
-        for (int l = 0; l <= L; ++l)
-            for (int a = 0; a < n_rad; ++a)
-                rad[((size_t)k * (L + 1) + l) * n_rad + a] =
-                    rn[k + (size_t)K * l] * gen_laguerre(k, l + 0.5, x[a]) * std::pow(r[a], l) * 0.5 * w[a];
+    std::vector<double> leg((size_t)N_Q * n_pol), trig((size_t)M2 * n_azi);
     for (int q = 0; q < N_Q; ++q) {
         const int l = l_of_q0(q), m = q - l * l - l;
         for (int p = 0; p < n_pol; ++p) leg[(size_t)q * n_pol + p] = shn[q] * legendre_plm(l, std::abs(m), mu[p]) * wp[p];

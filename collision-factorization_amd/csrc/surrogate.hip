@@ -159,17 +159,84 @@ int launch_post(int K, const double *f, const double *nvec, const double *svec, 
 // the cell approaches equilibrium (R is linear in g there), so the surrogate's error shrinks with |g|
 // instead of staying at a fixed fraction of the unit-Maxwellian remainder.
 
-// Maxwellian parameters (n, u_x, u_y, u_z, T, T, T) of each cell from its moments [B][CFK_N_MOMENTS]
-__global__ void maxwellian_params_kernel(const double *__restrict__ mom, double *__restrict__ par, int64_t B) {
-    const int64_t b = (int64_t)blockIdx.x * ET + threadIdx.x;
-    if (b >= B) return;
+// Each cell's isotropic Maxwellian M (n, u, T from its moments [B][CFK_N_MOMENTS]) on the basis, by the
+// Funk-Hecke theorem: M is axially symmetric about u, so for every spherical harmonic Y of degree l
+//   int M(r, v^) Y(v^) dOmega = lambda_l(r) Y(u^),  lambda_l(r) = 2 pi int_{-1}^{1} M_z(r, mu) P_l(mu) dmu,
+// with M_z the same Maxwellian drifted along z.  The coefficient (k, l, m) is therefore
+//   c = [sum_a rad[k][l][a] lambda_l(r_a)] N_lm P_l^|m|(u^_z) T_m(phi_u) / N_l0,
+// lambda_l on the projection's radial nodes and Gauss-Legendre polar nodes: the tutorials' projection
+// grid with the azimuthal integral done exactly -- AN_R x AN_P exponentials per cell instead of
+// AN_R x AN_P x 30.  Table tab: rad [K][L+1][AN_R] (R_{k,l}(r_a) w_a / 2), r [AN_R], mu [AN_P],
+// legl [L+1][AN_P] (2 pi N_l0 P_l(mu_p) w_p), ratio [N_Q] (N_lm / N_l0).  One workgroup per cell.
+constexpr int AN_R = 60, AN_P = 30, AN_T = 128;
+
+__device__ double plm_dev(int l, int m, double x) {  // P_l^m(x), Condon-Shortley phase (as legendre_plm)
+    double pmm = 1.0;
+    if (m > 0) {
+        const double sn = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
+        double fact = 1.0;
+        for (int i = 1; i <= m; ++i) {
+            pmm *= -fact * sn;
+            fact += 2.0;
+        }
+    }
+    if (l == m) return pmm;
+    double p1 = x * (2 * m + 1) * pmm;
+    if (l == m + 1) return p1;
+    double pl = 0.0;
+    for (int ll = m + 2; ll <= l; ++ll) {
+        pl = (x * (2 * ll - 1) * p1 - (ll + m - 1) * pmm) / (ll - m);
+        pmm = p1;
+        p1 = pl;
+    }
+    return pl;
+}
+
+__global__ __launch_bounds__(AN_T) void anchor_coeffs_kernel(const double *__restrict__ mom, const double *__restrict__ tab,
+                                                             double *__restrict__ cM, int64_t B, int L, int K, int N_Q) {
+    __shared__ double lam[AN_R][13];
+    __shared__ double cax[16][13];
+    const int tid = threadIdx.x, L1 = L + 1;
+    const double *rad = tab, *r = rad + (size_t)K * L1 * AN_R, *mu = r + AN_R, *legl = mu + AN_P,
+                 *ratio = legl + (size_t)L1 * AN_P;
+    const int64_t b = blockIdx.x;
     const double *o = mom + b * CFK_N_MOMENTS;
-    double *q = par + b * 7;
-    q[0] = o[0];
-    q[1] = o[1];
-    q[2] = o[2];
-    q[3] = o[3];
-    q[4] = q[5] = q[6] = o[4];
+    const double n = o[0], ux = o[1], uy = o[2], uz = o[3], T = o[4];
+    const double U = sqrt(ux * ux + uy * uy + uz * uz);
+    if (tid < AN_R) {
+        double acc[13];
+#pragma unroll
+        for (int l = 0; l < 13; ++l) acc[l] = 0.0;
+        if (n > 0.0) {  // (n <= 0: no Maxwellian, zero coefficients)
+            const double lp = log(n) - 1.5 * log(M_PI * T), ra = r[tid], r2 = ra * ra;
+            for (int p = 0; p < AN_P; ++p) {
+                const double e = exp(lp + r2 - (r2 - 2.0 * ra * U * mu[p] + U * U) / T);
+#pragma unroll
+                for (int l = 0; l < 13; ++l)
+                    if (l <= L) acc[l] = fma(legl[l * AN_P + p], e, acc[l]);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < 13; ++l)
+            if (l <= L) lam[tid][l] = acc[l];
+    }
+    __syncthreads();
+    for (int t = tid; t < K * L1; t += AN_T) {
+        const int k = t / L1, l = t % L1;
+        const double *rk = rad + ((size_t)k * L1 + l) * AN_R;
+        double sacc = 0.0;
+        for (int a = 0; a < AN_R; ++a) sacc = fma(rk[a], lam[a][l], sacc);
+        cax[k][l] = sacc;
+    }
+    __syncthreads();
+    const double x = U > 0.0 ? uz / U : 1.0, ph = U > 0.0 ? atan2(uy, ux) : 0.0;
+    for (int q = tid; q < N_Q; q += AN_T) {
+        int l = 0;
+        while ((l + 1) * (l + 1) <= q) ++l;
+        const int m = q - l * l - l, am = m < 0 ? -m : m;
+        const double Y = ratio[q] * plm_dev(l, am, x) * (m == 0 ? 1.0 : m > 0 ? cos(m * ph) : sin(am * ph));
+        for (int k = 0; k < K; ++k) cM[((int64_t)k * N_Q + q) * B + b] = cax[k][l] * Y;
+    }
 }
 
 // Per cell: n = f[0], g = (f - c_M) / n (written), |g| (floored at G_MIN); layout as dev_norm_kernel
@@ -432,8 +499,8 @@ struct cfk_surrogate_s {
     // scratch
     DevBuf<double> nvec, svec, Qx, cs, Qs, acc;
     DevBuf<float> Yt;  // network output, feature-major [NK][B]
-    // anchored decomposition: g, |g|, each cell's c_M and Q(c_M), moments and Maxwellian parameters
-    DevBuf<double> gap, aM, aQ, mom, par;
+    // anchored decomposition: g, |g|, each cell's c_M and Q(c_M), its moments
+    DevBuf<double> gap, aM, aQ, mom;
     int64_t cap = 0;
 
     int ensure(int64_t B) {
@@ -443,8 +510,7 @@ struct cfk_surrogate_s {
             (rc = Yt.alloc((size_t)NK * B)))
             return rc;
         if (anchored && ((rc = gap.alloc((size_t)NK * B)) || (rc = aM.alloc((size_t)NK * B)) ||
-                         (rc = aQ.alloc((size_t)NK * B)) || (rc = mom.alloc((size_t)CFK_N_MOMENTS * B)) ||
-                         (rc = par.alloc((size_t)7 * B))))
+                         (rc = aQ.alloc((size_t)NK * B)) || (rc = mom.alloc((size_t)CFK_N_MOMENTS * B))))
             return rc;
         cap = B;
         return CFK_OK;
@@ -456,15 +522,36 @@ struct cfk_surrogate_s {
             return rc;
         return CFK_OK;
     }
-    // Each cell's Maxwellian (same n, u, T; the tutorials' projection grid) and its exact Q, into cM and
-    // QM; mom and par hold B x CFK_N_MOMENTS and B x 7 values.
-    int anchor(double *cM, double *QM, const double *f, int64_t B, double *momb, double *parb, hipStream_t st) {
+    // Each cell's Maxwellian (same n, u, T; on the tutorials' projection grid) and its exact Q, into cM and
+    // QM; momb holds B x CFK_N_MOMENTS values.
+    int anchor(double *cM, double *QM, const double *f, int64_t B, double *momb, hipStream_t st) {
         int rc;
+        if (B <= 0) return CFK_OK;
+        if ((rc = anchor_tables())) return rc;
         if ((rc = cfk_moments(momb, f, B, N_Q, K, CFK_LAYOUT_BATCH_FASTEST, st))) return rc;
-        hipLaunchKernelGGL(maxwellian_params_kernel, dim3(nb(B)), dim3(ET), 0, st, momb, parb, B);
+        hipLaunchKernelGGL(anchor_coeffs_kernel, dim3((unsigned)B), dim3(AN_T), 0, st, momb, atab.p, cM, B, L, K, N_Q);
         CFK_HIP(hipGetLastError());
-        if ((rc = cfk_project_maxwellians(cM, parb, B, 1, L, K, 60, 30, 30, st))) return rc;
         return exact(QM, cM, B, st);
+    }
+    // the anchor kernel's tables (built on first use)
+    DevBuf<double> atab;
+    int anchor_tables() {
+        if (atab.p) return CFK_OK;
+        if (L > 12 || K > 16) return set_error(CFK_EUNSUP, "anchors: L=%d (<= 12), K=%d (<= 16)", L, K);
+        std::vector<double> r, rad, mu(AN_P), wp(AN_P), shn(N_Q);
+        projection_radial_table(L, K, AN_R, r, rad);
+        gauss_legendre(AN_P, -1.0, 1.0, mu.data(), wp.data());
+        sh_norm(L, shn.data());
+        std::vector<double> t(rad);
+        t.insert(t.end(), r.begin(), r.end());
+        t.insert(t.end(), mu.begin(), mu.end());
+        for (int l = 0; l <= L; ++l)
+            for (int p = 0; p < AN_P; ++p) t.push_back(2.0 * M_PI * shn[l * l + l] * legendre_plm(l, 0, mu[p]) * wp[p]);
+        for (int q = 0; q < N_Q; ++q) {
+            const int l = l_of_q0(q);
+            t.push_back(shn[q] / shn[l * l + l]);
+        }
+        return atab.upload(t.data(), t.size());
     }
     // Q = Q~(f) with the anchors (cM, QM) of f's cells
     int apply_anchored(double *Q, const double *f, const double *cM, const double *QM, int64_t B, bool use_net,
@@ -503,7 +590,7 @@ struct cfk_surrogate_s {
         int rc;
         if ((rc = ensure(B))) return rc;
         if (anchored) {
-            if ((rc = anchor(aM.p, aQ.p, f, B, mom.p, par.p, st))) return rc;
+            if ((rc = anchor(aM.p, aQ.p, f, B, mom.p, st))) return rc;
             return apply_anchored(Q, f, aM.p, aQ.p, B, use_net, st);
         }
         hipLaunchKernelGGL(dev_norm_kernel, dim3((unsigned)((B + 63) / 64)), dim3(64 * DN), 0, st, f, NK, B, nvec.p, svec.p);
@@ -716,13 +803,12 @@ int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev,
     if (s->anchored) {
         // 1. each sample's anchor (c_M, Q(c_M)), g and |g|; samples that are their own Maxwellian (|g| ~ 0:
         // one-component mixtures) carry no direction g / |g| and are left out
-        DevBuf<double> cM, QM, gp, nv, gn, momb, parb;
+        DevBuf<double> cM, QM, gp, nv, gn, momb;
         if ((rc = cM.alloc(tot)) || (rc = QM.alloc(tot)) || (rc = gp.alloc(tot)) || (rc = nv.alloc(n)) ||
-            (rc = gn.alloc(n)) || (rc = momb.alloc((size_t)CFK_N_MOMENTS * n)) || (rc = parb.alloc((size_t)7 * n)))
+            (rc = gn.alloc(n)) || (rc = momb.alloc((size_t)CFK_N_MOMENTS * n)))
             return rc;
-        if ((rc = s->anchor(cM.p, QM.p, f_dev, n, momb.p, parb.p, st))) return rc;
+        if ((rc = s->anchor(cM.p, QM.p, f_dev, n, momb.p, st))) return rc;
         momb.release();
-        parb.release();
         hipLaunchKernelGGL(gap_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64 * DN), 0, st, f_dev, cM.p, NK, n, gp.p, nv.p,
                            gn.p);
         CFK_HIP(hipGetLastError());
@@ -833,6 +919,20 @@ int cfk_surrogate_apply(cfk_surrogate s, double *Q_dev, const double *f_dev, int
     return s->apply(Q_dev, f_dev, B, use_network != 0, (hipStream_t)stream);
 }
 
+int cfk_surrogate_anchors(cfk_surrogate s, double *cM_dev, double *QM_dev, const double *f_dev, int64_t B, void *stream) {
+    if (!s || !cM_dev || !QM_dev || !f_dev) return set_error(CFK_EINVAL, "null argument");
+    if (B < 0 || B > 0x7fffffff) return set_error(CFK_EINVAL, "B=%lld out of range", (long long)B);
+    if (B == 0) return CFK_OK;
+    DevBuf<double> momb;
+    int rc;
+    if ((rc = momb.alloc((size_t)CFK_N_MOMENTS * B))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = s->anchor(cM_dev, QM_dev, f_dev, B, momb.p, st))) return rc;
+    // momb is freed on return: wait for the kernel that reads it
+    CFK_HIP(hipStreamSynchronize(st));
+    return CFK_OK;
+}
+
 int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, int32_t n_steps, int32_t mode, double tol,
                         int32_t sample_stride, double *residual_hist, cfk_relax_stats *stats, void *stream) {
     if (!s || !c_dev) return set_error(CFK_EINVAL, "null argument");
@@ -850,7 +950,7 @@ int cfk_surrogate_relax(cfk_surrogate s, double *c_dev, int64_t B, double dt, in
             return rc;
     }
     // anchored decomposition: each cell's Maxwellian and its exact Q once (the cells' invariants do not change)
-    if (s->anchored && mode != CFK_RELAX_EXACT && (rc = s->anchor(s->aM.p, s->aQ.p, c_dev, B, s->mom.p, s->par.p, st)))
+    if (s->anchored && mode != CFK_RELAX_EXACT && (rc = s->anchor(s->aM.p, s->aQ.p, c_dev, B, s->mom.p, st)))
         return rc;
     auto approx = [&](bool net) {
         return s->anchored ? s->apply_anchored(s->Qx.p, c_dev, s->aM.p, s->aQ.p, B, net, st)

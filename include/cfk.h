@@ -384,6 +384,13 @@ int cfk_surrogate_get_normalization(cfk_surrogate s, float *mu_x, float *inv_s_x
 int cfk_surrogate_fit(cfk_surrogate s, const double *f_dev, const double *Q_dev, int64_t n, int32_t epochs,
                       int32_t batch, float lr, float lr_final, uint64_t seed, int32_t flags, double *loss_hist,
                       void *stream);
+/* Each cell's anchor: cM_dev = the basis coefficients of the isotropic Maxwellian with the cell's n, u, T
+ * (cfk_moments of f_dev), QM_dev = Q(cM) from the plan's symmetrized FP64 operator; [K N_Q][B] each,
+ * batch-fastest.  The coefficients are those of cfk_project_maxwellians (60 x 30 x 30 grid) with the
+ * azimuthal integral exact: M is axially symmetric about u, so by the Funk-Hecke theorem its degree-l
+ * part is lambda_l(|v|) Y_lm(u / |u|) with lambda_l a polar integral of the z-drifted Maxwellian.  Any
+ * surrogate (anchored or not) can compute them; synchronizes the stream. */
+int cfk_surrogate_anchors(cfk_surrogate s, double *cM_dev, double *QM_dev, const double *f_dev, int64_t B, void *stream);
 /* Q_dev = Q~(f_dev) (overwrites).  use_network = 0 evaluates n^2 J d only (anchored: Q(c_M) + n^2 J g). */
 int cfk_surrogate_apply(cfk_surrogate s, double *Q_dev, const double *f_dev, int64_t B, int32_t use_network,
                         void *stream);

@@ -112,18 +112,23 @@ def test_anchored_linear_surrogate(torch_cuda):
     pert[nq] = 0.0                                            # and its energy coefficient
     fd = cM + torch.from_numpy(pert).cuda()
     Q = sur.apply(fd, use_network=False).cpu().numpy()
-    # reference: the anchors from the moments of f (unchanged by the perturbation), Q(c_M) exact
+    # the library's anchors: c_M against the 3-D projection of the Maxwellian with f's moments (the
+    # Funk-Hecke form integrates the azimuth exactly; the projection uses 30 nodes), Q(c_M) against the
+    # symmetrized FP64 operator
+    cMl, QMl = sur.anchors(fd)
     mom = cfk.moments(fd, 29, nq, K, layout="batch_fastest").cpu().numpy()
     par = np.zeros((29, 1, 7))
     par[:, 0, 0], par[:, 0, 1:4], par[:, 0, 4:7] = mom[:, 0], mom[:, 1:4], mom[:, 4:5]
     cMr = cfk.project_maxwellians(par, L, K)
-    QM = torch.zeros_like(cMr)
-    plan.execute(QM, cMr, 29, precision="fp64_sym", layout="batch_fastest")
-    QM = QM.cpu().numpy()
+    assert float((cMl - cMr).abs().max()) <= 1e-12 * float(cMr.abs().max())
+    QMr = torch.zeros_like(cMl)
+    plan.execute(QMr, cMl, 29, precision="fp64_sym", layout="batch_fastest")
+    assert float((QMl - QMr).abs().max()) <= 1e-13 * float(QMr.abs().max() + 1e-300)
     # the anchor carries the cell's mass, momentum and temperature
-    momM = cfk.moments(cMr, 29, nq, K, layout="batch_fastest").cpu().numpy()
+    momM = cfk.moments(cMl, 29, nq, K, layout="batch_fastest").cpu().numpy()
     assert np.max(np.abs(momM[:, :5] - mom[:, :5])) <= 1e-8
-    f, cMh = fd.cpu().numpy(), cMr.cpu().numpy()
+    # the anchored linear part Q(c_M) + n^2 J g against numpy with the plan's blocks
+    f, cMh, QM = fd.cpu().numpy(), cMl.cpu().numpy(), QMl.cpu().numpy()
     n = f[0]
     g = (f - cMh) / n
     ref = QM.copy()
@@ -141,3 +146,30 @@ def test_anchored_linear_surrogate(torch_cuda):
     Qx[[0, 1, 2, 3, nq]] = 0.0
     QM[[0, 1, 2, 3, nq]] = 0.0
     assert np.linalg.norm(Q - Qx) <= 0.5 * np.linalg.norm(Qx - QM)
+
+
+def test_anchor_coefficients_of_drifting_maxwellians(torch_cuda):
+    """Funk-Hecke anchors against the 3-D projection of the Maxwellian with the same moments, for drifts
+    in every direction (including none and along -z), at L=8, K=10, with the temperatures and drifts of
+    the training mixtures."""
+    torch = torch_cuda
+    L, K = 8, 10
+    nq = (L + 1) ** 2
+    lab, val, ic = cfk.real_gaunt(L)
+    ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
+    R = cfk.build_rtensor(K - 1, L, 1.0, cfk.sumfac_grid_sizes(K - 1, L, 0, 0), geometry=(lab, val, ic))
+    plan = cfk.CollisionPlan(ls, vs, ics, R, nq, K)
+    sur = cfk.Surrogate(plan, width=64, depth=1, seed=0)
+    params = cfk.maxwellian_mixture_params(64, seed=4)
+    params[0, :, 1:4] = 0.0                       # no drift
+    params[1, :, 1:4] = (0.0, 0.0, -0.4)          # along -z
+    f = cfk.project_maxwellians(params, L, K)     # mixtures: their own anchors differ from them
+    cM, QM = sur.anchors(f)
+    mom = cfk.moments(f, 64, nq, K, layout="batch_fastest").cpu().numpy()
+    par = np.zeros((64, 1, 7))
+    par[:, 0, 0], par[:, 0, 1:4], par[:, 0, 4:7] = mom[:, 0], mom[:, 1:4], mom[:, 4:5]
+    cMr = cfk.project_maxwellians(par, L, K)
+    err = float((cM - cMr).abs().max()) / float(cMr.abs().max())
+    print("anchor vs 3-D projection, L=8 K=10: %.2e" % err, flush=True)
+    assert err <= 1e-12
+    assert torch.isfinite(QM).all()
