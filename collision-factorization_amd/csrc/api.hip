@@ -322,9 +322,11 @@ static int default_tile_kc(int precision, int K) {
 }
 
 // The tile a plan runs for a precision, defaults resolved.  The mixed path's default kernel follows
-// the measured crossover (batch 1024-4096 on MI355X): the radial stage on the matrix cores with the
-// row split from K = 12 up (K = 14: 0.97 vs 1.44 ms for 1024 distributions at L = 12), the VALU
-// kernel below (K = 6, 8: the VALU kernel is faster; K = 10: within a few percent).
+// the measured crossover (batch 1024-4096 on MI355X): the radial stage on the matrix cores from K = 12
+// up, the VALU kernel below (K = 6, 8: the VALU kernel is faster; K = 10: the same).  The matrix-core
+// kernel splits rows between two waves at K = 12 (L = 10, 1024 distributions: 0.32 vs 0.37 ms per call)
+// but not at K = 14, where one wave per item already fills the GPU (L = 12, 1062 distributions: 0.91 vs
+// 1.01 ms; 4096: 2.94 vs 2.99 ms) and the batched path's K = 14 class runs 1.05 instead of 1.18 ms.
 Tile plan_tile(const cfk_plan_s *p, int precision) {
     Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
             p->nb[precision] > 0 ? p->nb[precision] : 1, p->pf[precision],
@@ -332,7 +334,7 @@ Tile plan_tile(const cfk_plan_s *p, int precision) {
     if (precision == CFK_PREC_MIXED) {
         if (tl.pf < 0) tl.pf = (p->K >= 12 && mx_supports_k(p->K)) ? 1 : 0;
         if (tl.pf == 1 && !mx_supports_k(p->K)) tl.pf = 0;
-        if (tl.split < 0) tl.split = tl.pf == 1 ? 1 : 0;
+        if (tl.split < 0) tl.split = tl.pf == 1 && p->K <= 12 ? 1 : 0;
         if (tl.pf == 1) {
             tl.kc = mx_chunk(p->K);
             tl.nb = 1;
