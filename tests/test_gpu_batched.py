@@ -222,7 +222,9 @@ def test_multi_rejects_unsupported_groups(torch_cuda):
 def test_each_mixed_kernel_per_k(groups, torch_cuda, pf, split):
     """The single-plan mixed path on each kernel -- radial stage on the matrix cores (pf=1) or on the
     VALU (pf=0), with and without the row split -- for every K of the batched path, against the
-    oracle on a sample and the FP64 GPU operator on the whole ragged batch."""
+    oracle on a sample and the FP64 GPU operator on the whole ragged batch, from batch-fastest and
+    from natural-layout input (the latter packs f from [B][K][N_Q], including the matrix-core kernel's
+    chunk-major rows from K = 12)."""
     torch = torch_cuda
     for i, k in enumerate(CONFIG2 + EXTRA):
         g = groups[k]
@@ -235,11 +237,14 @@ def test_each_mixed_kernel_per_k(groups, torch_cuda, pf, split):
         print("mixed kernel pf=%d split=%d L=%d K=%d" % (pf, split, g["L"], g["K"]), flush=True)
         Q = torch.zeros_like(fT)
         g["plan"].execute(Q, fT, B, precision="mixed", layout="batch_fastest")
+        Qnat = torch.zeros(B * g["nq"] * g["K"], dtype=torch.float64, device="cuda")
+        g["plan"].execute(Qnat, torch.from_numpy(f).cuda(), B, precision="mixed")
         torch.cuda.synchronize()
         g["plan"].set_tile("mixed")
         g["plan"].set_split("mixed")
         Qr = _rows(Q)
         assert rel_rows(Qr, _rows(ref)) <= TOL_MIXED, k
+        assert rel_rows(Qnat.cpu().numpy().reshape(B, -1), _rows(ref)) <= TOL_MIXED, k
         pick = np.r_[0:4, B - 4:B]
         assert rel_rows(Qr[pick], oracle_q(g, f[pick])) <= TOL_MIXED, k
         assert invariants_zero(Qr, g["nq"]), k
