@@ -236,7 +236,7 @@ int cfk_plan_linearize(cfk_plan plan, const double *f0, double *J, void *stream)
 
 /* Sets one precision's tile (KC, NB, PF as above; KC = 0 and NB = 0 select the defaults) without
  * timing anything.  For MIXED, PF = -1 (the default of a new plan) selects the kernel by K: the
- * matrix-core radial stage with the row split from K = 12 up, the VALU kernel below. */
+ * matrix-core radial stage from K = 12 up (with the row split at K = 12), the VALU kernel below. */
 int cfk_plan_set_tile(cfk_plan plan, int32_t precision, int32_t kc, int32_t nb, int32_t pf);
 
 /* Waves per workgroup of one precision's angular-first kernel: wpb in {1, 2, 4} puts that many batch
@@ -261,8 +261,8 @@ int cfk_plan_set_accumulate(cfk_plan plan, int32_t accumulate);
  * end of the launch; a waves-per-workgroup setting of 2 or more then puts two batch tiles (four waves)
  * in a workgroup.  Ignored for CFK_PREC_FP64.  The autotuner also tries it.  No reference counterpart. */
 int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split);
-/* (split = -1: the default of the selected kernel -- on for the matrix-core MIXED kernel, else off;
- * cfk_plan_get_split reports the resolved value.) */
+/* (split = -1: the default of the selected kernel -- on for the matrix-core MIXED kernel up to K = 12,
+ * else off; cfk_plan_get_split reports the resolved value.) */
 int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out);
 
 /* Routing-table statistics of a plan, for work models: stats[0] N_G, [1] S (distinct (t,q1) slices),
