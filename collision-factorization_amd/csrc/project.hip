@@ -177,7 +177,9 @@ extern "C" int cfk_project_maxwellians(double *c_dev, const double *params_dev, 
         return set_error(CFK_EINVAL, "grid %d x %d x %d out of range (256, 64, 64)", n_rad, n_pol, n_azi);
     const int N_Q = (L + 1) * (L + 1), M2 = 2 * L + 1, NA = n_pol * n_azi;
     const size_t lds = sizeof(double) * ((size_t)N_Q * n_pol + (size_t)M2 * n_azi + NA + (size_t)n_pol * M2 + 3 * NA);
-    if (lds > 64 * 1024) return set_error(CFK_EINVAL, "grid too large for one workgroup (%zu bytes of LDS)", lds);
+    // one workgroup per distribution holds the grid's tables: up to the 160 KB of LDS a gfx950 workgroup
+    // can address (the tutorials' 60 x 30 x 30 grid needs 81 KB at L = 12)
+    if (lds > 160 * 1024) return set_error(CFK_EINVAL, "grid too large for one workgroup (%zu bytes of LDS)", lds);
     if (N_Q > PT) return set_error(CFK_EINVAL, "N_Q=%d exceeds %d", N_Q, PT);
     int rc;
     if ((rc = check_device())) return rc;
@@ -217,6 +219,9 @@ extern "C" int cfk_project_maxwellians(double *c_dev, const double *params_dev, 
     ProjDev d{dt.p + o_rad, dt.p + o_leg, dt.p + o_trig, dt.p + o_r, dt.p + o_mu, dt.p + o_c, dt.p + o_s,
               L, K, N_Q, n_rad, n_pol, n_azi, n_comp};
     hipStream_t st = (hipStream_t)stream;
+    if (lds > 64 * 1024)
+        CFK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&project_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(project_kernel, dim3((unsigned)B), dim3(PT), lds, st, d, params_dev, c_dev, B);
     CFK_HIP(hipGetLastError());
     // the tables are freed on return: wait for the kernel that reads them

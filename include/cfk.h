@@ -337,7 +337,8 @@ int cfk_mlp_adam_step(cfk_mlp mlp, float lr, float beta1, float beta2, float eps
  * (alpha = 1/2) nodes in |v|^2 (weights halved), n_pol Gauss-Legendre nodes in cos(theta) and n_azi
  * uniform phi.  params_dev [B][n_comp][7] = (n, ux, uy, uz, Tx, Ty, Tz) per component (n = 0 skips
  * it); component density n pi^{-3/2} (Tx Ty Tz)^{-1/2} exp(-sum_d (v_d - u_d)^2 / T_d).  Output
- * c_dev [K][N_Q][B] (batch fastest), FP64.  n_comp <= 3, L <= 12, K <= 16.  Waits for the stream.
+ * c_dev [K][N_Q][B] (batch fastest), FP64.  n_comp <= 3, L <= 12, K <= 16; the grid's tables must fit
+ * one workgroup's LDS (160 KB: the 60 x 30 x 30 grid up to L = 12).  Waits for the stream.
  * ------------------------------------------------------------------------------------- */
 int cfk_project_maxwellians(double *c_dev, const double *params_dev, int64_t B, int32_t n_comp, int32_t L, int32_t K,
                             int32_t n_rad, int32_t n_pol, int32_t n_azi, void *stream);

@@ -148,12 +148,12 @@ def test_anchored_linear_surrogate(torch_cuda):
     assert np.linalg.norm(Q - Qx) <= 0.5 * np.linalg.norm(Qx - QM)
 
 
-def test_anchor_coefficients_of_drifting_maxwellians(torch_cuda):
+@pytest.mark.parametrize("L,K", [(8, 10), (12, 14)], ids=["L8_K10", "L12_K14"])
+def test_anchor_coefficients_of_drifting_maxwellians(torch_cuda, L, K):
     """Funk-Hecke anchors against the 3-D projection of the Maxwellian with the same moments, for drifts
-    in every direction (including none and along -z), at L=8, K=10, with the temperatures and drifts of
-    the training mixtures."""
+    in every direction (including none and along -z), with the temperatures and drifts of the training
+    mixtures; Q(c_M) against the symmetrized FP64 operator."""
     torch = torch_cuda
-    L, K = 8, 10
     nq = (L + 1) ** 2
     lab, val, ic = cfk.real_gaunt(L)
     ls, vs, ics = cfk.sort_t_q1(lab, val, ic)
@@ -170,6 +170,9 @@ def test_anchor_coefficients_of_drifting_maxwellians(torch_cuda):
     par[:, 0, 0], par[:, 0, 1:4], par[:, 0, 4:7] = mom[:, 0], mom[:, 1:4], mom[:, 4:5]
     cMr = cfk.project_maxwellians(par, L, K)
     err = float((cM - cMr).abs().max()) / float(cMr.abs().max())
-    print("anchor vs 3-D projection, L=8 K=10: %.2e" % err, flush=True)
+    print("anchor vs 3-D projection, L=%d K=%d: %.2e" % (L, K, err), flush=True)
     assert err <= 1e-12
-    assert torch.isfinite(QM).all()
+    QMr = torch.zeros_like(cM)
+    plan.execute(QMr, cM, 64, precision="fp64_sym", layout="batch_fastest")
+    cfk.conservation_project(QMr, 64, nq, K, layout="batch_fastest")
+    assert float((QM - QMr).abs().max()) <= 1e-13 * float(QMr.abs().max())
