@@ -581,9 +581,30 @@ struct cfk_surrogate_s {
     int exact(double *Q, const double *f, int64_t B, hipStream_t st) {
         int rc;
         if ((rc = fill_f64(Q, 0.0, (int64_t)NK * B, st))) return rc;
-        if ((rc = cfk_plan_execute(plan, CFK_VARIANT_ANGULAR_FIRST, CFK_PREC_FP64_SYM, CFK_LAYOUT_BATCH_FASTEST, Q, f, B,
-                                   st)))
-            return rc;
+        const int64_t chunk = std::min<int64_t>(max_batch(N_Q, K), (int64_t)1 << 17);
+        if (B <= chunk) {
+            if ((rc = cfk_plan_execute(plan, CFK_VARIANT_ANGULAR_FIRST, CFK_PREC_FP64_SYM, CFK_LAYOUT_BATCH_FASTEST, Q, f,
+                                       B, st)))
+                return rc;
+        } else {
+            // more cells than one call takes (a training set): column blocks of the batch-fastest arrays
+            // through contiguous staging copies
+            DevBuf<double> fc, Qc;
+            if ((rc = fc.alloc((size_t)NK * chunk)) || (rc = Qc.alloc((size_t)NK * chunk))) return rc;
+            for (int64_t s0 = 0; s0 < B; s0 += chunk) {
+                const int64_t n = std::min(chunk, B - s0);
+                CFK_HIP(hipMemcpy2DAsync(fc.p, n * sizeof(double), f + s0, B * sizeof(double), n * sizeof(double), NK,
+                                         hipMemcpyDeviceToDevice, st));
+                if ((rc = fill_f64(Qc.p, 0.0, (int64_t)NK * n, st)) ||
+                    (rc = cfk_plan_execute(plan, CFK_VARIANT_ANGULAR_FIRST, CFK_PREC_FP64_SYM, CFK_LAYOUT_BATCH_FASTEST,
+                                           Qc.p, fc.p, n, st)))
+                    return rc;
+                CFK_HIP(hipMemcpy2DAsync(Q + s0, B * sizeof(double), Qc.p, n * sizeof(double), n * sizeof(double), NK,
+                                         hipMemcpyDeviceToDevice, st));
+            }
+            // the staging buffers are freed on return: wait for the copies that read them
+            CFK_HIP(hipStreamSynchronize(st));
+        }
         return cfk_conservation_project(Q, B, N_Q, K, CFK_LAYOUT_BATCH_FASTEST, st);
     }
     int apply(double *Q, const double *f, int64_t B, bool use_net, hipStream_t st) {
