@@ -59,7 +59,7 @@ int cfk_device_count(void);
 
 /* ---------------------------------------------------------------------------------------
  * MEX-equivalent one-shot entry points: host pointers, one distribution, MATLAB layout.
- * A MEX gateway forwards its prhs[] here unchanged.
+ * A MEX gateway forwards its prhs[] here unchanged (mxGetPr of each argument, in order).
  * R_numel is mxGetNumberOfElements(prhs[5]) and bounds ic_map (the reference does not check).
  * ------------------------------------------------------------------------------------- */
 
