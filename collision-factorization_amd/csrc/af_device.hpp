@@ -59,8 +59,17 @@ __device__ __forceinline__ T fload(frsrc_t rs, uint32_t lo, int64_t u) {
 __device__ __forceinline__ float4 fload4(frsrc_t rs, uint32_t lo, int soff) {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, lo, soff, 0));
 }
+// Two consecutive FP64 values, the same way.
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ f64x2_t fload2d(frsrc_t rs, uint32_t lo, int soff) {
+    return __builtin_bit_cast(f64x2_t, __builtin_amdgcn_raw_buffer_load_b128(rs, lo, soff, 0));
+}
 #else
 typedef const void *frsrc_t;
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+__device__ inline f64x2_t fload2d(frsrc_t rs, uint32_t lo, int soff) {
+    return *(const f64x2_t *)((const char *)rs + lo + soff);
+}
 __device__ inline float4 fload4(frsrc_t rs, uint32_t lo, int soff) {
     return *(const float4 *)((const char *)rs + lo + soff);
 }
@@ -86,5 +95,12 @@ __device__ __forceinline__ float in_vgpr(float x) {
     asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
     return r;
 }
+// (FP64 FMAs issue one wave at a time whatever their operands: no copy.)
+__device__ __forceinline__ float to_vgpr(float x) { return in_vgpr(x); }
+__device__ __forceinline__ double to_vgpr(double x) { return x; }
+
+// Bytes of one chunk of KC coefficients in the packed rows: KC rounded up to whole 16-byte loads.
+template <typename T>
+constexpr int pk_chunk_bytes(int KC) { return (KC * (int)sizeof(T) + 15) / 16 * 16; }
 
 }  // namespace cfk

@@ -56,6 +56,9 @@ int launch_af_sym_f32(const cfk_plan_s *p, const Tile &tl, const float *fT, doub
 int launch_transpose_f32(const double *in, float *out, int64_t rows, int64_t cols, hipStream_t st);
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
 int launch_pack_rows(const double *in, float *out, int N_Q, int K, int KC, int64_t B, bool natural, hipStream_t st);
+int launch_pack_rows_f64(const double *in, double *out, int N_Q, int K, int KC, int64_t B, bool natural,
+                         hipStream_t st);
+int af_f64_sym_kc(int K, const Tile &tl);
 bool af_f32_packed(int K, const Tile &tl);
 bool mx_supports_k(int K);
 int mx_chunk(int K);
@@ -358,6 +361,11 @@ StagingNeed plan_staging_need(const cfk_plan_s *p, int precision, int layout, in
         const int kc = tl.pf == 1 ? mx_chunk(p->K) : p->K;
         n.f32 = packed ? (int64_t)p->N_Q * rl_of(p->K, kc) * B : NK * B;
         if (natural) n.q64 = NK * B;
+    } else if (precision == CFK_PREC_FP64_SYM) {
+        // packed FP64 rows (chunk-major, from either layout)
+        const int kc = af_f64_sym_kc(p->K, plan_tile(p, precision));
+        n.f64 = (int64_t)p->N_Q * ((p->K + kc - 1) / kc) * ((kc + 1) / 2 * 2) * B;
+        if (natural) n.q64 = NK * B;
     } else if (natural) {
         n.f64 = NK * B;
         n.q64 = NK * B;
@@ -440,10 +448,26 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
         return CFK_OK;
     }
     const bool sym = precision == CFK_PREC_FP64_SYM;
+    if (sym) {
+        // the symmetrized FP64 kernel reads f as chunk-major packed FP64 rows: one 16-byte load per two
+        // coefficients at one scalar offset per row chunk, instead of one 8-byte load per coefficient
+        if (!stg) {
+            if ((rc = p->fT.ensure((size_t)need.f64))) return rc;
+            s_f64 = p->fT.p;
+            if (natural) {
+                if ((rc = p->QT.ensure((size_t)need.q64))) return rc;
+                s_q64 = p->QT.p;
+            }
+        }
+        if ((rc = launch_pack_rows_f64(f, s_f64, p->N_Q, p->K, af_f64_sym_kc(p->K, tl), B, natural, st))) return rc;
+        KernelTimer tm(p, st, stg == nullptr);
+        if ((rc = launch_af_sym_f64(p, tl, s_f64, natural ? s_q64 : Q, B, natural ? 0 : p->accumulate, st))) return rc;
+        tm.stop();
+        return natural ? launch_transpose(s_q64, Q, NK, B, p->accumulate != 0, st) : CFK_OK;
+    }
     if (!natural) {
         KernelTimer tm(p, st, stg == nullptr);
-        rc = sym ? launch_af_sym_f64(p, tl, f, Q, B, p->accumulate, st)
-                 : launch_sparse_fp64(p, variant, tl, f, Q, B, p->accumulate, st);
+        rc = launch_sparse_fp64(p, variant, tl, f, Q, B, p->accumulate, st);
         if (rc) return rc;
         tm.stop();
         return CFK_OK;
@@ -457,8 +481,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if ((rc = launch_transpose(f, s_f64, B, NK, false, st))) return rc;
     {
         KernelTimer tm(p, st, stg == nullptr);
-        rc = sym ? launch_af_sym_f64(p, tl, s_f64, s_q64, B, 0, st)
-                 : launch_sparse_fp64(p, variant, tl, s_f64, s_q64, B, 0, st);
+        rc = launch_sparse_fp64(p, variant, tl, s_f64, s_q64, B, 0, st);
         if (rc) return rc;
         tm.stop();
     }

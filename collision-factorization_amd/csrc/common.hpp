@@ -93,11 +93,19 @@ constexpr int MULTI_MAX_GROUPS = 8;
 // at most 4 GiB (8-byte coefficients counted), and the mixed kernels' packed FP32 rows (RL floats per
 // (q, distribution): K rounded up to 4, or the matrix-core kernel's chunk-packed rows) take a signed
 // 32-bit offset.
+// The symmetrized FP64 kernel's packed FP64 rows (chunks of K, K/2 or K/3 coefficients, each rounded up
+// to 2) span at most 4 GiB.
 inline int64_t max_batch(int N_Q, int K) {
     const int64_t a = (((int64_t)1 << 32) - 1) / (8 * (int64_t)N_Q * K);
     const int rl = rl_of(K, mx_chunk(K)) > (K + 3) / 4 * 4 ? rl_of(K, mx_chunk(K)) : (K + 3) / 4 * 4;
     const int64_t b = (((int64_t)1 << 31) - 1) / (4 * (int64_t)N_Q * rl);
-    return a < b ? a : b;
+    int rd = 0;
+    for (const int kc : {K, (K + 1) / 2, (K + 2) / 3}) {
+        const int r = (K + kc - 1) / kc * ((kc + 1) / 2 * 2);
+        rd = r > rd ? r : rd;
+    }
+    const int64_t c = (((int64_t)1 << 32) - 1) / (8 * (int64_t)N_Q * rd);
+    return a < b ? (a < c ? a : c) : (b < c ? b : c);
 }
 
 // One (t, q1) slice of the (t,q1)-sorted COO: entries [z0, z1) share t and q1.  In the device copy

@@ -122,7 +122,7 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
     for (int i = 0; i < NB; ++i) {
         const int64_t b = (tile * NB + i) * WAVE + (threadIdx.x % WAVE);
         act[i] = b < B;
-        lo[i] = act[i] ? (uint32_t)b * (uint32_t)(PK ? kp_of(K) * 4 : sizeof(T)) : 0u;
+        lo[i] = act[i] ? (uint32_t)b * (uint32_t)(PK ? pk_chunk_bytes<T>(KC) : sizeof(T)) : 0u;
         lq[i] = act[i] ? (uint32_t)b * 8u : 0u;
     }
     int s0 = row_ptr_c[row], s1 = row_ptr_c[row + 1];
@@ -134,19 +134,45 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
             s0 = mid;
     }
     const int64_t kstride = (int64_t)N_Q * B;  // [K][N_Q][B]
-    static_assert(!PK || (std::is_same<T, float>::value && KC == K && GROUPED), "packed rows: FP32, full chunk, grouped");
-    constexpr int KP = kp_of(K);
-    const frsrc_t frs = make_frsrc(fT_in, (uint32_t)((PK ? KP : K) * kstride * (int64_t)sizeof(T)));
-    // PK: the K coefficients of row q (batch-fastest packed layout [N_Q][B][KP]) of lane's distribution i
-    auto load_row = [&](int q, int i, T (&dst)[K]) {
-        const int soff = q * (int)B * (KP * 4);
+    static_assert(!PK || GROUPED, "packed rows: grouped tables");
+    // PK: f as chunk-major packed rows -- chunk c of row (q, b) is KCQ values (KC rounded up to whole
+    // 16-byte loads, zero padded) at element ((q NCH + c) B + b) KCQ, so one chunk is KCQ / VPL 16-byte
+    // loads of the lane's own row (VPL = 4 floats or 2 doubles).  With KC = K in FP32 this is [N_Q][B][KP].
+    constexpr int VPL = 16 / (int)sizeof(T), KCQ = (KC + VPL - 1) / VPL * VPL, NCH = (K + KC - 1) / KC;
+    const uint32_t fbytes = PK ? (uint32_t)std::min<int64_t>((int64_t)N_Q * NCH * B * KCQ * (int64_t)sizeof(T), 0xFFFFFFFFLL)
+                               : (uint32_t)(K * kstride * (int64_t)sizeof(T));
+    const frsrc_t frs = make_frsrc(fT_in, fbytes);
+    // byte offset of chunk ch of row q (unsigned: a buffer load's scalar offset is an unsigned 32-bit value)
+    auto chunk_off = [&](int q, int ch) -> int {
+        return (int)((uint32_t)(q * NCH + ch) * (uint32_t)B * (uint32_t)(KCQ * (int)sizeof(T)));
+    };
+    // the KC coefficients of chunk ch of row q of lane's distribution i (zero past K)
+    auto load_chunk = [&](int q, int ch, int i, T (&dst)[KC]) {
+        const int soff = chunk_off(q, ch);
 #pragma unroll
-        for (int c = 0; c < KP / 4; ++c) {
-            const float4 v = fload4(frs, lo[i] + 16u * c, soff);
-            if (4 * c + 0 < K) dst[4 * c + 0] = v.x;
-            if (4 * c + 1 < K) dst[4 * c + 1] = v.y;
-            if (4 * c + 2 < K) dst[4 * c + 2] = v.z;
-            if (4 * c + 3 < K) dst[4 * c + 3] = v.w;
+        for (int c = 0; c < KCQ / VPL; ++c) {
+            if constexpr (std::is_same<T, float>::value) {
+                const float4 v = fload4(frs, lo[i] + 16u * c, soff);
+                const float ve[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (4 * c + e < KC) dst[4 * c + e] = ve[e];
+            } else {
+                const f64x2_t d = fload2d(frs, lo[i] + 16u * c, soff);
+                if (2 * c < KC) dst[2 * c] = d[0];
+                if (2 * c + 1 < KC) dst[2 * c + 1] = d[1];
+            }
+        }
+    };
+    // all K coefficients of row q
+    auto load_row = [&](int q, int i, T (&dst)[K]) {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            T part[KC];
+            load_chunk(q, ch, i, part);
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+                if (ch * KC + j < K) dst[ch * KC + j] = part[j];
         }
     };
     double q[NB][K];
@@ -206,9 +232,9 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
 #pragma unroll
                                 for (int i = 0; i < NB; ++i) {
                                     if constexpr (PK) {
-                                        T f3[K];
-                                        load_row(qv[e].y, i, f3);
-                                        const T gg = in_vgpr(gv[e]);
+                                        T f3[KC];
+                                        load_chunk(qv[e].y, c0 / KC, i, f3);
+                                        const T gg = to_vgpr(gv[e]);
 #pragma unroll
                                         for (int j = 0; j < KC; ++j) h[i][j] = fma(gg, f3[j], h[i][j]);
                                     } else {
@@ -281,9 +307,9 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
 #pragma unroll
                             for (int i = 0; i < NB; ++i) {
                                 if constexpr (PK) {
-                                    T f3[K];
-                                    load_row(qq.y, i, f3);
-                                    const T gg = in_vgpr(g);
+                                    T f3[KC];
+                                    load_chunk(qq.y, c0 / KC, i, f3);
+                                    const T gg = to_vgpr(g);
 #pragma unroll
                                     for (int j = 0; j < KC; ++j) h[i][j] = fma(gg, f3[j], h[i][j]);
                                 } else {
@@ -489,7 +515,8 @@ __global__ __launch_bounds__(4 * WAVE) void af_kernel(const T *__restrict__ fT, 
                                                   const T *__restrict__ zg, const double *__restrict__ R,
                                                   const float *__restrict__ R32, int64_t B, int N_Q, int accumulate,
                                                   const int32_t *__restrict__ row_mid) {
-    constexpr bool PK = std::is_same<T, float>::value && KC == K && GROUPED;
+    // packed rows: the symmetrized FP64 kernels (any chunk) and the FP32 kernels with a full k3 chunk
+    constexpr bool PK = GROUPED && (std::is_same<T, double>::value || KC == K);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     int64_t tile;
     int half = -1;
@@ -712,6 +739,36 @@ __global__ __launch_bounds__(256) void pack_rows_pair_kernel(const double *__res
     }
 }
 
+// FP64 f -> packed FP64 rows for the symmetrized FP64 kernels: chunk c of row (q, b) is KCQ = KC rounded up
+// to 2 doubles at element ((q NCH + c) B + b) KCQ (zero padded).  One thread per (q, distribution) row,
+// 16-byte stores; NATURAL as in pack_rows_kernel.
+template <bool NATURAL>
+__global__ __launch_bounds__(256) void pack_rows_f64_kernel(const double *__restrict__ in, double *__restrict__ out,
+                                                            int N_Q, int K, int KC, int64_t B) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)N_Q * B) return;
+    int64_t q, b;
+    if (NATURAL) {
+        b = i / N_Q;
+        q = i - b * N_Q;
+    } else {
+        q = i / B;
+        b = i - q * B;
+    }
+    const int KCQ = (KC + 1) / 2 * 2, NCH = (K + KC - 1) / KC;
+    for (int c = 0; c < NCH; ++c) {
+        double2 *o = reinterpret_cast<double2 *>(out + ((q * NCH + c) * B + b) * KCQ);
+        for (int v2 = 0; v2 < KCQ / 2; ++v2) {
+            double v[2];
+            for (int r = 0; r < 2; ++r) {
+                const int j = 2 * v2 + r, k = c * KC + j;
+                v[r] = (j < KC && k < K) ? (NATURAL ? in[(b * K + k) * N_Q + q] : in[((int64_t)k * N_Q + q) * B + b]) : 0.0;
+            }
+            o[v2] = make_double2(v[0], v[1]);
+        }
+    }
+}
+
 __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int layout) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
@@ -919,6 +976,26 @@ int launch_pack_rows(const double *in, float *out, int N_Q, int K, int KC, int64
 bool af_f32_packed(int K, const Tile &tl) {
     const int kc = CFK_TUNED_K(K) ? (tl.kc >= K ? K : kc_half(K)) : kc_default(K);
     return kc == K;
+}
+
+int launch_pack_rows_f64(const double *in, double *out, int N_Q, int K, int KC, int64_t B, bool natural,
+                         hipStream_t st) {
+    const int64_t n = (int64_t)N_Q * B;
+    if (n <= 0) return CFK_OK;
+    if (KC < 1 || KC > K) return set_error(CFK_EUNSUP, "packed FP64 rows: K=%d KC=%d", K, KC);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (natural)
+        hipLaunchKernelGGL(pack_rows_f64_kernel<true>, grid, dim3(256), 0, st, in, out, N_Q, K, KC, B);
+    else
+        hipLaunchKernelGGL(pack_rows_f64_kernel<false>, grid, dim3(256), 0, st, in, out, N_Q, K, KC, B);
+    CFK_HIP(hipGetLastError());
+    return CFK_OK;
+}
+
+// The k3 chunk the symmetrized FP64 launch for this tile runs with; mirrors the selection of launch_af_k.
+int af_f64_sym_kc(int K, const Tile &tl) {
+    if (!CFK_TUNED_K(K)) return kc_default(K);
+    return tl.kc >= K ? K : kc_part<double>(K);
 }
 
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st) {

@@ -40,10 +40,18 @@ struct cfk_multi_s {
     std::vector<double *> key_Q;
     std::vector<int64_t> key_B;
     std::vector<int> key_acc;  // each plan's output mode (cfk_plan_set_accumulate) at capture
-    DevBuf<int4> items;
-    DevBuf<MultiGroupDev> groups;
+    std::unique_ptr<DevBuf<int4>> items{new DevBuf<int4>()};
+    std::unique_ptr<DevBuf<MultiGroupDev>> groups{new DevBuf<MultiGroupDev>()};
     bool use_graph = true;
     bool prepared = false;
+    // Once a graph has been captured it is neither destroyed nor captured again before the multi plan
+    // is: a call with other pointers, sizes or output modes keeps the graph and the buffers it
+    // references (below) and runs the same kernels by direct launches from then on.  (Destroying the
+    // executable graph and capturing a new one during the plan's life crashed the host process.)
+    bool frozen = false;
+    std::vector<std::unique_ptr<DevBuf<int4>>> kept_items;
+    std::vector<std::unique_ptr<DevBuf<MultiGroupDev>>> kept_groups;
+    std::vector<std::unique_ptr<DevBuf<float>>> kept_f32;
     struct Launch {
         int K, KC, i0, i1;
         bool mx, split;  // radial stage on the matrix cores; two waves per item
@@ -54,6 +62,7 @@ struct cfk_multi_s {
     std::vector<int> pack_kc;  // per group: the k3 chunk width of its packed rows
     int n_items = 0;
     ~cfk_multi_s() {
+        if (launched) (void)hipEventSynchronize(launched);  // the last launch (graph or direct) is done
         reset();
         if (launched) (void)hipEventDestroy(launched);
         if (cap) (void)hipStreamDestroy(cap);
@@ -76,7 +85,18 @@ struct cfk_multi_s {
 
 static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *f, const int64_t *B) {
     const int G = (int)m->plans.size();
-    m->reset();
+    // the buffers below are replaced: wait for the last launch that reads them (graph or direct)
+    if (m->launched) (void)hipEventSynchronize(m->launched);
+    if (m->exec && !m->frozen) {
+        // keep the captured graph and what it reads
+        m->frozen = true;
+        m->kept_items.push_back(std::move(m->items));
+        m->kept_groups.push_back(std::move(m->groups));
+        m->items.reset(new DevBuf<int4>());
+        m->groups.reset(new DevBuf<MultiGroupDev>());
+        for (auto &b : m->f32) m->kept_f32.push_back(std::move(b));
+        m->f32.clear();
+    }
     m->prepared = false;
     // K-classes: groups that share the item kernel -- (K, KC) on the VALU kernel, (K, split) on the
     // matrix-core one -- as each plan's mixed-precision tile selects it
@@ -131,7 +151,7 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
         off.push_back((int)items.size());
     }
     m->n_items = (int)items.size();
-    if ((rc = m->items.upload(items.data(), items.size()))) return rc;
+    if ((rc = m->items->upload(items.data(), items.size()))) return rc;
     m->f32.clear();
     m->packed.assign(G, 0);
     m->pack_kc.assign(G, 0);
@@ -151,7 +171,7 @@ static int multi_prepare(cfk_multi_s *m, double *const *Q, const double *const *
                               p->sym_zg32.p, p->sym_R.p, p->sym_R32.p, mx ? p->sym_Rmx.p : nullptr,
                               p->sym_row_mid.p, B[g], p->N_Q, p->K, p->accumulate};
     }
-    if ((rc = m->groups.upload(gd.data(), gd.size()))) return rc;
+    if ((rc = m->groups->upload(gd.data(), gd.size()))) return rc;
     const int nc = (int)cls.size();
     m->launches.clear();
     for (int c = 0; c < nc; ++c) {
@@ -187,8 +207,8 @@ static int multi_enqueue(cfk_multi_s *m, hipStream_t st) {
     for (int c = 0; c < nc; ++c) {
         const cfk_multi_s::Launch &l = m->launches[c];
         CFK_HIP(hipStreamWaitEvent(m->branch[c], m->ev[MULTI_MAX_GROUPS], 0));
-        if ((rc = l.mx ? launch_multi_mx(m->groups.p, m->items.p, l.K, l.split, l.i0, l.i1, m->branch[c])
-                       : launch_multi(m->groups.p, m->items.p, l.K, l.KC, l.i0, l.i1, m->branch[c])))
+        if ((rc = l.mx ? launch_multi_mx(m->groups->p, m->items->p, l.K, l.split, l.i0, l.i1, m->branch[c])
+                       : launch_multi(m->groups->p, m->items->p, l.K, l.KC, l.i0, l.i1, m->branch[c])))
             return rc;
         CFK_HIP(hipEventRecord(m->ev[c], m->branch[c]));
         CFK_HIP(hipStreamWaitEvent(st, m->ev[c], 0));
@@ -265,10 +285,13 @@ int cfk_multi_execute(cfk_multi m, double *const *Q_dev, const double *const *f_
                                  [](int a, const cfk_plan_s *p) { return a == p->accumulate; });
     int rc;
     if (!same && (rc = multi_prepare(m, Q_dev, f_dev, B))) return rc;
-    if (!m->use_graph) return multi_enqueue(m, (hipStream_t)stream);
-    if (!m->exec && (rc = multi_capture(m))) return rc;
-    CFK_HIP(hipGraphLaunch(m->exec, (hipStream_t)stream));
     if (!m->launched) CFK_HIP(hipEventCreateWithFlags(&m->launched, hipEventDisableTiming));
+    if (!m->use_graph || m->frozen) {
+        if ((rc = multi_enqueue(m, (hipStream_t)stream))) return rc;
+    } else {
+        if (!m->exec && (rc = multi_capture(m))) return rc;
+        CFK_HIP(hipGraphLaunch(m->exec, (hipStream_t)stream));
+    }
     CFK_HIP(hipEventRecord(m->launched, (hipStream_t)stream));
     return CFK_OK;
 }

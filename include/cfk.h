@@ -296,9 +296,10 @@ int cfk_multi_destroy(cfk_multi multi);
  * (group, row, batch tile) items, ordered heaviest first within each radial size K; one kernel per
  * K runs its items (one workgroup per item, dispatched in that order), the kernels of different K
  * on parallel graph branches.  The conversions and the kernels are captured into a hipGraph on the
- * first call for a given set of pointers and sizes; later calls with the same set replay the graph
- * on `stream` (a change of a plan's output mode captures anew).  Each B[g] obeys cfk_plan_execute's
- * per-call limit. */
+ * first call; later calls with the same pointers, sizes and output modes replay it on `stream`.  The
+ * graph is captured once per multi plan: after a call with another set, every call enqueues the same
+ * kernels directly (the captured graph and the buffers it reads are kept until cfk_multi_destroy).
+ * Each B[g] obeys cfk_plan_execute's per-call limit. */
 int cfk_multi_execute(cfk_multi multi, double *const *Q_dev, const double *const *f_dev, const int64_t *B,
                       void *stream);
 

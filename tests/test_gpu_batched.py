@@ -177,7 +177,8 @@ def test_multi_full_config2_mix(groups, torch_cuda):
 
 def test_multi_overwrite_mode(groups, torch_cuda):
     """With every plan in overwrite mode the multi plan writes Q = Q(f) without reading Q (Q starts as
-    NaN) and gives the rows it gives when accumulating into zeros; switching the mode back re-captures."""
+    NaN) and gives the rows it gives when accumulating into zeros; after the first call the graph is
+    kept and the later sets (other Q buffers, other modes) run by direct launches."""
     torch = torch_cuda
     keys = list(CONFIG2)
     fs, Bs = [], [37, 64, 5, 70]
