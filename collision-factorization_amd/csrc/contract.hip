@@ -707,35 +707,52 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(const double *__restrict
     }
 }
 
-// Batch-fastest input with B even (and 16-byte aligned): one thread per (q, pair of distributions),
-// the K 16-byte loads of the two distributions' coefficients issued together, then each chunk of the
-// two rows written as 2 KCP / 4 float4.
+// Batch-fastest input with B even (and 16-byte aligned): one thread per (q, pair of distributions), the
+// K 16-byte loads of the two distributions' coefficients issued together.  Each chunk of the
+// workgroup's 512 rows goes through LDS, so that consecutive threads store consecutive 16-byte pieces of
+// the packed rows (a thread writing its own two rows would store at a 2 KCP-float stride).
 template <int K, int KC>
 __global__ __launch_bounds__(256) void pack_rows_pair_kernel(const double *__restrict__ in, float *__restrict__ out,
                                                              int N_Q, int64_t B) {
-    constexpr int KCP = kcp_of(KC), NCH = (K + KC - 1) / KC;
+    constexpr int KCP = kcp_of(KC), NCH = (K + KC - 1) / KC, V4 = KCP / 4;
+    __shared__ float4 st[512 * V4];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t half = B / 2;
-    if (i >= (int64_t)N_Q * half) return;
-    const int64_t q = i / half, b = 2 * (i - q * half);
+    const int64_t half = B / 2, n = (int64_t)N_Q * half;
+    const bool act = i < n;
     double2 v[K];
+    if (act) {
+        const int64_t q = i / half, b = 2 * (i - q * half);
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = *reinterpret_cast<const double2 *>(in + ((int64_t)k * N_Q + q) * B + b);
+        for (int k = 0; k < K; ++k) v[k] = *reinterpret_cast<const double2 *>(in + ((int64_t)k * N_Q + q) * B + b);
+    }
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        float4 *o = reinterpret_cast<float4 *>(out + ((q * NCH + c) * B + b) * KCP);
+        if (act) {
 #pragma unroll
-        for (int v4 = 0; v4 < KCP / 4; ++v4) {
-            float w0[4], w1[4];
+            for (int v4 = 0; v4 < V4; ++v4) {
+                float w0[4], w1[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int j = 4 * v4 + e, k = c * KC + j;
-                w0[e] = (j < KC && k < K) ? (float)v[j < KC && k < K ? k : 0].x : 0.0f;
-                w1[e] = (j < KC && k < K) ? (float)v[j < KC && k < K ? k : 0].y : 0.0f;
+                for (int e = 0; e < 4; ++e) {
+                    const int j = 4 * v4 + e, k = c * KC + j;
+                    w0[e] = (j < KC && k < K) ? (float)v[j < KC && k < K ? k : 0].x : 0.0f;
+                    w1[e] = (j < KC && k < K) ? (float)v[j < KC && k < K ? k : 0].y : 0.0f;
+                }
+                st[(2 * threadIdx.x) * V4 + v4] = make_float4(w0[0], w0[1], w0[2], w0[3]);
+                st[(2 * threadIdx.x + 1) * V4 + v4] = make_float4(w1[0], w1[1], w1[2], w1[3]);
             }
-            o[v4] = make_float4(w0[0], w0[1], w0[2], w0[3]);
-            o[KCP / 4 + v4] = make_float4(w1[0], w1[1], w1[2], w1[3]);
         }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 2 * V4; ++t) {
+            const int o = (int)threadIdx.x + 256 * t;  // 16-byte piece o of the workgroup's 512 rows
+            const int r = o / V4, v4 = o - r * V4;
+            const int64_t ir = (int64_t)blockIdx.x * 256 + r / 2;
+            if (ir < n) {
+                const int64_t q = ir / half, b = 2 * (ir - q * half) + (r & 1);
+                reinterpret_cast<float4 *>(out + ((q * NCH + c) * B + b) * KCP)[v4] = st[o];
+            }
+        }
+        __syncthreads();
     }
 }
 
