@@ -267,6 +267,7 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                     }
                 }
                 bool have = false;  // wave-uniform: a group is open
+                constexpr bool F2N = PK && std::is_same<T, float>::value && NB == 1;
                 for (int zb = zmain; zb < sl.z1; zb += ZB) {
                     int2 qv[ZB];
                     T gv[ZB];
@@ -277,7 +278,45 @@ __device__ __forceinline__ void af_item(const T *__restrict__ fT_in, double *__r
                     }
 #pragma unroll
                     for (int e = 0; e < ZB; ++e) {
-                        if (zb + e < sl.z1) {
+                        if (zb + e < sl.z1 && F2N) {
+                            // Packed FP32 rows: the entry's f3 chunk and, for a new group, its f2 row
+                            // are requested before the close of the previous group, whose products then
+                            // cover their latency (the new row goes to f2n while the close reads f2).
+                            const int2 qq = qv[e];
+                            const T gg = to_vgpr(gv[e]);
+                            T f3[NB][KC];
+#pragma unroll
+                            for (int i = 0; i < NB; ++i) load_chunk(qq.y, c0 / KC, i, f3[i]);
+                            if (qq.x < 0) {
+                                T f2n[NB][K];
+#pragma unroll
+                                for (int i = 0; i < NB; ++i) load_row(~qq.x, i, f2n[i]);
+                                if (have) {
+#pragma unroll
+                                    for (int i = 0; i < NB; ++i)
+#pragma unroll
+                                        for (int j = 0; j < KC; ++j) {
+#pragma unroll
+                                            for (int k2 = 0; k2 < K; ++k2)
+                                                phi[i][k2][j] = fma(f2[i][k2], h[i][j], phi[i][k2][j]);
+                                            h[i][j] = T(0);
+                                        }
+                                }
+                                have = true;
+#pragma unroll
+                                for (int i = 0; i < NB; ++i) {
+#pragma unroll
+                                    for (int j = 0; j < KC; ++j) h[i][j] = fma(gg, f3[i][j], h[i][j]);
+#pragma unroll
+                                    for (int k2 = 0; k2 < K; ++k2) f2[i][k2] = f2n[i][k2];
+                                }
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < NB; ++i)
+#pragma unroll
+                                    for (int j = 0; j < KC; ++j) h[i][j] = fma(gg, f3[i][j], h[i][j]);
+                            }
+                        } else if (zb + e < sl.z1) {
                             const int2 qq = qv[e];
                             const T g = gv[e];
                             if (qq.x < 0) {  // new q2 group: close the open one, load f[q2,:]

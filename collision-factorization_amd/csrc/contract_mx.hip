@@ -242,7 +242,18 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
             for (int e = 0; e < ZB; ++e) {
                 if (zb + e < sl.z1) {
                     const int2 qq = qv[e];
+                    // The entry's f3 chunk and, for a new group, its f2 row are requested before the
+                    // close of the previous group, whose products then cover their latency (the new
+                    // row goes to f2n while the close still reads f2).  At K = 12 the second row
+                    // costs the fourth wave per SIMD (121 -> 133 VGPRs) and measured slower: there the
+                    // row is loaded after the close.
+                    constexpr bool F2N = K != 12;
+                    const float gg = in_vgpr(gv[e]);
+                    float f3[KC];
+                    load_chunk(qq.y, ch, f3);
                     if (qq.x < 0) {
+                        float f2n[K];
+                        if constexpr (F2N) load_row(~qq.x, f2n);
                         if (have) {
 #pragma unroll
                             for (int j = 0; j < KC; ++j) {
@@ -252,13 +263,18 @@ __device__ __forceinline__ void af_mx_item(const float *__restrict__ fT_in, doub
                             }
                         }
                         have = true;
-                        load_row(~qq.x, f2);
-                    }
-                    float f3[KC];
-                    load_chunk(qq.y, ch, f3);
-                    const float gg = in_vgpr(gv[e]);
 #pragma unroll
-                    for (int j = 0; j < KC; ++j) h[j] = fmaf(gg, f3[j], h[j]);
+                        for (int j = 0; j < KC; ++j) h[j] = fmaf(gg, f3[j], h[j]);
+                        if constexpr (F2N) {
+#pragma unroll
+                            for (int k2 = 0; k2 < K; ++k2) f2[k2] = f2n[k2];
+                        } else {
+                            load_row(~qq.x, f2);
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < KC; ++j) h[j] = fmaf(gg, f3[j], h[j]);
+                    }
                 }
             }
         }
