@@ -553,24 +553,34 @@ __global__ __launch_bounds__(4 * WAVE) void af_kernel(const T *__restrict__ fT, 
                                                   const int32_t *__restrict__ row_order, const int2 *__restrict__ zq,
                                                   const T *__restrict__ zg, const double *__restrict__ R,
                                                   const float *__restrict__ R32, int64_t B, int N_Q, int accumulate,
-                                                  const int32_t *__restrict__ row_mid) {
+                                                  const int32_t *__restrict__ row_mid, int nx, int gx) {
     // packed rows: the symmetrized FP64 kernels (any chunk) and the FP32 kernels with a full k3 chunk
     constexpr bool PK = GROUPED && (std::is_same<T, double>::value || KC == K);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    // Workgroup index -> (column workgroup xw, output row).  The nx column workgroups are cut into groups
+    // of gx; the grid runs group by group, each group's rows heaviest first and, within a row, its column
+    // workgroups in order.  Workgroups dispatch in index order, so the waves in flight read the f rows of
+    // one group's distributions only, which stay in the XCDs' L2 (the whole batch's rows do not beyond
+    // a few thousand distributions).
+    const int64_t per = (int64_t)gx * N_Q, g = (int64_t)blockIdx.x / per, rem = (int64_t)blockIdx.x - g * per;
+    const int64_t x0 = g * gx;
+    const int gw = (int)(nx - x0 < gx ? nx - x0 : gx);
+    const int rix = (int)(rem / gw);
+    const int64_t xw = x0 + (rem - (int64_t)rix * gw);
     int64_t tile;
     int half = -1;
     if (row_mid) {
         // row-split launch: blockDim.x / 128 batch tiles per workgroup, two waves per tile, each wave half
         // of the row's slices.  Every wave reaches the barrier in af_item (a tile past the batch end runs
         // with all lanes inactive).
-        tile = (int64_t)blockIdx.x * (blockDim.x / (2 * WAVE)) + (w >> 1);
+        tile = xw * (blockDim.x / (2 * WAVE)) + (w >> 1);
         half = w & 1;
     } else {
-        tile = (int64_t)blockIdx.x * (blockDim.x / WAVE) + w;
+        tile = xw * (blockDim.x / WAVE) + w;
         if (tile * NB * WAVE >= B) return;  // wave-uniform; no barrier without a row split
     }
     af_item<T, K, KC, GROUPED, PROJ, NB, PF, PK>(fT, QT, slices, row_ptr, zq, zg, R, R32, B, N_Q, accumulate,
-                                                 row_order[blockIdx.y], tile, row_mid, half);
+                                                 row_order[rix], tile, row_mid, half);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -892,6 +902,20 @@ __global__ void conservation_kernel(double *Q, int64_t B, int N_Q, int K, int la
 // ---------------------------------------------------------------------------------------------
 namespace {
 
+// Distributions per column group of a grouped-table launch (symmetrized FP64 and mixed).  With the whole
+// batch as one group the waves in flight read the f rows of every distribution: at 16384 distributions
+// the symmetrized FP64 kernel took 0.87 ms per 4096 against 0.57 ms at 4096.  Groups of 1024 (8 column
+// workgroups of two batch tiles, one per XCD) measured best at L=8, K=10: mixed 0.257 ms and symmetrized
+// FP64 0.52 ms per 4096 at 4096 distributions, 0.23 / 0.47-0.48 ms per 4096 at 65536 (groups of 4096:
+// 0.266 / 0.57; 2048: 0.265 / 0.54; 512: 0.266 / 0.52).  The plain FP64 kernels run in groups of 4096
+// (at 4096 distributions 1.39-1.44 ms against 1.45-1.52 ms in groups of 1024; at 16384 distributions in
+// one group 1.78 ms per 4096 against 1.29 ms in groups of 4096).
+#ifndef AF_GROUP_COLS
+#define AF_GROUP_COLS 1024
+#endif
+#ifndef AF_GROUP_COLS_PLAIN
+#define AF_GROUP_COLS_PLAIN 4096
+#endif
 template <typename T, int K, int KC, bool GROUPED, int NB, bool PF>
 void launch_af_v(const SparseTables<T> &tb, int N_Q, int wpb, bool split, const T *fT, double *QT, int64_t B,
                  int acc, hipStream_t st) {
@@ -899,17 +923,25 @@ void launch_af_v(const SparseTables<T> &tb, int N_Q, int wpb, bool split, const 
     const int64_t tiles = (B + WAVE * NB - 1) / (WAVE * NB);
     // double runs the plain operator (no projection), float the mixed path (always projected)
     constexpr bool PROJ = std::is_same<T, float>::value;
+    // column groups of whole workgroups: AF_GROUP_COLS distributions for the grouped tables,
+    // AF_GROUP_COLS_PLAIN for the plain FP64 kernels
+    auto group_of = [&](int tpw) {
+        return (int)std::max<int64_t>(1, (GROUPED ? AF_GROUP_COLS : AF_GROUP_COLS_PLAIN) / ((int64_t)WAVE * NB * tpw));
+    };
     if (split && tb.row_mid) {
         const int tpw = wpb >= 2 ? 2 : 1;  // batch tiles per workgroup (two waves each, at most 4 waves)
-        const dim3 grid((unsigned)((tiles + tpw - 1) / tpw), (unsigned)N_Q);
+        const int nx = (int)((tiles + tpw - 1) / tpw);
+        const dim3 grid((unsigned)((int64_t)nx * N_Q));
         hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(2 * WAVE * tpw),
                            (size_t)tpw * NB * K * WAVE * sizeof(double), st, fT, QT, tb.slices, tb.row_ptr,
-                           tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, tb.row_mid);
+                           tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, tb.row_mid, nx, group_of(tpw));
         return;
     }
-    const dim3 grid((unsigned)((tiles + wpb - 1) / wpb), (unsigned)N_Q);
+    const int nx = (int)((tiles + wpb - 1) / wpb);
+    const dim3 grid((unsigned)((int64_t)nx * N_Q));
     hipLaunchKernelGGL((af_kernel<T, K, KC, GROUPED, PROJ, NB, PF>), grid, dim3(WAVE * wpb), 0, st, fT, QT, tb.slices,
-                       tb.row_ptr, tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, (const int32_t *)nullptr);
+                       tb.row_ptr, tb.row_order, tb.zq, tb.zg, tb.R, tb.R32, B, N_Q, acc, (const int32_t *)nullptr, nx,
+                       group_of(wpb));
 }
 
 // k3 chunk candidates: the full width and half of it.
