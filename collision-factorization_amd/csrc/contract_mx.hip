@@ -468,21 +468,29 @@ __global__ __launch_bounds__(4 * WAVE) __attribute__((amdgpu_waves_per_eu(3))) v
                                                        const int2 *__restrict__ zq, const float *__restrict__ zg,
                                                        const double *__restrict__ R, const float *__restrict__ Rmx,
                                                        int64_t B, int N_Q, int accumulate,
-                                                       const int32_t *__restrict__ row_mid) {
+                                                       const int32_t *__restrict__ row_mid, int nx, int gx) {
     extern __shared__ double mx_lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    // Workgroup index -> (column workgroup xw, output row): the nx column workgroups run in groups of gx,
+    // each group's rows heaviest first, so the waves in flight read the f rows of one group only (they
+    // stay in L2).
+    const int64_t per = (int64_t)gx * N_Q, g = (int64_t)blockIdx.x / per, rem = (int64_t)blockIdx.x - g * per;
+    const int64_t x0 = g * gx;
+    const int gw = (int)(nx - x0 < gx ? nx - x0 : gx);
+    const int rix = (int)(rem / gw);
+    const int64_t xw = x0 + (rem - (int64_t)rix * gw);
     int64_t tile;
     int half = -1;
     if (SPLIT) {
         // every wave reaches the barrier of the combine (a tile past the batch end runs with all
         // lanes inactive)
-        tile = (int64_t)blockIdx.x * (blockDim.x / (2 * WAVE)) + (w >> 1);
+        tile = xw * (blockDim.x / (2 * WAVE)) + (w >> 1);
         half = w & 1;
     } else {
-        tile = (int64_t)blockIdx.x * (blockDim.x / WAVE) + w;
+        tile = xw * (blockDim.x / WAVE) + w;
         if (tile * WAVE >= B) return;  // wave-uniform; no barrier without a row split
     }
-    af_mx_item<K, KC, SPLIT>(fT, QT, slices, row_ptr, zq, zg, R, Rmx, B, N_Q, accumulate, row_order[blockIdx.y], tile,
+    af_mx_item<K, KC, SPLIT>(fT, QT, slices, row_ptr, zq, zg, R, Rmx, B, N_Q, accumulate, row_order[rix], tile,
                              row_mid, half, mx_lds);
 }
 
@@ -577,12 +585,14 @@ void launch_mx_k(const cfk_plan_s *p, const float *Rmx, int wpb, const float *fT
         tpw = wpb == 2 || wpb == 4 ? wpb : 1;
         waves = tpw;
     }
-    const dim3 grid((unsigned)((tiles + tpw - 1) / tpw), (unsigned)p->N_Q);
+    // column groups of 1024 distributions (whole workgroups), as the VALU kernels of the grouped tables
+    const int nx = (int)((tiles + tpw - 1) / tpw), gx = std::max(1, 1024 / (WAVE * tpw));
+    const dim3 grid((unsigned)((int64_t)nx * p->N_Q));
     constexpr int rd = mx_region_doubles<K, KC, SPLIT>();
     const size_t lds_bytes = sizeof(double) * (size_t)waves * rd;
     hipLaunchKernelGGL((af_mx_kernel<K, KC, SPLIT>), grid, dim3(WAVE * waves), lds_bytes, st, fT, QT,
                        p->sym_slices.p, p->sym_row_ptr.p, p->sym_row_order.p, p->sym_zq.p, p->sym_zg32.p, p->sym_R.p,
-                       Rmx, B, p->N_Q, acc, SPLIT ? p->sym_row_mid.p : nullptr);
+                       Rmx, B, p->N_Q, acc, SPLIT ? p->sym_row_mid.p : nullptr, nx, gx);
 }
 }  // namespace
 
