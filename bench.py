@@ -29,7 +29,7 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "fp32_vector_sgpr_operand": 78.6e12}
 # from two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) scaled by a plain-copy calibration; the commands
 # and raw figures are in the named profile.  A kernel without an entry here reports traffic as null.
 PROFILED_TRAFFIC = {
-    "valu": (54700 * 1024, "profiles/af_kernel_mixed_hs_l8_k10_b4096.txt"),
+    "valu": (53908 * 1024, "profiles/af_kernel_mixed_hs_l8_k10_b4096.txt"),
     "mfma": (55416 * 1024, "profiles/af_mx_kernel_mixed_hs_l8_k10_b4096.txt"),
 }
 
