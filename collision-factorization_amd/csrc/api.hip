@@ -330,14 +330,28 @@ static int default_tile_kc(int precision, int K) {
 // kernel splits rows between two waves at K = 12 (L = 10, 1024 distributions: 0.32 vs 0.37 ms per call)
 // but not at K = 14, where one wave per item already fills the GPU (L = 12, 1062 distributions: 0.91 vs
 // 1.01 ms; 4096: 2.94 vs 2.99 ms) and the batched path's K = 14 class runs 1.05 instead of 1.18 ms.
-Tile plan_tile(const cfk_plan_s *p, int precision) {
+//
+// Small batches (B <= 1024 when the caller passes B; a surrogate's sampled residual check is 1% of the
+// cells): the waves in flight are far fewer than the GPU's slots, so a call takes as long as its
+// slowest wave.  Unless the plan fixes them, both symmetrized precisions then split rows between two
+// waves, and up to 256 distributions the symmetrized FP64 kernel takes the whole k3 range in one chunk
+// (K <= 10; one pass over the slice's entries instead of two).  Measured at L=8, K=10: 164
+// distributions 0.098 ms in FP64 (0.197 ms with the large-batch tile, 0.113 ms split with two chunks)
+// and 0.065 ms mixed (0.115 ms); 656 distributions 0.17 ms in FP64 split with two chunks (0.29 ms
+// unsplit, 0.26 ms split in one chunk) and 0.081 ms mixed (0.137 ms).
+Tile plan_tile(const cfk_plan_s *p, int precision, int64_t B = 0) {
+    const bool small = B > 0 && B <= 1024;
     Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
             p->nb[precision] > 0 ? p->nb[precision] : 1, p->pf[precision],
             p->wpb[precision] > 0 ? p->wpb[precision] : 1, p->split[precision]};
+    if (precision == CFK_PREC_FP64_SYM && small) {
+        if (p->kc[precision] <= 0 && p->K <= 10 && B <= 256) tl.kc = p->K;
+        if (tl.split < 0) tl.split = 1;
+    }
     if (precision == CFK_PREC_MIXED) {
         if (tl.pf < 0) tl.pf = (p->K >= 12 && mx_supports_k(p->K)) ? 1 : 0;
         if (tl.pf == 1 && !mx_supports_k(p->K)) tl.pf = 0;
-        if (tl.split < 0) tl.split = tl.pf == 1 && p->K <= 12 ? 1 : 0;
+        if (tl.split < 0) tl.split = (tl.pf == 1 && p->K <= 12) || (tl.pf == 0 && small) ? 1 : 0;
         if (tl.pf == 1) {
             tl.kc = mx_chunk(p->K);
             tl.nb = 1;
@@ -356,14 +370,14 @@ StagingNeed plan_staging_need(const cfk_plan_s *p, int precision, int layout, in
     const int64_t NK = (int64_t)p->N_Q * p->K;
     const bool natural = layout == CFK_LAYOUT_NATURAL;
     if (precision == CFK_PREC_MIXED) {
-        const Tile tl = plan_tile(p, precision);
+        const Tile tl = plan_tile(p, precision, B);
         const bool packed = tl.pf == 1 || af_f32_packed(p->K, tl);
         const int kc = tl.pf == 1 ? mx_chunk(p->K) : p->K;
         n.f32 = packed ? (int64_t)p->N_Q * rl_of(p->K, kc) * B : NK * B;
         if (natural) n.q64 = NK * B;
     } else if (precision == CFK_PREC_FP64_SYM) {
         // packed FP64 rows (chunk-major, from either layout)
-        const int kc = af_f64_sym_kc(p->K, plan_tile(p, precision));
+        const int kc = af_f64_sym_kc(p->K, plan_tile(p, precision, B));
         n.f64 = (int64_t)p->N_Q * ((p->K + kc - 1) / kc) * ((kc + 1) / 2 * 2) * B;
         if (natural) n.q64 = NK * B;
     } else if (natural) {
@@ -392,7 +406,7 @@ int plan_execute(cfk_plan_s *p, int variant, int precision, int layout, double *
     if (p->N_G == 0) return CFK_OK;  // no Gaunt entries: Q(f) = 0
     const int64_t NK = (int64_t)p->N_Q * p->K;
     const bool natural = layout == CFK_LAYOUT_NATURAL;
-    const Tile tl = plan_tile(p, precision);
+    const Tile tl = plan_tile(p, precision, B);
     int rc;
     // staging: the caller's (fixed area, no allocation) or the plan's own buffers, grown as needed
     const StagingNeed need = plan_staging_need(p, precision, layout, B);

@@ -162,7 +162,7 @@ struct cfk_plan_s {
     int nb[3] = {0, 0, 0};   // distributions per lane
     int pf[3] = {0, -1, 0};  // FP64: prefetch next nonzero (plain tables); mixed: matrix-core radial stage (-1: by K)
     int wpb[3] = {0, 0, 0};  // waves per workgroup of the angular-first kernels (0: default)
-    int split[3] = {0, -1, 0};  // row split between two waves (symmetrized tables; -1: default of the kernel)
+    int split[3] = {0, -1, -1};  // row split between two waves (symmetrized tables; -1: default of the kernel)
     // kernel timing (cfk_plan_set_timing)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

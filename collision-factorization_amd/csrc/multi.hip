@@ -20,7 +20,7 @@ int launch_multi_mx(const MultiGroupDev *groups, const int4 *items, int K, bool 
 int launch_to_f32(const double *in, float *out, int64_t n, hipStream_t st);
 int launch_pack_rows(const double *in, float *out, int N_Q, int K, int KC, int64_t B, bool natural, hipStream_t st);
 bool multi_supports_k(int K);
-Tile plan_tile(const cfk_plan_s *p, int precision);
+Tile plan_tile(const cfk_plan_s *p, int precision, int64_t B = 0);
 int ensure_mx_tables(cfk_plan_s *p);
 }  // namespace cfk
 
