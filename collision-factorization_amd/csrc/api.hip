@@ -338,7 +338,10 @@ static int default_tile_kc(int precision, int K) {
 // (K <= 10; one pass over the slice's entries instead of two).  Measured at L=8, K=10: 164
 // distributions 0.098 ms in FP64 (0.197 ms with the large-batch tile, 0.113 ms split with two chunks)
 // and 0.065 ms mixed (0.115 ms); 656 distributions 0.17 ms in FP64 split with two chunks (0.29 ms
-// unsplit, 0.26 ms split in one chunk) and 0.081 ms mixed (0.137 ms).
+// unsplit, 0.26 ms split in one chunk) and 0.081 ms mixed (0.137 ms).  The matrix-core kernel at L=12,
+// K=14: 164 distributions 0.32 ms split against 0.52 ms, 656 distributions 0.72 against 0.70 ms.  (With
+// the current kernels and launch order 1062 distributions measured 0.86 ms split against 0.92 ms, the
+// reverse of the figures above; the large-batch default at K = 14 is left as measured there.)
 Tile plan_tile(const cfk_plan_s *p, int precision, int64_t B = 0) {
     const bool small = B > 0 && B <= 1024;
     Tile tl{p->kc[precision] > 0 ? p->kc[precision] : default_tile_kc(precision, p->K),
@@ -351,7 +354,7 @@ Tile plan_tile(const cfk_plan_s *p, int precision, int64_t B = 0) {
     if (precision == CFK_PREC_MIXED) {
         if (tl.pf < 0) tl.pf = (p->K >= 12 && mx_supports_k(p->K)) ? 1 : 0;
         if (tl.pf == 1 && !mx_supports_k(p->K)) tl.pf = 0;
-        if (tl.split < 0) tl.split = (tl.pf == 1 && p->K <= 12) || (tl.pf == 0 && small) ? 1 : 0;
+        if (tl.split < 0) tl.split = (tl.pf == 1 && p->K <= 12) || small ? 1 : 0;
         if (tl.pf == 1) {
             tl.kc = mx_chunk(p->K);
             tl.nb = 1;
