@@ -264,9 +264,8 @@ int cfk_plan_set_split(cfk_plan plan, int32_t precision, int32_t split);
 /* (split = -1: the default of the selected kernel -- on for the matrix-core MIXED kernel up to K = 12,
  * else off; cfk_plan_get_split reports that value.  A call of at most 1024 distributions also splits
  * rows for both symmetrized precisions (either mixed kernel) unless the plan fixes the split, and up to
- * 256 distributions the
- * symmetrized FP64 kernel takes the whole k3 range in one chunk (K <= 10) unless the plan fixes its
- * tile: with few waves in flight a call lasts as long as its slowest wave.) */
+ * 256 distributions the symmetrized FP64 kernel takes the whole k3 range in one chunk (K <= 10) unless
+ * the plan fixes its tile: with few waves in flight a call lasts as long as its slowest wave.) */
 int cfk_plan_get_split(cfk_plan plan, int32_t precision, int32_t *split_out);
 
 /* Routing-table statistics of a plan, for work models: stats[0] N_G, [1] S (distinct (t,q1) slices),
